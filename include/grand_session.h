@@ -1,0 +1,76 @@
+/*
+ * grand_session.h -- device-resident decoding sessions (libgranddecoder.so).
+ *
+ * No counterpart in the reference, whose five entry points take host buffers
+ * and rebuild every device structure per call (grand_decoder.cu:790-2277).  A
+ * session uploads one code (packed check matrix, signatures, RREF, L_dem,
+ * LLRs, phase masks) once and then decodes batches whose inputs and outputs
+ * already live in HBM, asynchronously on a caller's HIP stream.  The five ABI
+ * entry points are built on it; the benchmark, the serving loop and graph
+ * replay use it directly.
+ *
+ * Kinds: GRAND_KIND_CC (grand_decode_cc), GRAND_KIND_PROB_CC
+ * (grand_decode_prob_cc), GRAND_KIND_CLN, GRAND_KIND_PROB_CLN,
+ * GRAND_KIND_ML_CLN.  For the CSS kinds `H` is Hx and `H2` is Hz, selected by
+ * type_of_error as in the ABI; for the DEM kinds `H` is H_dem and `H2` is L_dem.
+ * `file` is the probability (phase-mask) file or the LLR file.
+ */
+#ifndef GRAND_SESSION_H
+#define GRAND_SESSION_H
+
+#include "grand_decoder.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GRAND_KIND_CC 0
+#define GRAND_KIND_PROB_CC 1
+#define GRAND_KIND_CLN 2
+#define GRAND_KIND_PROB_CLN 3
+#define GRAND_KIND_ML_CLN 4
+
+typedef struct GrandSession GrandSession;
+
+/* Validates like the matching ABI entry point (same messages, same order) and
+ * uploads the code.  Returns NULL on invalid input. */
+GrandSession *grand_session_create(int kind, const int *H, const int *H2, int n, int m, int n_logicos,
+                                   char type_of_error, const GrandDecoderConfig *config, const char *file,
+                                   const GrandDecoderExConfig *ex);
+
+/* Decodes num_shots shots.  d_input: num_shots x m error rows (CSS kinds with
+ * input_is_errors = 1) or num_shots x n syndrome/detector rows; d_logicals:
+ * num_shots x n_logicos (DEM kinds).  d_results: num_shots records.
+ * d_decoded: optional num_shots x ceil(m/32) uint32 words of the decoded
+ * pattern; d_decoded_logicals: optional num_shots x ceil(n_logicos/32) words.
+ * d_times: optional num_shots x 2 device timestamps (claim, completion).
+ * All pointers are device memory; work is queued on `stream` (a hipStream_t,
+ * NULL = the session's own stream) and the call returns without waiting.
+ * Returns 0, or -1 for invalid arguments. */
+int grand_session_decode_device(GrandSession *s, const int *d_input, int input_is_errors, const int *d_logicals,
+                                int num_shots, GrandDecoderResult *d_results, unsigned *d_decoded,
+                                unsigned *d_decoded_logicals, unsigned long long *d_times, void *stream);
+
+/* Error bits raised by the kernels of the last decode on this session
+ * (1 = an input value other than 0/1, 2 = an active set larger than the
+ * supported 1024 positions); synchronizes `stream`. */
+int grand_session_error_bits(GrandSession *s, void *stream);
+
+/* Device wall-clock period in milliseconds per tick (for d_times). */
+double grand_session_ms_per_tick(GrandSession *s);
+
+/* Words per decoded pattern (ceil(m/32)) and per decoded logical row. */
+int grand_session_pattern_words(GrandSession *s);
+int grand_session_logical_words(GrandSession *s);
+
+/* Bytes of device memory the session holds for its code plus the scratch a
+ * batch of num_shots needs (the serving loop admits requests by this). */
+size_t grand_session_footprint(GrandSession *s, int num_shots);
+
+void grand_session_destroy(GrandSession *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRAND_SESSION_H */

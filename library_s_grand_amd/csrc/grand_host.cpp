@@ -1,0 +1,873 @@
+// grand_host.cpp -- host side of libgranddecoder.so: the reference C ABI, the
+// device-resident session API, code packing/caching and the two engines
+// (fused persistent kernel, plain one-launch-per-weight-class loop).
+//
+// Validation order and messages follow each reference driver:
+//   grand_decode_prob_cc  grand_decoder.cu:833-869
+//   grand_decode_cc       grand_decoder.cu:1155-1180
+//   grand_decode_cln      grand_decoder.cu:1440-1446
+//   grand_decode_prob_cln grand_decoder.cu:1713-1746
+//   grand_decode_ML_cln   grand_decoder.cu:2039-2053
+// followed by this library's own limits (n <= 512, m <= 4096, 0/1 matrices,
+// finite LLRs), each reported on stderr with a -1 return.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <list>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "grand_types.h"
+#include "../../include/grand_session.h"
+
+extern "C" {
+size_t grand_fused_lds_bytes(int cap, int ml);
+hipError_t grand_launch_fill(unsigned *p, unsigned v, int n, hipStream_t st);
+hipError_t grand_launch_triage(const grand::DevCode *code, const grand::DevCall *call, int grid, hipStream_t st);
+hipError_t grand_launch_fused(const grand::DevCode *code, const grand::DevCall *call, int grid, hipStream_t st);
+hipError_t grand_launch_expand(const uint32_t *packed, int words, int cols, int rows, int *out, hipStream_t st);
+size_t grand_plain_state_bytes();
+size_t grand_plain_record_bytes();
+hipError_t grand_launch_plain_active(const grand::DevCode *code, const grand::DevCall *call, int shot, int thr,
+                                     int sort_tam, void *scratch, void *state, hipStream_t st);
+hipError_t grand_launch_plain_class(const grand::DevCode *code, const grand::DevCall *call, int shot, int w, int tam,
+                                    int R, unsigned long long nchunks, int grid, int ml, void *scratch, void *state,
+                                    void *records, hipStream_t st);
+hipError_t grand_launch_plain_ml_reduce(void *records, int nrec, void *out, hipStream_t st);
+hipError_t grand_launch_plain_finalize(const grand::DevCode *code, const grand::DevCall *call, int shot,
+                                       const grand::ShotOutcome *o, void *scratch, unsigned long long hit_rank,
+                                       hipStream_t st);
+}
+
+namespace grand {
+namespace {
+
+bool hip_check(hipError_t e, const char *what) {
+    if (e == hipSuccess) return true;
+    fprintf(stderr, "grand: HIP error in %s: %s\n", what, hipGetErrorString(e));
+    return false;
+}
+#define HIPX(expr) do { if (!hip_check((expr), #expr)) return -1; } while (0)
+
+// ------------------------------------------------------------------ runtime
+struct DevMem {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevMem() = default;
+    DevMem(const DevMem &) = delete;
+    DevMem &operator=(const DevMem &) = delete;
+    ~DevMem() { if (p) (void)hipFree(p); }
+    bool ensure(size_t b) {
+        if (b <= bytes) return true;
+        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+        size_t want = std::max<size_t>(b, 256);
+        if (!hip_check(hipMalloc(&p, want), "hipMalloc")) return false;
+        bytes = want;
+        return true;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct Runtime {
+    bool ok = false;
+    int device = 0;
+    int cu_count = 256;
+    double ms_per_tick = 1e-5;
+    hipStream_t stream = nullptr;
+    DevMem nCr;
+    std::vector<unsigned long long> nCr_host;
+    std::recursive_mutex mu;
+};
+
+Runtime &rt() {
+    static Runtime r;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        r.nCr_host.assign((size_t)(kMaxTam + 1) * kNcrStride, 0ull);
+        for (int n = 0; n <= kMaxTam; ++n)
+            for (int k = 0; k < kNcrStride; ++k) {
+                if (k > n) continue;
+                unsigned long long v = 1;
+                for (int i = 1; i <= k; ++i) v = v * (unsigned long long)(n - i + 1) / (unsigned long long)i;
+                r.nCr_host[(size_t)n * kNcrStride + k] = v;
+            }
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
+            fprintf(stderr, "grand: no HIP device available\n");
+            return;
+        }
+        if (!hip_check(hipGetDevice(&r.device), "hipGetDevice")) return;
+        hipDeviceProp_t prop;
+        if (!hip_check(hipGetDeviceProperties(&prop, r.device), "hipGetDeviceProperties")) return;
+        r.cu_count = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, r.device) == hipSuccess && khz > 0)
+            r.ms_per_tick = 1.0 / (double)khz;
+        if (!hip_check(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking), "hipStreamCreate")) return;
+        if (!r.nCr.ensure(r.nCr_host.size() * sizeof(unsigned long long))) return;
+        if (!hip_check(hipMemcpy(r.nCr.p, r.nCr_host.data(), r.nCr_host.size() * sizeof(unsigned long long),
+                                 hipMemcpyHostToDevice), "hipMemcpy nCr")) return;
+        r.ok = true;
+    });
+    return r;
+}
+
+uint64_t ncr_host(int n, int k) {
+    if (n < 0 || k < 0 || k > n || n > kMaxTam || k >= kNcrStride) return 0;
+    return rt().nCr_host[(size_t)n * kNcrStride + k];
+}
+
+// ----------------------------------------------------------------- file readers
+struct Masks {
+    int n = 0;
+    int tam_upper[kMaxMasks];
+    int order[kMaxMasks][3];
+};
+
+// grand_decoder.cu:728-752: up to 20 lines of "%d %d %d %d %f %f %f".
+int read_masks(const char *path, Masks &mk) {
+    mk.n = 0;
+    FILE *f = path ? fopen(path, "r") : nullptr;
+    if (!f) { fprintf(stderr, "Error opening probability file: %s\n", path ? path : "(null)"); return 0; }
+    while (mk.n < kMaxMasks) {
+        int t, a, b, c;
+        float p0, p1, p2;
+        if (fscanf(f, "%d %d %d %d %f %f %f", &t, &a, &b, &c, &p0, &p1, &p2) != 7) break;
+        mk.tam_upper[mk.n] = t;
+        mk.order[mk.n][0] = a; mk.order[mk.n][1] = b; mk.order[mk.n][2] = c;
+        ++mk.n;
+    }
+    fclose(f);
+    return mk.n;
+}
+
+// grand_decoder.cu:695-713: m floats.
+bool read_llr(const char *path, std::vector<float> &llr, int m) {
+    FILE *f = path ? fopen(path, "r") : nullptr;
+    if (!f) { fprintf(stderr, "Error opening LLR file: %s\n", path ? path : "(null)"); return false; }
+    llr.assign(m, 0.0f);
+    for (int i = 0; i < m; ++i)
+        if (fscanf(f, "%f", &llr[i]) != 1) {
+            fprintf(stderr, "Error reading LLR file (expected %d values): %s\n", m, path);
+            fclose(f);
+            return false;
+        }
+    fclose(f);
+    return true;
+}
+
+// ------------------------------------------------------------------- hashing
+struct Hasher {
+    uint64_t h = 0x6a09e667f3bcc908ull;
+    void mix(uint64_t v) {
+        h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+        h *= 0xff51afd7ed558ccdull;
+        h ^= h >> 33;
+    }
+    void ints(const int *p, size_t n) {
+        size_t i = 0;
+        for (; i + 2 <= n; i += 2) mix((uint64_t)(uint32_t)p[i] | ((uint64_t)(uint32_t)p[i + 1] << 32));
+        if (i < n) mix((uint64_t)(uint32_t)p[i]);
+        mix(n);
+    }
+};
+
+bool all_binary(const int *p, size_t n) {
+    int acc = 0;
+    for (size_t i = 0; i < n; ++i) acc |= p[i] & ~1;
+    return acc == 0;
+}
+
+// ------------------------------------------------------------------ code upload
+uint64_t splitmix64(uint64_t &x) {
+    uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+struct CodeEntry {
+    uint64_t key = 0;
+    DevCode dev{};
+    DevMem mem;
+    int family = kCC;
+    int n = 0, m = 0, NL = 0;
+    size_t bytes = 0;
+};
+
+// Packs H (n x m), the RREF of the other CSS matrix (cc) or L (cln), and LLRs.
+std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, const int *other, const int *L, int n,
+                                       int m, int NL, const std::vector<float> *llr) {
+    auto ce = std::make_shared<CodeEntry>();
+    ce->key = key; ce->family = family; ce->n = n; ce->m = m; ce->NL = NL;
+    const int D = (n + 31) / 32, M32 = (m + 31) / 32, NL32 = std::max(1, (NL + 31) / 32);
+
+    std::vector<uint32_t> Hcw((size_t)D * m, 0), Hcm((size_t)m * D, 0), Hrow((size_t)n * M32, 0);
+    std::vector<int> colw(m, 0);
+    for (int r = 0; r < n; ++r)
+        for (int c = 0; c < m; ++c)
+            if (H[(size_t)r * m + c]) {
+                Hcw[(size_t)(r >> 5) * m + c] |= 1u << (r & 31);
+                Hcm[(size_t)c * D + (r >> 5)] |= 1u << (r & 31);
+                Hrow[(size_t)r * M32 + (c >> 5)] |= 1u << (c & 31);
+                ++colw[c];
+            }
+    // grand_decoder.cu:376-416: max column weight, regularity, score offset
+    int max_soma = 0;
+    for (int c = 0; c < m; ++c) max_soma = std::max(max_soma, colw[c]);
+    bool regular = true;
+    for (int c = 1; c < m; ++c) if (colw[c] != colw[0]) { regular = false; break; }
+    std::vector<int> off(m, 0);
+    for (int c = 0; c < m; ++c)
+        off[c] = (colw[c] == 0) ? kNegScore : (regular ? 0 : max_soma - colw[c]);
+
+    // Signatures: distinct nonzero random words per row, XORed per column.
+    std::vector<uint32_t> rowsig(n), sig(m, 0);
+    uint64_t seed = 0x243f6a8885a308d3ull;
+    for (int r = 0; r < n; ++r) {
+        for (;;) {
+            const uint32_t v = (uint32_t)(splitmix64(seed) >> 32);
+            bool dup = (v == 0);
+            for (int q = 0; q < r && !dup; ++q) dup = (rowsig[q] == v);
+            if (!dup) { rowsig[r] = v; break; }
+        }
+    }
+    for (int c = 0; c < m; ++c) {
+        uint32_t v = 0;
+        for (int d = 0; d < D; ++d) {
+            uint32_t w = Hcm[(size_t)c * D + d];
+            while (w) { const int b = __builtin_ctz(w); v ^= rowsig[d * 32 + b]; w &= w - 1; }
+        }
+        sig[c] = v;
+    }
+
+    // RREF of the other CSS matrix over GF(2) (grand_decoder.cu:418-454); the
+    // degenerate test only depends on its row space.
+    std::vector<uint32_t> R;
+    std::vector<int> piv;
+    int rank = 0;
+    if (family == kCC) {
+        std::vector<uint32_t> rows((size_t)n * M32, 0);
+        for (int r = 0; r < n; ++r)
+            for (int c = 0; c < m; ++c)
+                if (other[(size_t)r * m + c]) rows[(size_t)r * M32 + (c >> 5)] |= 1u << (c & 31);
+        int rr = 0;
+        for (int c = 0; c < m && rr < n; ++c) {
+            const int wd = c >> 5;
+            const uint32_t bit = 1u << (c & 31);
+            int p = -1;
+            for (int i = rr; i < n; ++i) if (rows[(size_t)i * M32 + wd] & bit) { p = i; break; }
+            if (p < 0) continue;
+            if (p != rr) for (int j = 0; j < M32; ++j) std::swap(rows[(size_t)rr * M32 + j], rows[(size_t)p * M32 + j]);
+            for (int i = 0; i < n; ++i)
+                if (i != rr && (rows[(size_t)i * M32 + wd] & bit))
+                    for (int j = 0; j < M32; ++j) rows[(size_t)i * M32 + j] ^= rows[(size_t)rr * M32 + j];
+            piv.push_back(c);
+            ++rr;
+        }
+        rank = rr;
+        R.assign(rows.begin(), rows.begin() + (size_t)rank * M32);
+    }
+    std::vector<uint32_t> Lp;
+    if (family == kCLN) {
+        Lp.assign((size_t)m * NL32, 0);
+        for (int c = 0; c < m; ++c)
+            for (int q = 0; q < NL; ++q)
+                if (L[(size_t)c * NL + q]) Lp[(size_t)c * NL32 + (q >> 5)] |= 1u << (q & 31);
+    }
+
+    // One allocation, 256-byte aligned segments.
+    struct Seg { const void *src; size_t bytes; size_t off; };
+    std::vector<Seg> segs;
+    size_t total = 0;
+    auto add = [&](const void *src, size_t bytes) {
+        segs.push_back({src, bytes, total});
+        total += (bytes + 255) & ~(size_t)255;
+        return segs.size() - 1;
+    };
+    const size_t iHcw = add(Hcw.data(), Hcw.size() * 4), iHcm = add(Hcm.data(), Hcm.size() * 4),
+                 iHrow = add(Hrow.data(), Hrow.size() * 4), iSig = add(sig.data(), sig.size() * 4),
+                 iRs = add(rowsig.data(), rowsig.size() * 4), iOff = add(off.data(), off.size() * 4),
+                 iR = add(R.data(), R.size() * 4), iPiv = add(piv.data(), piv.size() * 4),
+                 iL = add(Lp.data(), Lp.size() * 4),
+                 iLlr = add(llr ? llr->data() : nullptr, llr ? llr->size() * 4 : 0);
+    if (!ce->mem.ensure(total)) return nullptr;
+    char *base = ce->mem.as<char>();
+    for (const Seg &s : segs)
+        if (s.bytes && !hip_check(hipMemcpy(base + s.off, s.src, s.bytes, hipMemcpyHostToDevice), "hipMemcpy code"))
+            return nullptr;
+    ce->bytes = total;
+    DevCode &dc = ce->dev;
+    dc.n = n; dc.m = m; dc.D = D; dc.M32 = M32; dc.NL = NL; dc.NL32 = NL32;
+    dc.max_soma = max_soma;
+    dc.rank = rank;
+    dc.Hcw = (const uint32_t *)(base + segs[iHcw].off);
+    dc.Hcm = (const uint32_t *)(base + segs[iHcm].off);
+    dc.Hrow = (const uint32_t *)(base + segs[iHrow].off);
+    dc.sig = (const uint32_t *)(base + segs[iSig].off);
+    dc.rowsig = (const uint32_t *)(base + segs[iRs].off);
+    dc.off = (const int *)(base + segs[iOff].off);
+    dc.R = (const uint32_t *)(base + segs[iR].off);
+    dc.piv = (const int *)(base + segs[iPiv].off);
+    dc.Lp = (const uint32_t *)(base + segs[iL].off);
+    dc.llr = (const float *)(base + segs[iLlr].off);
+    dc.nCr = rt().nCr.as<unsigned long long>();
+    return ce;
+}
+
+// Small LRU of uploaded codes, keyed by content hash.
+std::list<std::shared_ptr<CodeEntry>> g_codes;
+
+std::shared_ptr<CodeEntry> find_code(uint64_t key) {
+    for (auto it = g_codes.begin(); it != g_codes.end(); ++it)
+        if ((*it)->key == key) {
+            auto e = *it;
+            g_codes.erase(it);
+            g_codes.push_front(e);
+            return e;
+        }
+    return nullptr;
+}
+
+void remember_code(const std::shared_ptr<CodeEntry> &e) {
+    g_codes.push_front(e);
+    while (g_codes.size() > 16) g_codes.pop_back();
+}
+
+}  // namespace
+
+// ================================================================= sessions
+struct Workspace {
+    DevMem s_packed, e_packed, lg_packed, s_sig, worklist, counters, dec, declog;
+    int shots = 0;
+    bool ensure(const DevCode &c, int shots_) {
+        const size_t S = (size_t)std::max(shots_, 1);
+        if (!s_packed.ensure(S * c.D * 4) || !e_packed.ensure(S * c.M32 * 4) || !lg_packed.ensure(S * c.NL32 * 4) ||
+            !s_sig.ensure(S * 4) || !worklist.ensure(S * 4) || !counters.ensure(64) || !dec.ensure(S * c.M32 * 4) ||
+            !declog.ensure(S * c.NL32 * 4))
+            return false;
+        shots = shots_;
+        return true;
+    }
+    size_t bytes_for(const DevCode &c, int shots_) const {
+        const size_t S = (size_t)std::max(shots_, 1);
+        return S * 4 * ((size_t)c.D + 2 * c.M32 + 2 * c.NL32 + 2) + 64;
+    }
+};
+
+}  // namespace grand
+
+struct GrandSession {
+    int kind = 0;
+    int family = grand::kCC;
+    int search = grand::kUniform;
+    std::shared_ptr<grand::CodeEntry> code;
+    grand::DevCall call{};
+    GrandDecoderExConfig ex{};
+    grand::Workspace ws;
+    grand::DevMem plain_scratch, plain_state, plain_records, plain_out;
+};
+
+namespace grand {
+namespace {
+
+int kind_family(int kind) { return (kind <= GRAND_KIND_PROB_CC) ? kCC : kCLN; }
+int kind_search(int kind) {
+    switch (kind) {
+        case GRAND_KIND_PROB_CC: case GRAND_KIND_PROB_CLN: return kPhase;
+        case GRAND_KIND_ML_CLN: return kML;
+        default: return kUniform;
+    }
+}
+
+// Autotuned defaults (see tune_* in the Python package); overridable per call.
+struct Tuning { int waves_per_cu = 8; int claim_batch = 1; int rmax = 32; };
+Tuning g_tuning;
+
+// Validates exactly as the reference driver for `kind` would, then applies the
+// library's own limits, and creates a session (code upload cached by content).
+GrandSession *create_session(int kind, const int *H, const int *H2, int n, int m, int NL, char type,
+                             const GrandDecoderConfig *cfg, const char *file, const GrandDecoderExConfig *ex) {
+    if (!cfg) { fprintf(stderr, "grand: config is NULL\n"); return nullptr; }
+    const int family = kind_family(kind), search = kind_search(kind);
+    const int W = cfg->max_error_weight;
+    Masks mk;
+    int num_phases = 1;
+    std::vector<float> llr;
+    const int *Hs = nullptr, *other = nullptr;
+    if (family == kCC) {
+        if (type == 'x' || type == 'X') { Hs = H2; other = H; }          // x errors: Hz searched, Hx other
+        else if (type == 'z' || type == 'Z') { Hs = H; other = H2; }
+        else { fprintf(stderr, "Invalid type_of_error: %c. Must be 'x' or 'z'.\n", type); return nullptr; }
+        if (search == kPhase) {
+            read_masks(file, mk);
+            if (W == 6) num_phases = 2; else if (W == 7) num_phases = 3;
+            else if (W > 0 && W < 6) num_phases = 1;
+            else { fprintf(stderr, "Invalid max_error_weight: %d. Must be 6 or 7.\n", W); return nullptr; }
+            if (mk.n == 0) {   // the reference indexes masks[-1] here (grand_decoder.cu:761)
+                fprintf(stderr, "No phase order masks read from: %s\n", file ? file : "(null)");
+                return nullptr;
+            }
+        } else if (W <= 0 || W > 7) {
+            fprintf(stderr, "Invalid max_error_weight: %d. Must be between 1 and 7.\n", W);
+            return nullptr;
+        }
+    } else {
+        Hs = H;
+        if (W <= 0 || W > 7) {
+            fprintf(stderr, "Invalid max_error_weight: %d. Must be between 1 and 7.\n", W);
+            return nullptr;
+        }
+        if (search == kML && m > 0 && !read_llr(file, llr, m)) return nullptr;
+        if (search == kPhase) {
+            if (read_masks(file, mk) == 0) {
+                fprintf(stderr, "No phase order masks read from: %s\n", file ? file : "(null)");
+                return nullptr;
+            }
+            num_phases = (W == 6) ? 2 : (W == 7) ? 3 : 1;
+        }
+    }
+    // library limits
+    if (n < 1 || n > kMaxWords * 32) { fprintf(stderr, "grand: n = %d outside [1, %d]\n", n, kMaxWords * 32); return nullptr; }
+    if (m < 1 || m > kMaxM) { fprintf(stderr, "grand: m = %d outside [1, %d]\n", m, kMaxM); return nullptr; }
+    if (family == kCLN && (NL < 0 || NL > 2048)) { fprintf(stderr, "grand: n_logicos = %d outside [0, 2048]\n", NL); return nullptr; }
+    if (!Hs || (family == kCC && !other) || (family == kCLN && NL > 0 && !H2)) {
+        fprintf(stderr, "grand: matrix pointer is NULL\n");
+        return nullptr;
+    }
+    for (float v : llr)
+        if (!std::isfinite(v)) { fprintf(stderr, "grand: LLR file holds a non-finite value\n"); return nullptr; }
+    Runtime &R = rt();
+    if (!R.ok) { fprintf(stderr, "grand: no usable GPU\n"); return nullptr; }
+
+    Hasher h;
+    h.mix((uint64_t)family); h.mix((uint64_t)n); h.mix((uint64_t)m); h.mix((uint64_t)NL);
+    h.ints(Hs, (size_t)n * m);
+    bool bin = all_binary(Hs, (size_t)n * m);
+    if (family == kCC) { h.ints(other, (size_t)n * m); bin = bin && all_binary(other, (size_t)n * m); }
+    else if (NL > 0) { h.ints(H2, (size_t)m * NL); bin = bin && all_binary(H2, (size_t)m * NL); }
+    if (!llr.empty()) { h.mix(0x4c4c52ull); h.ints(reinterpret_cast<const int *>(llr.data()), llr.size()); }
+    if (!bin) { fprintf(stderr, "grand: check/logical matrices must hold only 0 and 1\n"); return nullptr; }
+
+    std::shared_ptr<CodeEntry> ce = find_code(h.h);
+    if (!ce) {
+        ce = upload_code(h.h, family, Hs, other, family == kCLN ? H2 : nullptr, n, m, NL, llr.empty() ? nullptr : &llr);
+        if (!ce) return nullptr;
+        remember_code(ce);
+    }
+    auto *s = new GrandSession();
+    s->kind = kind; s->family = family; s->search = search; s->code = ce;
+    if (ex) s->ex = *ex;
+    DevCall &c = s->call;
+    c.family = family; c.kind = search;
+    c.W = std::min(W, kMaxW);
+    c.mode = cfg->mode_of_search; c.thr_h = cfg->threshold_for_hybrid; c.max_size = cfg->max_size_active_positions;
+    c.num_phases = num_phases;
+    c.n_masks = mk.n;
+    for (int i = 0; i < mk.n; ++i) {
+        c.mask_tam_upper[i] = mk.tam_upper[i];
+        for (int k = 0; k < 3; ++k) c.mask_order[i][k] = mk.order[i][k];
+    }
+    int cap = std::min(m, kMaxTam);
+    if (family == kCLN) cap = std::min(cap, std::max(0, cfg->max_size_active_positions));
+    c.cap = cap;
+    c.rmax = g_tuning.rmax;
+    c.claim_batch = (s->ex.claim_batch > 0) ? s->ex.claim_batch : g_tuning.claim_batch;
+    c.ms_per_tick = R.ms_per_tick;
+    return s;
+}
+
+// ------------------------------------------------------------- plain engine
+// Host loop over nonzero shots, one launch per (shot, iteration, weight class),
+// a synchronous read-back after each: the reference driver's structure
+// (grand_decoder.cu:1284-1343) with the deterministic CPU-order search.
+int run_plain(GrandSession *s, hipStream_t st, int num_shots) {
+    const DevCode &code = s->code->dev;
+    DevCall &call = s->call;
+    unsigned nwork = 0;
+    HIPX(hipMemcpyAsync(&nwork, call.counters, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPX(hipStreamSynchronize(st));
+    std::vector<int> work(nwork);
+    if (nwork) HIPX(hipMemcpyAsync(work.data(), call.worklist, sizeof(int) * nwork, hipMemcpyDeviceToHost, st));
+    HIPX(hipStreamSynchronize(st));
+    const int cap = std::max(call.cap, 1);
+    const int kRecords = 4096;
+    if (!s->plain_scratch.ensure((size_t)cap * 6 * 4) || !s->plain_state.ensure(grand_plain_state_bytes()) ||
+        !s->plain_records.ensure((size_t)kRecords * grand_plain_record_bytes()) ||
+        !s->plain_out.ensure(grand_plain_record_bytes()))
+        return -1;
+    struct StateHost { int tam_new; int pad; unsigned long long best; float prefix[kMaxW + 1]; } sh;
+    struct RecHost { unsigned key; int pad; unsigned long long rank; int c[kMaxW]; } rec;
+    std::vector<int> P;
+    const int rmax = call.rmax;
+
+    for (unsigned wi = 0; wi < nwork; ++wi) {
+        const int shot = work[wi];
+        ShotOutcome o{};
+        o.hibrido = -2;
+        for (int l = 0; l < kMaxW; ++l) o.win[l] = -1;
+        unsigned long long hit_rank = 0;
+        int it = 0, tam = 0, tam_old = 0;
+        bool found = false;
+        while (it < code.max_soma && it < 3) {
+            const int thr = code.max_soma - it;
+            ++it;
+            HIPX(grand_launch_plain_active(&code, &call, shot, thr, 0, s->plain_scratch.p, s->plain_state.p, st));
+            HIPX(hipMemcpyAsync(&sh, s->plain_state.p, sizeof(sh), hipMemcpyDeviceToHost, st));
+            HIPX(hipStreamSynchronize(st));
+            tam = sh.tam_new;
+            if (call.family == kCLN) { if (tam > call.max_size) tam = tam_old; else tam_old = tam; }
+            if (tam <= 0) continue;
+            if (tam > call.cap) { fprintf(stderr, "grand: active set of %d positions exceeds %d\n", tam, kMaxTam); return -2; }
+            int br = 0;
+            if ((call.mode == 1 && tam > call.thr_h && tam <= call.max_size) || call.mode == 0) br = 1;
+            else if ((call.mode == 1 && tam <= call.thr_h) || call.mode == -1) br = -1;
+            if (br == 0) continue;
+            o.hibrido = (br > 0) ? (o.hibrido == -1 ? 1 : 0) : -1;
+
+            if (call.kind == kML) {
+                HIPX(grand_launch_plain_active(&code, &call, shot, thr, tam, s->plain_scratch.p, s->plain_state.p, st));
+                HIPX(hipMemcpyAsync(&sh, s->plain_state.p, sizeof(sh), hipMemcpyDeviceToHost, st));
+                P.resize(tam);
+                HIPX(hipMemcpyAsync(P.data(), s->plain_scratch.as<int>() + 3 * cap, sizeof(int) * tam,
+                                    hipMemcpyDeviceToHost, st));
+                HIPX(hipStreamSynchronize(st));
+                bool have = false;
+                float best = 0.0f;
+                const int max_w = std::min(call.W, tam);
+                for (int w = 1; w <= max_w; ++w) {
+                    if (have && sh.prefix[w] >= best) break;
+                    const uint64_t Nw = ncr_host(tam, w);
+                    o.count += Nw;
+                    const uint64_t per = (Nw + 63) / 64;
+                    const int R = (int)std::min<uint64_t>(std::max<uint64_t>(per, 1), (uint64_t)rmax);
+                    const uint64_t nchunks = (Nw + 64ull * R - 1) / (64ull * R);
+                    const int grid = (int)std::min<uint64_t>(nchunks, (uint64_t)kRecords);
+                    HIPX(grand_launch_plain_class(&code, &call, shot, w, tam, R, nchunks, grid, 1, s->plain_scratch.p,
+                                                  s->plain_state.p, s->plain_records.p, st));
+                    HIPX(grand_launch_plain_ml_reduce(s->plain_records.p, grid, s->plain_out.p, st));
+                    HIPX(hipMemcpyAsync(&rec, s->plain_out.p, sizeof(rec), hipMemcpyDeviceToHost, st));
+                    HIPX(hipStreamSynchronize(st));
+                    if (rec.key == 0xFFFFFFFFu) continue;
+                    const uint32_t b = (rec.key & 0x80000000u) ? (rec.key & 0x7FFFFFFFu) : ~rec.key;
+                    float cost;
+                    std::memcpy(&cost, &b, 4);
+                    if (!have || cost < best) {
+                        have = true; best = cost; o.fw = w;
+                        for (int l = 0; l < kMaxW; ++l) o.win[l] = (l < w) ? P[rec.c[l]] : -1;
+                    }
+                }
+                found = have;
+            } else {
+                // weight list as in the fused kernel
+                int wl[2 * kMaxW], nw = 0;
+                if (call.kind == kUniform) { for (int w = 1; w <= call.W; ++w) wl[nw++] = w; }
+                else {
+                    int mi = call.n_masks - 1;
+                    for (int i = 0; i < call.n_masks; ++i) if (tam <= call.mask_tam_upper[i]) { mi = i; break; }
+                    for (int k = 0; k < call.num_phases; ++k) {
+                        const int ph = call.mask_order[mi][k];
+                        if (ph == 0) { for (int w = 1; w <= 5 && w <= tam && w <= kMaxW; ++w) wl[nw++] = w; }
+                        else if (ph == 1) { if (6 <= tam) wl[nw++] = 6; }
+                    }
+                }
+                for (int k = 0; k < nw && !found; ++k) {
+                    const int w = wl[k];
+                    const uint64_t Nw = ncr_host(tam, w);
+                    if (Nw == 0) continue;
+                    const uint64_t per = (Nw + 63) / 64;
+                    const int R = (int)std::min<uint64_t>(std::max<uint64_t>(per, 1), (uint64_t)rmax);
+                    const uint64_t nchunks = (Nw + 64ull * R - 1) / (64ull * R);
+                    const int grid = (int)std::min<uint64_t>(nchunks, (uint64_t)kRecords);
+                    HIPX(grand_launch_plain_class(&code, &call, shot, w, tam, R, nchunks, grid, 0, s->plain_scratch.p,
+                                                  s->plain_state.p, s->plain_records.p, st));
+                    HIPX(hipMemcpyAsync(&sh, s->plain_state.p, sizeof(sh), hipMemcpyDeviceToHost, st));
+                    HIPX(hipStreamSynchronize(st));
+                    if (sh.best != ~0ull) {
+                        found = true;
+                        o.fw = w;
+                        hit_rank = sh.best;
+                        o.count += sh.best + 1;
+                    } else {
+                        o.count += Nw;
+                    }
+                }
+            }
+            if (found) break;
+        }
+        o.found = found ? 1 : 0;
+        o.iteration = it;
+        o.tam = tam;
+        HIPX(grand_launch_plain_finalize(&code, &call, shot, &o, s->plain_scratch.p, hit_rank, st));
+    }
+    (void)num_shots;
+    return 0;
+}
+
+int decode_device(GrandSession *s, const int *d_input, int input_errors, const int *d_logicals, int num_shots,
+                  GrandDecoderResult *d_results, unsigned *d_dec, unsigned *d_declog, unsigned long long *d_times,
+                  hipStream_t st) {
+    Runtime &R = rt();
+    if (num_shots <= 0) return 0;
+    const DevCode &code = s->code->dev;
+    if (!s->ws.ensure(code, num_shots)) return -1;
+    DevCall &c = s->call;
+    c.num_shots = num_shots;
+    c.input_errors = (s->family == kCC) ? (input_errors ? 1 : 0) : 0;
+    c.input = d_input;
+    c.logicals = d_logicals;
+    c.s_packed = s->ws.s_packed.as<uint32_t>();
+    c.e_packed = s->ws.e_packed.as<uint32_t>();
+    c.lg_packed = s->ws.lg_packed.as<uint32_t>();
+    c.s_sig = s->ws.s_sig.as<uint32_t>();
+    c.worklist = s->ws.worklist.as<int>();
+    c.counters = s->ws.counters.as<unsigned>();
+    c.results = d_results;
+    c.dec = d_dec ? d_dec : s->ws.dec.as<uint32_t>();
+    c.declog = d_declog ? d_declog : s->ws.declog.as<uint32_t>();
+    c.times = d_times;
+
+    // worklist length, claim cursor and error bits restart with every batch
+    HIPX(grand_launch_fill(c.counters, 0u, 3, st));
+    const int groups = (num_shots + 63) / 64;
+    const int tgrid = std::max(1, std::min(groups, R.cu_count * 16));
+    HIPX(grand_launch_triage(&code, &c, tgrid, st));
+    if (s->ex.engine == GRAND_ENGINE_PLAIN) {
+        const int rc = run_plain(s, st, num_shots);
+        if (rc == -2) { (void)grand_launch_fill(c.counters + 2, kErrTamOverflow, 1, st); return 0; }
+        return rc;
+    }
+    const int wpc = (s->ex.waves_per_cu > 0) ? s->ex.waves_per_cu : g_tuning.waves_per_cu;
+    const int grid = std::max(1, std::min(R.cu_count * wpc, num_shots));
+    HIPX(grand_launch_fused(&code, &c, grid, st));
+    return 0;
+}
+
+// ----------------------------------------------------------------- ABI path
+struct AbiBuffers {
+    DevMem input, logicals, results, times, dec_int, declog_int, counters_zero;
+};
+AbiBuffers g_abi;
+
+bool print_enabled() {
+    const char *v = getenv("GRAND_PRINT_SHOTS");
+    return v && v[0] == '1';
+}
+
+void print_shot(int family, bool with_errors, int shot, const GrandDecoderResult &r) {
+    if (family == kCC && with_errors)
+        printf("shot %d: found=%d, original_weight=%d, found_weight=%d, degenerate=%d, is_equal=%d, iteration=%d, tam=%d, total_tested_combinations=%llu, hybrid=%d, time_sum_ms=%.5f, total_time_search_ms=%.5f, transfer_time_ms=%.5f, total_to_find_ms=%.5f\n",
+               shot + 1, r.found, r.original_weight, r.found_weight, r.degenerate, r.is_equal, r.iteration, r.tam,
+               r.total_tested_combinations, r.hibrido, r.time_sum_ms, r.total_time_search_ms, r.transfer_time_ms,
+               r.total_to_find_ms);
+    else if (family == kCC)
+        printf("shot %d: found=%d, found_weight=%d, iteration=%d, tam=%d, total_tested_combinations=%llu, hybrid=%d, time_sum_ms=%.5f, total_time_search_ms=%.5f, transfer_time_ms=%.5f, total_to_find_ms=%.5f\n",
+               shot + 1, r.found, r.found_weight, r.iteration, r.tam, r.total_tested_combinations, r.hibrido,
+               r.time_sum_ms, r.total_time_search_ms, r.transfer_time_ms, r.total_to_find_ms);
+    else
+        printf("shot %d: found=%d, found_weight=%d, degenerate=%d, iteration=%d, tam=%d, total_tested_combinations=%llu, hybrid=%d, time_sum_ms=%.5f, total_time_search_ms=%.5f, transfer_time_ms=%.5f, total_to_find_ms=%.5f\n",
+               shot + 1, r.found, r.found_weight, r.degenerate, r.iteration, r.tam, r.total_tested_combinations,
+               r.hibrido, r.time_sum_ms, r.total_time_search_ms, r.transfer_time_ms, r.total_to_find_ms);
+}
+
+int report_error_bits(unsigned bits) {
+    if (bits & kErrNonBinary) fprintf(stderr, "grand: input rows must hold only 0 and 1\n");
+    if (bits & kErrTamOverflow) fprintf(stderr, "grand: an active set exceeds the supported %d positions\n", kMaxTam);
+    return bits ? -1 : 0;
+}
+
+int run_abi(int kind, const int *H, const int *H2, int n, int m, int NL, char type, const GrandDecoderConfig *cfg,
+            const char *file, const GrandDecoderExConfig *ex, const int *input, int input_errors, const int *logicals,
+            int num_shots, GrandDecoderResult *results, int *decoded, int *decoded_logicals) {
+    std::lock_guard<std::recursive_mutex> lock(rt().mu);
+    GrandSession *s = create_session(kind, H, H2, n, m, NL, type, cfg, file, ex);
+    if (!s) return -1;
+    std::unique_ptr<GrandSession> guard(s);
+    if (num_shots <= 0) return 0;
+    const int family = s->family;
+    if (!input || !results || (family == kCLN && NL > 0 && !logicals)) {
+        fprintf(stderr, "grand: input/results/logicals pointer is NULL\n");
+        return -1;
+    }
+    Runtime &R = rt();
+    hipStream_t st = R.stream;
+    const DevCode &code = s->code->dev;
+    const int cols = (family == kCC && input_errors) ? m : n;
+    const size_t S = (size_t)num_shots;
+    AbiBuffers &B = g_abi;
+    if (!B.input.ensure(S * cols * 4) || !B.results.ensure(S * sizeof(GrandDecoderResult)) ||
+        (family == kCLN && !B.logicals.ensure(S * std::max(NL, 1) * 4)))
+        return -1;
+    HIPX(hipMemcpyAsync(B.input.p, input, S * cols * 4, hipMemcpyHostToDevice, st));
+    if (family == kCLN && NL > 0) HIPX(hipMemcpyAsync(B.logicals.p, logicals, S * NL * 4, hipMemcpyHostToDevice, st));
+    const bool plain = s->ex.engine == GRAND_ENGINE_PLAIN;
+    std::vector<double> host_ms;
+    auto t0 = std::chrono::steady_clock::now();
+    if (decode_device(s, B.input.as<int>(), input_errors, B.logicals.as<int>(), num_shots,
+                      B.results.as<GrandDecoderResult>(), nullptr, nullptr, nullptr, st) != 0)
+        return -1;
+    (void)t0;
+    HIPX(hipMemcpyAsync(results, B.results.p, S * sizeof(GrandDecoderResult), hipMemcpyDeviceToHost, st));
+    if (decoded) {
+        if (!B.dec_int.ensure(S * m * 4)) return -1;
+        HIPX(grand_launch_expand(s->call.dec, code.M32, m, num_shots, B.dec_int.as<int>(), st));
+        HIPX(hipMemcpyAsync(decoded, B.dec_int.p, S * m * 4, hipMemcpyDeviceToHost, st));
+    }
+    if (decoded_logicals && family == kCLN && NL > 0) {
+        if (!B.declog_int.ensure(S * NL * 4)) return -1;
+        HIPX(grand_launch_expand(s->call.declog, code.NL32, NL, num_shots, B.declog_int.as<int>(), st));
+        HIPX(hipMemcpyAsync(decoded_logicals, B.declog_int.p, S * NL * 4, hipMemcpyDeviceToHost, st));
+    }
+    unsigned bits = 0;
+    HIPX(hipMemcpyAsync(&bits, s->call.counters + 2, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPX(hipStreamSynchronize(st));
+    (void)plain;
+    if (report_error_bits(bits)) return -1;
+    if (print_enabled())
+        for (int i = 0; i < num_shots; ++i) print_shot(family, family == kCC && input_errors, i, results[i]);
+    return 0;
+}
+
+}  // namespace
+}  // namespace grand
+
+// ====================================================================== C ABI
+extern "C" {
+
+using namespace grand;
+
+int grand_decode_cc_ex(const int *Hx, const int *Hz, int n, int m, int *errors, int *syndromes, int num_shots,
+                       const GrandDecoderConfig *config, GrandDecoderResult *results, char type_of_error,
+                       const GrandDecoderExConfig *ex, int *decoded) {
+    return run_abi(GRAND_KIND_CC, Hx, Hz, n, m, 0, type_of_error, config, nullptr, ex, errors ? errors : syndromes,
+                   errors != nullptr, nullptr, num_shots, results, decoded, nullptr);
+}
+
+int grand_decode_prob_cc_ex(const int *Hx, const int *Hz, int n, int m, int *errors, int *syndromes, int num_shots,
+                            const GrandDecoderConfig *config, GrandDecoderResult *results, char type_of_error,
+                            const char *probability_file, const GrandDecoderExConfig *ex, int *decoded) {
+    return run_abi(GRAND_KIND_PROB_CC, Hx, Hz, n, m, 0, type_of_error, config, probability_file, ex,
+                   errors ? errors : syndromes, errors != nullptr, nullptr, num_shots, results, decoded, nullptr);
+}
+
+int grand_decode_cln_ex(int *H_dem, const int *L_dem, int n, int m, int n_logicos, int *detectors, int *logicals,
+                        int num_shots, const GrandDecoderConfig *config, GrandDecoderResult *results,
+                        const GrandDecoderExConfig *ex, int *decoded, int *decoded_logicals) {
+    return run_abi(GRAND_KIND_CLN, H_dem, L_dem, n, m, n_logicos, 'x', config, nullptr, ex, detectors, 0, logicals,
+                   num_shots, results, decoded, decoded_logicals);
+}
+
+int grand_decode_prob_cln_ex(int *H_dem, const int *L_dem, int n, int m, int n_logicos, int *detectors,
+                             int *logicals, int num_shots, const GrandDecoderConfig *config,
+                             GrandDecoderResult *results, const char *probability_file,
+                             const GrandDecoderExConfig *ex, int *decoded, int *decoded_logicals) {
+    return run_abi(GRAND_KIND_PROB_CLN, H_dem, L_dem, n, m, n_logicos, 'x', config, probability_file, ex, detectors,
+                   0, logicals, num_shots, results, decoded, decoded_logicals);
+}
+
+int grand_decode_ML_cln_ex(int *H_dem, const int *L_dem, int n, int m, int n_logicos, int *detectors,
+                           int *logicals, int num_shots, const GrandDecoderConfig *config,
+                           GrandDecoderResult *results, const char *llr_file, const GrandDecoderExConfig *ex,
+                           int *decoded, int *decoded_logicals) {
+    return run_abi(GRAND_KIND_ML_CLN, H_dem, L_dem, n, m, n_logicos, 'x', config, llr_file, ex, detectors, 0,
+                   logicals, num_shots, results, decoded, decoded_logicals);
+}
+
+int grand_decode_cc(const int *Hx, const int *Hz, int n, int m, int *errors, int *syndromes, int num_shots,
+                    const GrandDecoderConfig *config, GrandDecoderResult *results, char type_of_error) {
+    return grand_decode_cc_ex(Hx, Hz, n, m, errors, syndromes, num_shots, config, results, type_of_error, nullptr,
+                              nullptr);
+}
+
+int grand_decode_prob_cc(const int *Hx, const int *Hz, int n, int m, int *errors, int *syndromes, int num_shots,
+                         const GrandDecoderConfig *config, GrandDecoderResult *results, char type_of_error,
+                         const char *probability_file) {
+    return grand_decode_prob_cc_ex(Hx, Hz, n, m, errors, syndromes, num_shots, config, results, type_of_error,
+                                   probability_file, nullptr, nullptr);
+}
+
+int grand_decode_cln(int *H_dem, const int *L_dem, int n, int m, int n_logicos, int *detectors, int *logicals,
+                     int num_shots, const GrandDecoderConfig *config, GrandDecoderResult *results) {
+    return grand_decode_cln_ex(H_dem, L_dem, n, m, n_logicos, detectors, logicals, num_shots, config, results,
+                               nullptr, nullptr, nullptr);
+}
+
+int grand_decode_prob_cln(int *H_dem, const int *L_dem, int n, int m, int n_logicos, int *detectors, int *logicals,
+                          int num_shots, const GrandDecoderConfig *config, GrandDecoderResult *results,
+                          const char *probability_file) {
+    return grand_decode_prob_cln_ex(H_dem, L_dem, n, m, n_logicos, detectors, logicals, num_shots, config, results,
+                                    probability_file, nullptr, nullptr, nullptr);
+}
+
+int grand_decode_ML_cln(int *H_dem, const int *L_dem, int n, int m, int n_logicos, int *detectors, int *logicals,
+                        int num_shots, const GrandDecoderConfig *config, GrandDecoderResult *results,
+                        const char *llr_file) {
+    return grand_decode_ML_cln_ex(H_dem, L_dem, n, m, n_logicos, detectors, logicals, num_shots, config, results,
+                                  llr_file, nullptr, nullptr, nullptr);
+}
+
+// ----------------------------------------------------------------- sessions
+GrandSession *grand_session_create(int kind, const int *H, const int *H2, int n, int m, int n_logicos,
+                                   char type_of_error, const GrandDecoderConfig *config, const char *file,
+                                   const GrandDecoderExConfig *ex) {
+    if (kind < GRAND_KIND_CC || kind > GRAND_KIND_ML_CLN) { fprintf(stderr, "grand: unknown kind %d\n", kind); return nullptr; }
+    std::lock_guard<std::recursive_mutex> lock(rt().mu);
+    return create_session(kind, H, H2, n, m, n_logicos, type_of_error, config, file, ex);
+}
+
+int grand_session_decode_device(GrandSession *s, const int *d_input, int input_is_errors, const int *d_logicals,
+                                int num_shots, GrandDecoderResult *d_results, unsigned *d_decoded,
+                                unsigned *d_decoded_logicals, unsigned long long *d_times, void *stream) {
+    if (!s || (num_shots > 0 && (!d_input || !d_results))) return -1;
+    if (s->family == kCLN && s->code->NL > 0 && num_shots > 0 && !d_logicals) return -1;
+    hipStream_t st = stream ? (hipStream_t)stream : rt().stream;
+    return decode_device(s, d_input, input_is_errors, d_logicals, num_shots, d_results, d_decoded, d_decoded_logicals,
+                         d_times, st);
+}
+
+int grand_session_error_bits(GrandSession *s, void *stream) {
+    if (!s || !s->call.counters) return 0;
+    hipStream_t st = stream ? (hipStream_t)stream : rt().stream;
+    unsigned bits = 0;
+    if (hipMemcpyAsync(&bits, s->call.counters + 2, sizeof(unsigned), hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
+    if (hipStreamSynchronize(st) != hipSuccess) return -1;
+    return (int)bits;
+}
+
+double grand_session_ms_per_tick(GrandSession *s) { return s ? s->call.ms_per_tick : 0.0; }
+int grand_session_pattern_words(GrandSession *s) { return s ? s->code->dev.M32 : 0; }
+int grand_session_logical_words(GrandSession *s) { return s ? s->code->dev.NL32 : 0; }
+
+size_t grand_session_footprint(GrandSession *s, int num_shots) {
+    if (!s) return 0;
+    return s->code->bytes + s->ws.bytes_for(s->code->dev, num_shots);
+}
+
+void grand_session_destroy(GrandSession *s) {
+    std::lock_guard<std::recursive_mutex> lock(rt().mu);
+    delete s;
+}
+
+// Tuning knobs (written by the autotuner).
+void grand_set_tuning(int waves_per_cu, int claim_batch, int rmax) {
+    if (waves_per_cu > 0) g_tuning.waves_per_cu = waves_per_cu;
+    if (claim_batch > 0) g_tuning.claim_batch = claim_batch;
+    if (rmax > 0) g_tuning.rmax = rmax;
+}
+
+void grand_get_tuning(int *waves_per_cu, int *claim_batch, int *rmax) {
+    if (waves_per_cu) *waves_per_cu = g_tuning.waves_per_cu;
+    if (claim_batch) *claim_batch = g_tuning.claim_batch;
+    if (rmax) *rmax = g_tuning.rmax;
+}
+
+}  // extern "C"

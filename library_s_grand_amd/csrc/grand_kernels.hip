@@ -1,0 +1,940 @@
+// grand_kernels.hip -- gfx950 kernels of the GRAND decoder.
+//
+// The search is GF(2) integer work on 64-wide wavefronts (no MFMA):
+//
+//  * triage_kernel     one wave per shot: packs the int32 input row with ballots,
+//                      forms s = H e when errors are given, writes the result of
+//                      zero-syndrome shots and appends the others to a worklist.
+//  * fused_kernel      persistent: every wave claims worklist shots and runs the
+//                      whole per-shot pipeline of the reference driver (column
+//                      scores, up to three active sets, weight-ordered search,
+//                      post-processing) with wave-local state in LDS.
+//  * plain_*           the one-launch-per-weight-class engine the host drives shot
+//                      by shot, built from the same device functions.
+//
+// Candidate test.  Every column c carries a 32-bit signature sig[c] = M h_c where
+// M is a fixed random GF(2) matrix (32 x n); the map is linear, so a pattern
+// {c_1..c_w} can only satisfy H e = s if sig(s) ^ sig[c_1] ^ ... ^ sig[c_w] == 0.
+// Lanes walk their combinations keeping prefix XORs of signatures (one LDS read
+// and one XOR per pattern); a signature hit is confirmed exactly against the
+// bit-packed columns.  Results are therefore exact; the signature only decides
+// which patterns get the full D-word comparison.
+//
+// Enumeration order and counts follow the reference CPU search
+// (grand_decoder.cu:264-350, :534-666): weight classes in list order, ranks in
+// lexicographic order, the first match wins and the count includes it.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "grand_types.h"
+
+namespace grand {
+
+// ------------------------------------------------------------------ wave helpers
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// Number of set bits of `mask` below this lane.
+__device__ __forceinline__ int lanes_below(uint64_t mask) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint32_t wave_xor(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v ^= (uint32_t)__shfl_xor((int)v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const uint32_t t = (uint32_t)__shfl_xor((int)v, o, 64);
+        v = t < v ? t : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int bcast_i(int v, int src) { return __shfl(v, src, 64); }
+__device__ __forceinline__ uint32_t bcast_u(uint32_t v, int src) { return (uint32_t)__shfl((int)v, src, 64); }
+__device__ __forceinline__ float bcast_f(float v, int src) { return __shfl(v, src, 64); }
+
+__device__ __forceinline__ unsigned long long now_ticks() { return wall_clock64(); }
+
+// fp32 -> u32 key whose unsigned order is the float order; -0 is folded onto +0
+// so that equal costs compare equal, as the reference's `cost < best` does.
+__device__ __forceinline__ uint32_t order_key(float f) {
+    uint32_t b = __float_as_uint(f == 0.0f ? 0.0f : f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__device__ __forceinline__ uint64_t ncr(const unsigned long long *t, int n, int k) {
+    return (n < 0 || k < 0 || k > n) ? 0ull : t[n * kNcrStride + k];
+}
+
+// --------------------------------------------------------------- combinatorics
+// Lexicographic unranking (rank 0 = {0..w-1}); the reference scans linearly
+// (grand_decoder.cu:83-100), here each position is a binary search on the
+// hockey-stick identity  sum_{y=p+1}^{x-1} C(t-1-y, k-1) = C(t-1-p, k) - C(t-x, k).
+template <int W>
+__device__ __forceinline__ void unrank_t(uint64_t r, int tam, const unsigned long long *nCr, int *c) {
+    int prev = -1;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        const int k = W - i;
+        const uint64_t tot = nCr[(tam - 1 - prev) * kNcrStride + k];
+        const uint64_t T = tot - r;
+        int lo = prev + 1, hi = tam - k;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (nCr[(tam - mid) * kNcrStride + k] >= T) lo = mid; else hi = mid - 1;
+        }
+        r -= tot - nCr[(tam - lo) * kNcrStride + k];
+        c[i] = lo;
+        prev = lo;
+    }
+}
+
+// Same with the weight known only at run time (per lane).
+__device__ __forceinline__ void unrank_dyn(uint64_t r, int tam, int w, const unsigned long long *nCr, int *c) {
+    int prev = -1;
+#pragma unroll
+    for (int i = 0; i < kMaxW; ++i) {
+        if (i < w) {
+            const int k = w - i;
+            const uint64_t tot = nCr[(tam - 1 - prev) * kNcrStride + k];
+            const uint64_t T = tot - r;
+            int lo = prev + 1, hi = tam - k;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (nCr[(tam - mid) * kNcrStride + k] >= T) lo = mid; else hi = mid - 1;
+            }
+            r -= tot - nCr[(tam - lo) * kNcrStride + k];
+            c[i] = lo;
+            prev = lo;
+        }
+    }
+}
+
+// ------------------------------------------------------------- wave shot state
+// Per-wave LDS layout (cap entries each): A (active columns), sig of A, and for
+// ML the LLR-sorted copies P / sigP / llrP plus the unsorted llr of A.
+struct WaveLds {
+    int *A;
+    uint32_t *sigA;
+    float *llrA;
+    int *P;
+    uint32_t *sigP;
+    float *llrP;
+    uint32_t *ew;      // packed error words of the shot (triage / finalize), kMaxM / 32
+};
+
+__device__ __forceinline__ WaveLds carve_lds(uint32_t *base, int cap, bool ml) {
+    WaveLds L;
+    L.ew = base;
+    uint32_t *p = base + kMaxM / 32;
+    L.A = (int *)p; p += cap;
+    L.sigA = p; p += cap;
+    if (ml) {
+        L.llrA = (float *)p; p += cap;
+        L.P = (int *)p; p += cap;
+        L.sigP = p; p += cap;
+        L.llrP = (float *)p; p += cap;
+    } else {
+        L.llrA = nullptr; L.P = nullptr; L.sigP = nullptr; L.llrP = nullptr;
+    }
+    return L;
+}
+
+// Exact check of a candidate: s ^ XOR_l column(cols[l]) == 0 over all D words.
+__device__ __forceinline__ bool exact_match(const DevCode &code, const uint32_t *s, const int *cols, int w) {
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < kMaxWords; ++d) {
+        if (d < code.D) {
+            uint32_t x = s[d];
+#pragma unroll
+            for (int l = 0; l < kMaxW; ++l)
+                if (l < w) x ^= code.Hcm[(size_t)cols[l] * code.D + d];
+            ok = ok && (x == 0u);
+        }
+    }
+    return ok;
+}
+
+// Active set for threshold thr: ascending columns with score >= thr, where
+// score[c] = |supp(h_c) & supp(s)| + off[c]  (grand_decoder.cu:352-416, :495-504).
+// Writes the first min(tam, cap) entries; returns the full tam.
+__device__ int build_active(const DevCode &code, const uint32_t *s, int thr, const WaveLds &L, int cap) {
+    const int lane = lane_id();
+    int tam = 0;
+    for (int c0 = 0; c0 < code.m; c0 += 64) {
+        const int c = c0 + lane;
+        int score = kNegScore;
+        if (c < code.m) {
+            int sc = 0;
+#pragma unroll
+            for (int d = 0; d < kMaxWords; ++d)
+                if (d < code.D) sc += __popc(code.Hcw[(size_t)d * code.m + c] & s[d]);
+            score = sc + code.off[c];
+        }
+        const bool on = score >= thr;
+        const uint64_t b = ballot(on);
+        if (on) {
+            const int p = tam + lanes_below(b);
+            if (p < cap) { L.A[p] = c; L.sigA[p] = code.sig[c]; }
+        }
+        tam += __popcll(b);
+    }
+    __syncthreads();
+    return tam;
+}
+
+// Stable sort of A[0..tam) by llr (ties keep active-list order), the order of
+// the reference's insertion sort (grand_decoder.cu:598-610).
+__device__ void sort_by_llr(const DevCode &code, const WaveLds &L, int tam) {
+    const int lane = lane_id();
+    for (int i = lane; i < tam; i += 64) L.llrA[i] = code.llr[L.A[i]];
+    __syncthreads();
+    for (int i0 = 0; i0 < tam; i0 += 64) {
+        const int i = i0 + lane;
+        const float li = (i < tam) ? L.llrA[i] : 0.0f;
+        int rank = 0;
+        for (int j = 0; j < tam; ++j) {
+            const float lj = L.llrA[j];
+            rank += (lj < li) || (lj == li && j < i);
+        }
+        if (i < tam) { L.P[rank] = L.A[i]; L.sigP[rank] = L.sigA[i]; L.llrP[rank] = li; }
+    }
+    __syncthreads();
+}
+
+// --------------------------------------------------------------- lane scanners
+// A lane walks `len` consecutive ranks of one weight-W class starting at rank0.
+// MODE 0: stop at the first exact match, return its offset (or -1).
+// MODE 1 (ML): visit all, keep the first strictly cheapest match (cost, offset).
+template <int W, int MODE>
+__device__ __forceinline__ int scan_range(const DevCode &code, const uint32_t *s, uint32_t t0, uint64_t rank0, int len,
+                                          int tam, const int *act, const uint32_t *sg, const float *llr, int *best_c,
+                                          float &best_cost) {
+    if (len <= 0) return -1;
+    int c[W];
+    unrank_t<W>(rank0, tam, code.nCr, c);
+    uint32_t acc[W];    // acc[l] = t0 ^ sg[c0] ^ ... ^ sg[c_l]
+    uint32_t a = t0;
+#pragma unroll
+    for (int l = 0; l < W - 1; ++l) { a ^= sg[c[l]]; acc[l] = a; }
+    uint32_t pre = a;   // prefix of the first W-1 elements
+    int found = -1;
+    for (int j = 0; j < len; ++j) {
+        const uint32_t x = pre ^ sg[c[W - 1]];
+        if (x == 0u) {
+            int cols[kMaxW];
+#pragma unroll
+            for (int l = 0; l < kMaxW; ++l) cols[l] = (l < W) ? act[c[l]] : 0;
+            if (exact_match(code, s, cols, W)) {
+                if (MODE == 0) { found = j;
+#pragma unroll
+                    for (int l = 0; l < W; ++l) best_c[l] = c[l];
+                    break;
+                }
+                float cost = 0.0f;
+#pragma unroll
+                for (int l = 0; l < W; ++l) cost = __fadd_rn(cost, llr[c[l]]);
+                if (found < 0 || cost < best_cost) {
+                    found = j; best_cost = cost;
+#pragma unroll
+                    for (int l = 0; l < W; ++l) best_c[l] = c[l];
+                }
+            }
+        }
+        if (j + 1 == len) break;
+        // lexicographic successor
+        c[W - 1] += 1;
+        if (c[W - 1] >= tam) {
+            int i = -1;
+#pragma unroll
+            for (int l = 0; l < W - 1; ++l) if (c[l] < tam - W + l) i = l;
+#pragma unroll
+            for (int l = 0; l < W; ++l) {
+                if (l == i) c[l] += 1;
+                else if (l > 0 && l > i) c[l] = c[l - 1] + 1;
+            }
+#pragma unroll
+            for (int l = 0; l < W - 1; ++l)
+                if (l >= i) acc[l] = ((l == 0) ? t0 : acc[l - 1]) ^ sg[c[l]];
+            pre = (W >= 2) ? acc[(W >= 2) ? W - 2 : 0] : t0;
+        }
+    }
+    return found;
+}
+
+template <int MODE>
+__device__ __forceinline__ int scan_range_w(int w, const DevCode &code, const uint32_t *s, uint32_t t0, uint64_t rank0,
+                                            int len, int tam, const int *act, const uint32_t *sg, const float *llr,
+                                            int *best_c, float &best_cost) {
+    switch (w) {
+        case 1: return scan_range<1, MODE>(code, s, t0, rank0, len, tam, act, sg, llr, best_c, best_cost);
+        case 2: return scan_range<2, MODE>(code, s, t0, rank0, len, tam, act, sg, llr, best_c, best_cost);
+        case 3: return scan_range<3, MODE>(code, s, t0, rank0, len, tam, act, sg, llr, best_c, best_cost);
+        case 4: return scan_range<4, MODE>(code, s, t0, rank0, len, tam, act, sg, llr, best_c, best_cost);
+        case 5: return scan_range<5, MODE>(code, s, t0, rank0, len, tam, act, sg, llr, best_c, best_cost);
+        default: return scan_range<6, MODE>(code, s, t0, rank0, len, tam, act, sg, llr, best_c, best_cost);
+    }
+}
+
+// ------------------------------------------------------------ first-hit search
+// Weight list wl[0..nw) searched in order over A[0..tam); on a hit writes the
+// decoded columns to win and returns true; count gets the patterns tested.
+__device__ bool first_hit_wave(const DevCode &code, const DevCall &call, const uint32_t *s, uint32_t t0, int tam,
+                               const int *wl, int nw, const WaveLds &L, int *win, int &fw, unsigned long long &count) {
+    const int lane = lane_id();
+    uint64_t N[kMaxW];
+    uint64_t total = 0;
+#pragma unroll
+    for (int k = 0; k < kMaxW; ++k) {
+        N[k] = (k < nw) ? ncr(code.nCr, tam, wl[k]) : 0ull;
+        total += N[k];
+    }
+    uint64_t pos = 0, class_start = 0;
+    int kc = 0;
+    while (kc < nw && N[kc] == 0) ++kc;
+    const int rmax = call.rmax;
+    while (pos < total) {
+        const uint64_t class_end = class_start + N[kc];
+        const uint64_t rem_class = class_end - pos;
+        int hit_lane = -1, hit_w = 0;
+        uint64_t hit_idx = 0;
+        int c[kMaxW] = {0, 0, 0, 0, 0, 0};
+        if (rem_class <= 128u) {
+            // one pattern per lane, 64 consecutive flattened indices (may span classes)
+            const uint64_t idx = pos + (uint64_t)lane;
+            bool cand = false;
+            int w = 0;
+            if (idx < total) {
+                uint64_t cs = class_start;
+                int k = kc;
+                while (idx >= cs + N[k]) { cs += N[k]; ++k; }
+                w = wl[k];
+                unrank_dyn(idx - cs, tam, w, code.nCr, c);
+                uint32_t x = t0;
+#pragma unroll
+                for (int l = 0; l < kMaxW; ++l) if (l < w) x ^= L.sigA[c[l]];
+                if (x == 0u) {
+                    int cols[kMaxW];
+#pragma unroll
+                    for (int l = 0; l < kMaxW; ++l) cols[l] = (l < w) ? L.A[c[l]] : 0;
+                    cand = exact_match(code, s, cols, w);
+                }
+            }
+            const uint64_t b = ballot(cand);
+            if (b) { hit_lane = __ffsll((long long)b) - 1; hit_idx = pos + (uint64_t)hit_lane; hit_w = bcast_i(w, hit_lane); }
+            pos += 64;
+            if (pos > total) pos = total;
+        } else {
+            uint64_t r64 = rem_class / 64u;
+            const int R = (int)(r64 < (uint64_t)rmax ? r64 : (uint64_t)rmax);   // >= 2 here
+            const uint64_t start = pos + (uint64_t)lane * (uint64_t)R;
+            float dummy = 0.0f;
+            const int w = wl[kc];
+            const int j = scan_range_w<0>(w, code, s, t0, start - class_start, R, tam, L.A, L.sigA, nullptr, c, dummy);
+            const uint64_t b = ballot(j >= 0);
+            if (b) {
+                hit_lane = __ffsll((long long)b) - 1;
+                hit_idx = pos + (uint64_t)hit_lane * (uint64_t)R + (uint64_t)bcast_i(j, hit_lane);
+                hit_w = w;
+            }
+            pos += 64ull * (uint64_t)R;
+        }
+        if (hit_lane >= 0) {
+            fw = hit_w;
+#pragma unroll
+            for (int l = 0; l < kMaxW; ++l) {
+                const int cl = bcast_i(c[l], hit_lane);
+                win[l] = (l < hit_w) ? L.A[cl] : -1;
+            }
+            count += hit_idx + 1u;
+            return true;
+        }
+        while (kc < nw && pos >= class_start + N[kc]) { class_start += N[kc]; ++kc; }
+    }
+    count += total;
+    return false;
+}
+
+// -------------------------------------------------------------------- ML search
+// grand_decoder.cu:590-666 over the LLR-sorted active list P: weights 1..max_w
+// in order, stop before weight w when a best exists and prefix[w] >= best, keep
+// the first strictly cheapest match (fp32 costs summed in pattern order).
+__device__ bool ml_wave(const DevCode &code, const DevCall &call, const uint32_t *s, uint32_t t0, int tam,
+                        const WaveLds &L, int *win, int &fw, unsigned long long &count) {
+    const int lane = lane_id();
+    sort_by_llr(code, L, tam);
+    const int max_w = call.W < tam ? call.W : tam;
+    float prefix[kMaxW + 1];
+    prefix[0] = 0.0f;
+#pragma unroll
+    for (int w = 1; w <= kMaxW; ++w) prefix[w] = (w <= max_w) ? __fadd_rn(prefix[w - 1], L.llrP[w - 1]) : 0.0f;
+
+    bool have = false;
+    float best = 0.0f;
+    for (int w = 1; w <= max_w; ++w) {
+        if (have && prefix[w] >= best) break;
+        const uint64_t Nw = ncr(code.nCr, tam, w);
+        count += Nw;
+        uint64_t pos = 0;
+        while (pos < Nw) {
+            const uint64_t rem = Nw - pos;
+            uint64_t r64 = (rem + 63u) / 64u;
+            const int R = (int)(r64 < (uint64_t)call.rmax ? r64 : (uint64_t)call.rmax);
+            const uint64_t lstart = (uint64_t)lane * (uint64_t)R;
+            const int len = lstart >= rem ? 0 : (int)((rem - lstart) < (uint64_t)R ? (rem - lstart) : (uint64_t)R);
+            int c[kMaxW] = {0, 0, 0, 0, 0, 0};
+            float cost = 0.0f;
+            const int j = scan_range_w<1>(w, code, s, t0, pos + lstart, len, tam, L.P, L.sigP, L.llrP, c, cost);
+            const uint32_t key = (j >= 0) ? order_key(cost) : 0xFFFFFFFFu;
+            const uint32_t kmin = wave_min(key);
+            const uint64_t b = ballot(j >= 0 && key == kmin);
+            if (b) {
+                const int bl = __ffsll((long long)b) - 1;
+                const float cbest = bcast_f(cost, bl);
+                if (!have || cbest < best) {
+                    have = true;
+                    best = cbest;
+                    fw = w;
+#pragma unroll
+                    for (int l = 0; l < kMaxW; ++l) {
+                        const int cl = bcast_i(c[l], bl);
+                        win[l] = (l < w) ? L.P[cl] : -1;
+                    }
+                }
+            }
+            pos += 64ull * (uint64_t)R;
+        }
+    }
+    return have;
+}
+
+// --------------------------------------------------------- per-iteration logic
+// Weight list of one iteration: 1..W (uniform) or the phase lists of the mask
+// chosen for tam (grand_decoder.cu:754-786, :1001-1003).
+__device__ int weight_list(const DevCall &call, int tam, int *wl) {
+    int nw = 0;
+    if (call.kind == kUniform) {
+        for (int w = 1; w <= call.W; ++w) wl[nw++] = w;
+        return nw;
+    }
+    int mi = call.n_masks - 1;
+    for (int i = 0; i < call.n_masks; ++i) if (tam <= call.mask_tam_upper[i]) { mi = i; break; }
+    for (int k = 0; k < call.num_phases; ++k) {
+        const int ph = call.mask_order[mi][k];
+        if (ph == 0) { for (int w = 1; w <= 5 && w <= tam && w <= kMaxW; ++w) wl[nw++] = w; }
+        else if (ph == 1) { if (6 <= tam) wl[nw++] = 6; }
+    }
+    return nw;
+}
+
+// Branch of the reference driver for an iteration with tam > 0:
+// 1 = "GPU" branch, -1 = "CPU" branch, 0 = no search (grand_decoder.cu:1296, :1327).
+__device__ __forceinline__ int branch_of(const DevCall &call, int tam) {
+    if ((call.mode == 1 && tam > call.thr_h && tam <= call.max_size) || call.mode == 0) return 1;
+    if ((call.mode == 1 && tam <= call.thr_h) || call.mode == -1) return -1;
+    return 0;
+}
+
+// ----------------------------------------------------------------- result write
+// The 72-byte record is written as 18 dwords by 18 lanes.
+__device__ void write_result(GrandDecoderResult *dst, int found, int fw, int ow, int deg, int eq, int tam,
+                             uint64_t count, int hib, int it, double t_search, double t_find) {
+    const int lane = lane_id();
+    uint32_t v = 0;
+    const uint64_t ts = (uint64_t)__double_as_longlong(t_search);
+    const uint64_t tf = (uint64_t)__double_as_longlong(t_find);
+    switch (lane) {
+        case 0: v = (uint32_t)found; break;
+        case 1: v = (uint32_t)fw; break;
+        case 2: v = (uint32_t)ow; break;
+        case 3: v = (uint32_t)deg; break;
+        case 4: v = (uint32_t)eq; break;
+        case 5: v = (uint32_t)tam; break;
+        case 6: v = (uint32_t)count; break;
+        case 7: v = (uint32_t)(count >> 32); break;
+        case 8: v = (uint32_t)hib; break;
+        case 9: v = (uint32_t)it; break;
+        case 12: v = (uint32_t)ts; break;
+        case 13: v = (uint32_t)(ts >> 32); break;
+        case 16: v = (uint32_t)tf; break;
+        case 17: v = (uint32_t)(tf >> 32); break;
+        default: v = 0; break;
+    }
+    if (lane < 18) reinterpret_cast<uint32_t *>(dst)[lane] = v;
+}
+
+// Post-processing and result record of a searched shot (grand_decoder.cu:1359-1370
+// for the CSS family, :1623-1648 for the DEM family).
+__device__ void finalize_shot(const DevCode &code, const DevCall &call, int shot, const ShotOutcome &o) {
+    const int lane = lane_id();
+    const int M32 = code.M32;
+    uint32_t dw[2] = {0u, 0u};   // decoded pattern words lane + 64 q, q < 2 (m <= 4096)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int wd = lane + 64 * q;
+        uint32_t v = 0;
+        if (o.found) {
+#pragma unroll
+            for (int l = 0; l < kMaxW; ++l)
+                if (l < o.fw && (o.win[l] >> 5) == wd) v |= 1u << (o.win[l] & 31);
+        }
+        dw[q] = v;
+        if (wd < M32) call.dec[(size_t)shot * M32 + wd] = v;
+    }
+    int ow = 0, eq = 0, deg = 0;
+    if (call.family == kCC && call.input_errors) {
+        uint32_t ew[2] = {0u, 0u};
+        int pc = 0;
+        bool same = true;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int wd = lane + 64 * q;
+            if (wd < M32) ew[q] = call.e_packed[(size_t)shot * M32 + wd];
+            pc += __popc(ew[q]);
+            same = same && (ew[q] == dw[q]);
+        }
+        // original weight = popcount(e) summed across the wave
+        for (int o2 = 32; o2 >= 1; o2 >>= 1) pc += __shfl_xor(pc, o2, 64);
+        ow = pc;
+        if (o.found) {
+            eq = ballot(!same) == 0ull;
+            uint32_t dl[2] = {ew[0] ^ dw[0], ew[1] ^ dw[1]};
+            for (int r = 0; r < code.rank; ++r) {
+                const int p = code.piv[r];
+                const int wd = p >> 5;
+                const uint32_t word = bcast_u((wd >> 6) ? dl[1] : dl[0], wd & 63);
+                if ((word >> (p & 31)) & 1u) {
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        const int w2 = lane + 64 * q;
+                        if (w2 < M32) dl[q] ^= code.R[(size_t)r * M32 + w2];
+                    }
+                }
+            }
+            deg = ballot((dl[0] | dl[1]) != 0u) == 0ull;
+        }
+    } else if (call.family == kCLN) {
+        const int NL32 = code.NL32;
+        bool same = true;
+        for (int wd = lane; wd < NL32; wd += 64) {
+            uint32_t le = 0;
+            if (o.found) {
+#pragma unroll
+                for (int l = 0; l < kMaxW; ++l)
+                    if (l < o.fw) le ^= code.Lp[(size_t)o.win[l] * NL32 + wd];
+            }
+            call.declog[(size_t)shot * NL32 + wd] = le;
+            same = same && (le == call.lg_packed[(size_t)shot * NL32 + wd]);
+        }
+        if (o.found) deg = ballot(!same) == 0ull;
+    }
+    const double t_search = (double)(o.t1 - o.t0) * call.ms_per_tick;
+    write_result(&call.results[shot], o.found, o.found ? o.fw : 0, ow, deg, eq, o.tam, o.count, o.hibrido,
+                 o.iteration, t_search, o.found ? t_search : 0.0);
+    if (call.times && lane == 0) { call.times[2 * (size_t)shot] = o.t0; call.times[2 * (size_t)shot + 1] = o.t1; }
+}
+
+// The whole per-shot pipeline in one wave (grand_decoder.cu:1284-1343 and the
+// equivalent loops of the other four drivers).
+__device__ void decode_shot_wave(const DevCode &code, const DevCall &call, int shot, const WaveLds &L) {
+    ShotOutcome o;
+    o.t0 = now_ticks();
+    uint32_t s[kMaxWords];
+#pragma unroll
+    for (int d = 0; d < kMaxWords; ++d) s[d] = (d < code.D) ? call.s_packed[(size_t)shot * code.D + d] : 0u;
+    const uint32_t t0 = call.s_sig[shot];
+
+    o.found = 0; o.fw = 0; o.count = 0; o.hibrido = -2;
+#pragma unroll
+    for (int l = 0; l < kMaxW; ++l) o.win[l] = -1;
+    int it = 0, tam = 0, tam_old = 0;
+    while (it < code.max_soma && it < 3) {
+        const int thr = code.max_soma - it;
+        ++it;
+        tam = build_active(code, s, thr, L, call.cap);
+        if (call.family == kCLN) { if (tam > call.max_size) tam = tam_old; else tam_old = tam; }
+        if (tam <= 0) continue;
+        if (tam > call.cap) {                      // beyond the supported active-set size
+            if (lane_id() == 0) atomicOr(&call.counters[2], kErrTamOverflow);
+            break;
+        }
+        const int br = branch_of(call, tam);
+        if (br == 0) continue;
+        o.hibrido = (br > 0) ? (o.hibrido == -1 ? 1 : 0) : -1;
+        bool hit;
+        if (call.kind == kML) {
+            hit = ml_wave(code, call, s, t0, tam, L, o.win, o.fw, o.count);
+        } else {
+            int wl[2 * kMaxW];
+            const int nw = weight_list(call, tam, wl);
+            hit = first_hit_wave(code, call, s, t0, tam, wl, nw, L, o.win, o.fw, o.count);
+        }
+        if (hit) { o.found = 1; break; }
+    }
+    o.iteration = it;
+    o.tam = tam;
+    // the decoded columns are reported sorted (the ML winner is in LLR order)
+    if (o.found) {
+#pragma unroll
+        for (int a = 0; a < kMaxW; ++a)
+#pragma unroll
+            for (int b = 0; b < kMaxW - 1 - a; ++b)
+                if (b + 1 < o.fw && o.win[b] > o.win[b + 1]) { const int t = o.win[b]; o.win[b] = o.win[b + 1]; o.win[b + 1] = t; }
+    }
+    o.t1 = now_ticks();
+    finalize_shot(code, call, shot, o);
+}
+
+// ================================================================== kernels
+
+// One wave per shot, 64 shots per group so that nonzero shots are appended to
+// the worklist with one atomic per group.
+__global__ __launch_bounds__(64) void triage_kernel(DevCode code, DevCall call) {
+    extern __shared__ uint32_t lds_raw[];
+    const int lane = lane_id();
+    uint32_t *ew = lds_raw;     // kMaxM / 32 words
+    const int groups = (call.num_shots + 63) / 64;
+    unsigned bad = 0;
+    for (int g = blockIdx.x; g < groups; g += gridDim.x) {
+        const int base = g * 64;
+        const int cnt = (call.num_shots - base) < 64 ? (call.num_shots - base) : 64;
+        bool my_nz = false;
+        for (int j = 0; j < cnt; ++j) {
+            const int shot = base + j;
+            uint32_t sw[kMaxWords];
+#pragma unroll
+            for (int d = 0; d < kMaxWords; ++d) sw[d] = 0u;
+            int ow = 0;
+            uint32_t sg = 0;
+            if (call.input_errors) {
+                const int *row = call.input + (size_t)shot * code.m;
+                for (int c0 = 0; c0 < code.m; c0 += 64) {
+                    const int c = c0 + lane;
+                    const int v = (c < code.m) ? row[c] : 0;
+                    bad |= (unsigned)(v & ~1);
+                    const uint64_t b = ballot(v == 1);
+                    if (lane == 0) {
+                        ew[c0 >> 5] = (uint32_t)b;
+                        if ((c0 >> 5) + 1 < code.M32) ew[(c0 >> 5) + 1] = (uint32_t)(b >> 32);
+                    }
+                    ow += __popcll(b);
+                }
+                __syncthreads();
+                for (int wd = lane; wd < code.M32; wd += 64) call.e_packed[(size_t)shot * code.M32 + wd] = ew[wd];
+                for (int r0 = 0; r0 < code.n; r0 += 64) {
+                    const int r = r0 + lane;
+                    uint32_t par = 0;
+                    if (r < code.n)
+                        for (int wd = 0; wd < code.M32; ++wd) par ^= __popc(code.Hrow[(size_t)r * code.M32 + wd] & ew[wd]);
+                    if (par & 1u) sg ^= code.rowsig[r];
+                    const uint64_t b = ballot(par & 1u);
+#pragma unroll
+                    for (int d = 0; d < kMaxWords; ++d) {
+                        if (d == (r0 >> 5)) sw[d] = (uint32_t)b;
+                        if (d == (r0 >> 5) + 1) sw[d] = (uint32_t)(b >> 32);
+                    }
+                }
+                __syncthreads();
+            } else {
+                const int *row = call.input + (size_t)shot * code.n;
+                for (int r0 = 0; r0 < code.n; r0 += 64) {
+                    const int r = r0 + lane;
+                    const int v = (r < code.n) ? row[r] : 0;
+                    bad |= (unsigned)(v & ~1);
+                    if (v == 1) sg ^= code.rowsig[r];
+                    const uint64_t b = ballot(v == 1);
+#pragma unroll
+                    for (int d = 0; d < kMaxWords; ++d) {
+                        if (d == (r0 >> 5)) sw[d] = (uint32_t)b;
+                        if (d == (r0 >> 5) + 1) sw[d] = (uint32_t)(b >> 32);
+                    }
+                }
+            }
+            if (call.family == kCLN) {
+                const int *lrow = call.logicals + (size_t)shot * code.NL;
+                for (int q0 = 0; q0 < code.NL; q0 += 64) {
+                    const int q = q0 + lane;
+                    const int v = (q < code.NL) ? lrow[q] : 0;
+                    bad |= (unsigned)(v & ~1);
+                    const uint64_t b = ballot(v == 1);
+                    if (lane == 0) {
+                        call.lg_packed[(size_t)shot * code.NL32 + (q0 >> 5)] = (uint32_t)b;
+                        if ((q0 >> 5) + 1 < code.NL32) call.lg_packed[(size_t)shot * code.NL32 + (q0 >> 5) + 1] = (uint32_t)(b >> 32);
+                    }
+                }
+            }
+            int swt = 0;
+#pragma unroll
+            for (int d = 0; d < kMaxWords; ++d) swt += __popc(sw[d]);
+            sg = wave_xor(sg);   // signature of s: XOR of the row signatures of its set bits
+            if (lane < code.D) {
+#pragma unroll
+                for (int d = 0; d < kMaxWords; ++d) if (d == lane) call.s_packed[(size_t)shot * code.D + d] = sw[d];
+            }
+            if (lane == 0) call.s_sig[shot] = sg;
+            const bool nz = swt != 0;
+            if (!nz) {
+                write_result(&call.results[shot], 0, 0, (call.family == kCC) ? ow : 0, 0, 0, 0, 0ull, -2, 0, 0.0, 0.0);
+                for (int wd = lane; wd < code.M32; wd += 64) call.dec[(size_t)shot * code.M32 + wd] = 0u;
+                if (call.family == kCLN)
+                    for (int wd = lane; wd < code.NL32; wd += 64) call.declog[(size_t)shot * code.NL32 + wd] = 0u;
+                if (call.times && lane == 0) { call.times[2 * (size_t)shot] = 0; call.times[2 * (size_t)shot + 1] = 0; }
+            }
+            if (lane == j) my_nz = nz;
+        }
+        const uint64_t mask = ballot(my_nz);
+        if (mask) {
+            unsigned pos = 0;
+            if (lane == 0) pos = atomicAdd(&call.counters[0], (unsigned)__popcll(mask));
+            pos = (unsigned)bcast_i((int)pos, 0);
+            if (my_nz) call.worklist[pos + lanes_below(mask)] = base + lane;
+        }
+    }
+    if (ballot(bad != 0u) && lane == 0) atomicOr(&call.counters[2], kErrNonBinary);
+}
+
+// Persistent search kernel: waves claim worklist entries in batches until none remain.
+__global__ __launch_bounds__(64) void fused_kernel(DevCode code, DevCall call) {
+    extern __shared__ uint32_t lds_raw[];
+    const WaveLds L = carve_lds(lds_raw, call.cap, call.kind == kML);
+    const unsigned total = call.counters[0];
+    const unsigned B = (unsigned)call.claim_batch;
+    for (;;) {
+        unsigned first = 0;
+        if (lane_id() == 0) first = atomicAdd(&call.counters[1], B);
+        first = (unsigned)bcast_i((int)first, 0);
+        if (first >= total) break;
+        const unsigned last = (first + B < total) ? first + B : total;
+        for (unsigned i = first; i < last; ++i) decode_shot_wave(code, call, call.worklist[i], L);
+    }
+}
+
+// ------------------------------------------------------------ plain engine
+// Per-shot state the host loop reads back after every launch.
+struct PlainState {
+    int tam_new;
+    int pad;
+    unsigned long long best;      // first-hit: lowest matching rank in the class
+    float prefix[kMaxW + 1];
+};
+
+// Builds A for threshold thr into the shot's global scratch (and, for ML with
+// sort_tam > 0, the LLR-sorted copy with its prefix sums).
+__global__ __launch_bounds__(64) void plain_active_kernel(DevCode code, DevCall call, int shot, int thr, int sort_tam,
+                                                          int *gA, uint32_t *gsig, float *gllr, int *gP, uint32_t *gsigP,
+                                                          float *gllrP, PlainState *st) {
+    WaveLds L;
+    L.A = gA; L.sigA = gsig; L.llrA = gllr; L.P = gP; L.sigP = gsigP; L.llrP = gllrP; L.ew = nullptr;
+    if (sort_tam > 0) {
+        sort_by_llr(code, L, sort_tam);
+        if (lane_id() == 0) {
+            float p = 0.0f;
+            st->prefix[0] = 0.0f;
+            for (int w = 1; w <= kMaxW; ++w) { if (w <= sort_tam) p = __fadd_rn(p, gllrP[w - 1]); st->prefix[w] = p; }
+        }
+        return;
+    }
+    uint32_t s[kMaxWords];
+#pragma unroll
+    for (int d = 0; d < kMaxWords; ++d) s[d] = (d < code.D) ? call.s_packed[(size_t)shot * code.D + d] : 0u;
+    const int tam = build_active(code, s, thr, L, call.cap);
+    if (lane_id() == 0) { st->tam_new = tam; st->best = ~0ull; }
+}
+
+// One wave per chunk of 64 * R ranks of the weight-w class (grid-stride).
+// First-hit: atomicMin of the matching rank.  ML: per-wave best record.
+struct MlRecord {
+    unsigned key;          // order_key(cost), 0xFFFFFFFF = none
+    int pad;
+    unsigned long long rank;
+    int c[kMaxW];
+};
+
+__global__ __launch_bounds__(64) void plain_class_kernel(DevCode code, DevCall call, int shot, int w, int tam, int R,
+                                                         unsigned long long nchunks, int ml, const int *gA,
+                                                         const uint32_t *gsig, const float *gllr, PlainState *st,
+                                                         MlRecord *rec) {
+    const int lane = lane_id();
+    uint32_t s[kMaxWords];
+#pragma unroll
+    for (int d = 0; d < kMaxWords; ++d) s[d] = (d < code.D) ? call.s_packed[(size_t)shot * code.D + d] : 0u;
+    const uint32_t t0 = call.s_sig[shot];
+    const uint64_t Nw = ncr(code.nCr, tam, w);
+    uint32_t bkey = 0xFFFFFFFFu;
+    uint64_t brank = 0;
+    int bc[kMaxW] = {0, 0, 0, 0, 0, 0};
+    for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        const uint64_t base = ch * 64ull * (uint64_t)R;
+        if (!ml && base > __hip_atomic_load(&st->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        const uint64_t lstart = base + (uint64_t)lane * (uint64_t)R;
+        const int len = lstart >= Nw ? 0 : (int)((Nw - lstart) < (uint64_t)R ? (Nw - lstart) : (uint64_t)R);
+        int c[kMaxW];
+        float cost = 0.0f;
+        if (!ml) {
+            const int j = scan_range_w<0>(w, code, s, t0, lstart, len, tam, gA, gsig, nullptr, c, cost);
+            const uint64_t b = ballot(j >= 0);
+            if (b) {
+                const int hl = __ffsll((long long)b) - 1;
+                const uint64_t r = base + (uint64_t)hl * (uint64_t)R + (uint64_t)bcast_i(j, hl);
+                if (lane == 0) atomicMin(&st->best, (unsigned long long)r);
+                break;
+            }
+        } else {
+            const int j = scan_range_w<1>(w, code, s, t0, lstart, len, tam, gA, gsig, gllr, c, cost);
+            const uint32_t key = (j >= 0) ? order_key(cost) : 0xFFFFFFFFu;
+            const uint32_t kmin = wave_min(key);
+            const uint64_t b = ballot(j >= 0 && key == kmin);
+            if (b && kmin < bkey) {   // chunks visited in rank order: strict keeps the earliest
+                const int bl = __ffsll((long long)b) - 1;
+                bkey = kmin;
+                brank = lstart + (uint64_t)bcast_i(j, bl);
+                brank = (uint64_t)__shfl((long long)brank, bl, 64);
+#pragma unroll
+                for (int l = 0; l < kMaxW; ++l) bc[l] = bcast_i(c[l], bl);
+            }
+        }
+    }
+    if (ml && lane == 0) {
+        MlRecord r;
+        r.key = bkey; r.pad = 0; r.rank = brank;
+#pragma unroll
+        for (int l = 0; l < kMaxW; ++l) r.c[l] = bc[l];
+        rec[blockIdx.x] = r;
+    }
+}
+
+// Reduces per-wave ML records: smallest (key, rank).
+__global__ __launch_bounds__(64) void plain_ml_reduce_kernel(const MlRecord *rec, int nrec, MlRecord *out) {
+    if (lane_id() != 0) return;
+    MlRecord b = rec[0];
+    for (int i = 1; i < nrec; ++i) {
+        const MlRecord r = rec[i];
+        if (r.key < b.key || (r.key == b.key && r.rank < b.rank)) b = r;
+    }
+    *out = b;
+}
+
+// Rebuilds the hit of a first-hit class (rank -> columns) for the finalizer.
+__global__ __launch_bounds__(64) void plain_finalize_kernel(DevCode code, DevCall call, int shot, ShotOutcome o,
+                                                            const int *gA, unsigned long long hit_rank) {
+    if (o.found && o.win[0] < 0) {
+        int c[kMaxW];
+        unrank_dyn(hit_rank, o.tam, o.fw, code.nCr, c);
+#pragma unroll
+        for (int l = 0; l < kMaxW; ++l) o.win[l] = (l < o.fw) ? gA[c[l]] : -1;
+    }
+    if (o.found) {
+#pragma unroll
+        for (int a = 0; a < kMaxW; ++a)
+#pragma unroll
+            for (int b = 0; b < kMaxW - 1 - a; ++b)
+                if (b + 1 < o.fw && o.win[b] > o.win[b + 1]) { const int t = o.win[b]; o.win[b] = o.win[b + 1]; o.win[b + 1] = t; }
+    }
+    finalize_shot(code, call, shot, o);
+}
+
+// Expands packed words to the ABI's int32 0/1 rows.
+__global__ void expand_bits_kernel(const uint32_t *packed, int words, int cols, int rows, int *out) {
+    const size_t total = (size_t)rows * cols;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = i / cols;
+        const int c = (int)(i - r * cols);
+        out[i] = (int)((packed[r * words + (c >> 5)] >> (c & 31)) & 1u);
+    }
+}
+
+__global__ void fill_u32_kernel(unsigned *p, unsigned v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace grand
+
+// ----------------------------------------------------------- launch wrappers
+extern "C" {
+using grand::DevCall;
+using grand::DevCode;
+
+size_t grand_fused_lds_bytes(int cap, int ml) {
+    return sizeof(uint32_t) * ((size_t)grand::kMaxM / 32 + (size_t)cap * (ml ? 6 : 2));
+}
+
+hipError_t grand_launch_fill(unsigned *p, unsigned v, int n, hipStream_t st) {
+    hipLaunchKernelGGL(grand::fill_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t grand_launch_triage(const DevCode *code, const DevCall *call, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(grand::triage_kernel, dim3(grid), dim3(64), sizeof(uint32_t) * (grand::kMaxM / 32), st, *code, *call);
+    return hipGetLastError();
+}
+
+hipError_t grand_launch_fused(const DevCode *code, const DevCall *call, int grid, hipStream_t st) {
+    const size_t lds = grand_fused_lds_bytes(call->cap, call->kind == grand::kML);
+    hipLaunchKernelGGL(grand::fused_kernel, dim3(grid), dim3(64), lds, st, *code, *call);
+    return hipGetLastError();
+}
+
+hipError_t grand_launch_expand(const uint32_t *packed, int words, int cols, int rows, int *out, hipStream_t st) {
+    const size_t total = (size_t)rows * cols;
+    int grid = (int)((total + 255) / 256);
+    if (grid > 65536) grid = 65536;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(grand::expand_bits_kernel, dim3(grid), dim3(256), 0, st, packed, words, cols, rows, out);
+    return hipGetLastError();
+}
+
+size_t grand_plain_state_bytes() { return sizeof(grand::PlainState); }
+size_t grand_plain_record_bytes() { return sizeof(grand::MlRecord); }
+
+hipError_t grand_launch_plain_active(const DevCode *code, const DevCall *call, int shot, int thr, int sort_tam,
+                                     void *scratch, void *state, hipStream_t st) {
+    const int cap = call->cap;
+    int *gA = (int *)scratch;
+    uint32_t *gsig = (uint32_t *)(gA + cap);
+    float *gllr = (float *)(gsig + cap);
+    int *gP = (int *)(gllr + cap);
+    uint32_t *gsigP = (uint32_t *)(gP + cap);
+    float *gllrP = (float *)(gsigP + cap);
+    hipLaunchKernelGGL(grand::plain_active_kernel, dim3(1), dim3(64), 0, st, *code, *call, shot, thr, sort_tam, gA,
+                       gsig, gllr, gP, gsigP, gllrP, (grand::PlainState *)state);
+    return hipGetLastError();
+}
+
+hipError_t grand_launch_plain_class(const DevCode *code, const DevCall *call, int shot, int w, int tam, int R,
+                                    unsigned long long nchunks, int grid, int ml, void *scratch, void *state,
+                                    void *records, hipStream_t st) {
+    const int cap = call->cap;
+    int *gA = (int *)scratch;
+    uint32_t *gsig = (uint32_t *)(gA + cap);
+    float *gllr = (float *)(gsig + cap);
+    int *gP = (int *)(gllr + cap);
+    uint32_t *gsigP = (uint32_t *)(gP + cap);
+    float *gllrP = (float *)(gsigP + cap);
+    hipLaunchKernelGGL(grand::plain_class_kernel, dim3(grid), dim3(64), 0, st, *code, *call, shot, w, tam, R, nchunks,
+                       ml, ml ? gP : gA, ml ? gsigP : gsig, gllrP, (grand::PlainState *)state,
+                       (grand::MlRecord *)records);
+    return hipGetLastError();
+}
+
+hipError_t grand_launch_plain_ml_reduce(void *records, int nrec, void *out, hipStream_t st) {
+    hipLaunchKernelGGL(grand::plain_ml_reduce_kernel, dim3(1), dim3(64), 0, st, (const grand::MlRecord *)records, nrec,
+                       (grand::MlRecord *)out);
+    return hipGetLastError();
+}
+
+hipError_t grand_launch_plain_finalize(const DevCode *code, const DevCall *call, int shot,
+                                       const grand::ShotOutcome *o, void *scratch, unsigned long long hit_rank,
+                                       hipStream_t st) {
+    hipLaunchKernelGGL(grand::plain_finalize_kernel, dim3(1), dim3(64), 0, st, *code, *call, shot, *o,
+                       (const int *)scratch, hit_rank);
+    return hipGetLastError();
+}
+}

@@ -1,0 +1,79 @@
+// grand_types.h -- device-resident code/call descriptors shared by the host
+// driver (grand_host.cpp) and the gfx950 kernels (grand_kernels.hip).
+#pragma once
+#include <stdint.h>
+#include "../../include/grand_decoder.h"
+
+namespace grand {
+
+constexpr int kMaxW = GRAND_MAX_W;   // largest enumerated weight
+constexpr int kMaxTam = 1024;        // largest active set (reference: 300, grand_decoder.cu:13)
+constexpr int kMaxWords = 16;        // 32-bit words per packed column -> n <= 512 checks
+constexpr int kMaxM = 4096;          // columns -> <= 128 packed words per pattern
+constexpr int kMaxMasks = 20;        // phase-order masks (grand_decoder.cu:11)
+constexpr int kNcrStride = 8;        // nCr table row stride (k = 0..7)
+constexpr int kNegScore = -(1 << 20);
+
+enum Family : int { kCC = 0, kCLN = 1 };
+enum Kind : int { kUniform = 0, kPhase = 1, kML = 2 };
+
+// Error bits raised by kernels and turned into a -1 return by the host.
+enum ErrBits : unsigned { kErrNonBinary = 1u, kErrTamOverflow = 2u };
+
+// Immutable per-code device data.  Column bits are kept twice: word-major
+// (Hcw[d * m + c]) for coalesced score computation, column-major
+// (Hcm[c * D + d]) for the exact check of a candidate.
+struct DevCode {
+    int n, m, D, M32, NL, NL32;
+    int max_soma;
+    int rank;                         // rank of the other CSS matrix (cc)
+    const uint32_t *Hcw;              // [D][m]
+    const uint32_t *Hcm;              // [m][D]
+    const uint32_t *Hrow;             // [n][M32]  rows of H, for s = H e
+    const uint32_t *sig;              // [m] 32-bit GF(2)-linear signature of each column
+    const uint32_t *rowsig;           // [n] signature of each unit row vector
+    const int *off;                   // [m] score offset (irregular codes), kNegScore for empty columns
+    const uint32_t *R;                // [rank][M32] RREF of the other CSS matrix (cc)
+    const int *piv;                   // [rank]
+    const uint32_t *Lp;               // [m][NL32] logical flips per mechanism (cln)
+    const float *llr;                 // [m] (ML)
+    const unsigned long long *nCr;    // [(kMaxTam + 1) * kNcrStride]
+};
+
+// Per-call parameters and per-shot buffers.
+struct DevCall {
+    int family, kind;
+    int W;                            // max_error_weight clamped to kMaxW
+    int mode, thr_h, max_size;
+    int num_phases, n_masks;
+    int mask_tam_upper[kMaxMasks];
+    int mask_order[kMaxMasks][3];
+    int cap;                          // active-list capacity held in LDS (<= kMaxTam)
+    int rmax;                         // patterns per lane per chunk in the range path
+    int claim_batch;                  // shots taken per worklist claim
+    int num_shots;
+    int input_errors;                 // 1: input rows are error patterns (cc), 0: syndromes/detectors
+    double ms_per_tick;               // device wall clock period
+    const int *input;                 // [shots][m] or [shots][n]
+    const int *logicals;              // [shots][NL] (cln)
+    uint32_t *s_packed;               // [shots][D]
+    uint32_t *e_packed;               // [shots][M32] (cc with errors)
+    uint32_t *lg_packed;              // [shots][NL32] (cln)
+    uint32_t *s_sig;                  // [shots]
+    int *worklist;                    // nonzero-syndrome shots
+    unsigned *counters;               // [0] worklist length, [1] claim cursor, [2] error bits
+    GrandDecoderResult *results;      // [shots]
+    uint32_t *dec;                    // [shots][M32] decoded pattern
+    uint32_t *declog;                 // [shots][NL32] decoded logicals (cln)
+    unsigned long long *times;        // [shots][2] claim / completion timestamps, or null
+};
+
+// Search outcome of one shot, handed from a search engine to the finalizer.
+struct ShotOutcome {
+    int found, fw, iteration, tam, hibrido;
+    int win[kMaxW];                   // decoded columns (ascending within the pattern order)
+    unsigned long long count;
+    unsigned long long t0, t1;        // device timestamps
+};
+
+}  // namespace grand

@@ -1,0 +1,103 @@
+"""GPU parity: the HIP product against the CPU oracle on seeded workloads of
+every entry point, bit-exact on every result field except the timings, on the
+decoded patterns and on the decoded logicals."""
+import numpy as np
+import pytest
+
+import library_s_grand_amd as G
+import grandtest as T
+
+pytestmark = pytest.mark.gpu
+
+
+def _both(oracle, product, runner, *args, ex=None, **kw):
+    o = runner(oracle, *args, **kw)
+    p = runner(product, *args, ex=ex, **kw)
+    return o, p
+
+
+@pytest.mark.parametrize("use_errors", [True, False])
+def test_cc_bb72_w3(oracle, product, use_errors):
+    hx, hz, e = T.cc_workload(72, 4000, 0.005, seed=1)
+    cfg = G.make_config(max_error_weight=3)
+    syn = T.codes.syndromes(hz, e)
+    args = dict(errors=e if use_errors else None, syndromes=None if use_errors else syn, cfg=cfg, etype="x")
+    o, p = _both(oracle, product, T.run_cc, G.KIND_CC, hx, hz, **args)
+    T.assert_same(o, p, "cc BB72")
+    assert o[1]["found"].sum() > 100
+
+
+def test_cc_z_errors_bb144(oracle, product):
+    hx, hz, e = T.cc_workload(144, 2000, 0.004, seed=2)
+    cfg = G.make_config(max_error_weight=4)
+    o, p = _both(oracle, product, T.run_cc, G.KIND_CC, hx, hz, errors=e, cfg=cfg, etype="z")
+    T.assert_same(o, p, "cc BB144 z")
+
+
+def test_prob_cc_bb144_w4(oracle, product, tmp_path):
+    hx, hz, e = T.cc_workload(144, 6000, "loguniform", seed=3)
+    pf = T.phase_file(tmp_path)
+    cfg = G.make_config(max_error_weight=4)
+    o, p = _both(oracle, product, T.run_cc, G.KIND_PROB_CC, hx, hz, errors=e, cfg=cfg, prob_file=pf)
+    T.assert_same(o, p, "prob_cc BB144")
+
+
+def test_prob_cc_phase_orders(oracle, product, tmp_path):
+    # masks that put weight 6 first for mid-size active sets (W = 6 -> 2 phases)
+    masks = [(5, 0, 1, 2, 1, 0, 0), (15, 1, 0, 2, .3, .7, 0), (300, 0, 1, 2, .6, .4, 0)]
+    pf = T.phase_file(tmp_path, masks)
+    hx, hz, e = T.cc_workload(72, 1500, 0.01, seed=4)
+    cfg = G.make_config(max_error_weight=6)
+    o, p = _both(oracle, product, T.run_cc, G.KIND_PROB_CC, hx, hz, errors=e, cfg=cfg, prob_file=pf)
+    T.assert_same(o, p, "prob_cc phases")
+
+
+@pytest.mark.parametrize("mode,thr", [(0, 0), (1, 8), (1, 40), (7, 0)])
+def test_cc_modes(oracle, product, mode, thr):
+    hx, hz, e = T.cc_workload(72, 1500, 0.008, seed=5)
+    cfg = G.make_config(max_error_weight=4, mode_of_search=mode, threshold_for_hybrid=thr,
+                        max_size_active_positions=30)
+    o, p = _both(oracle, product, T.run_cc, G.KIND_CC, hx, hz, errors=e, cfg=cfg)
+    T.assert_same(o, p, f"cc mode {mode}")
+
+
+def test_cln_dem(oracle, product):
+    H, L, pr, det, lg = T.dem_workload(1500, seed=6)
+    cfg = G.make_config(max_error_weight=3, max_size_active_positions=48)
+    o, p = _both(oracle, product, T.run_cln, G.KIND_CLN, H, L, det, lg, cfg=cfg)
+    T.assert_same(o, p, "cln")
+
+
+def test_prob_cln_dem(oracle, product, tmp_path):
+    H, L, pr, det, lg = T.dem_workload(1500, seed=7)
+    pf = T.phase_file(tmp_path)
+    cfg = G.make_config(max_error_weight=4, max_size_active_positions=40)
+    o, p = _both(oracle, product, T.run_cln, G.KIND_PROB_CLN, H, L, det, lg, cfg=cfg, file=pf)
+    T.assert_same(o, p, "prob_cln")
+
+
+def test_ml_cln_dem(oracle, product, tmp_path):
+    H, L, pr, det, lg = T.dem_workload(600, seed=8)
+    lf = T.llr_file(tmp_path, pr)
+    cfg = G.make_config(max_error_weight=4, max_size_active_positions=28)
+    o, p = _both(oracle, product, T.run_cln, G.KIND_ML_CLN, H, L, det, lg, cfg=cfg, file=lf)
+    T.assert_same(o, p, "ML cln")
+
+
+@pytest.mark.parametrize("kind", [G.KIND_CC, G.KIND_PROB_CC])
+def test_plain_engine_matches(oracle, product, kind, tmp_path):
+    hx, hz, e = T.cc_workload(144, 300, "loguniform", seed=9)
+    pf = T.phase_file(tmp_path)
+    cfg = G.make_config(max_error_weight=4)
+    o = T.run_cc(oracle, kind, hx, hz, errors=e, cfg=cfg, prob_file=pf)
+    p = T.run_cc(product, kind, hx, hz, errors=e, cfg=cfg, prob_file=pf, ex=G.make_ex(engine=G.GRAND_ENGINE_PLAIN))
+    T.assert_same(o, p, "plain cc")
+
+
+def test_plain_engine_ml(oracle, product, tmp_path):
+    H, L, pr, det, lg = T.dem_workload(120, seed=10)
+    lf = T.llr_file(tmp_path, pr)
+    cfg = G.make_config(max_error_weight=4, max_size_active_positions=28)
+    o = T.run_cln(oracle, G.KIND_ML_CLN, H, L, det, lg, cfg=cfg, file=lf)
+    p = T.run_cln(product, G.KIND_ML_CLN, H, L, det, lg, cfg=cfg, file=lf, ex=G.make_ex(engine=G.GRAND_ENGINE_PLAIN))
+    T.assert_same(o, p, "plain ML")
