@@ -43,20 +43,28 @@ GrandSession *grand_session_create(int kind, const int *H, const int *H2, int n,
  * num_shots x n_logicos (DEM kinds).  d_results: num_shots records.
  * d_decoded: optional num_shots x ceil(m/32) uint32 words of the decoded
  * pattern; d_decoded_logicals: optional num_shots x ceil(n_logicos/32) words.
- * d_times: optional num_shots x 2 device timestamps (claim, completion).
+ * d_stats: optional num_shots x 3 uint64: device tick when the shot's search
+ * started, tick when it completed (0, 0 for zero-syndrome shots), and the
+ * weighted pattern count  sum over tested patterns of (weight + 1).
  * All pointers are device memory; work is queued on `stream` (a hipStream_t,
  * NULL = the session's own stream) and the call returns without waiting.
  * Returns 0, or -1 for invalid arguments. */
 int grand_session_decode_device(GrandSession *s, const int *d_input, int input_is_errors, const int *d_logicals,
                                 int num_shots, GrandDecoderResult *d_results, unsigned *d_decoded,
-                                unsigned *d_decoded_logicals, unsigned long long *d_times, void *stream);
+                                unsigned *d_decoded_logicals, unsigned long long *d_stats, void *stream);
 
 /* Error bits raised by the kernels of the last decode on this session
  * (1 = an input value other than 0/1, 2 = an active set larger than the
  * supported 1024 positions); synchronizes `stream`. */
 int grand_session_error_bits(GrandSession *s, void *stream);
 
-/* Device wall-clock period in milliseconds per tick (for d_times). */
+/* HIP-event timing of the kernels of each decode, recorded on the decode's
+ * stream: triage (pack + syndrome + worklist) and search (the persistent
+ * kernel, or the whole plain loop).  last_timing waits for the last decode. */
+int grand_session_enable_timing(GrandSession *s, int on);
+int grand_session_last_timing(GrandSession *s, float *triage_ms, float *search_ms);
+
+/* Device wall-clock period in milliseconds per tick (for d_stats). */
 double grand_session_ms_per_tick(GrandSession *s);
 
 /* Words per decoded pattern (ceil(m/32)) and per decoded logical row. */

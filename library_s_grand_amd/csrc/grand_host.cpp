@@ -375,6 +375,11 @@ struct GrandSession {
     GrandDecoderExConfig ex{};
     grand::Workspace ws;
     grand::DevMem plain_scratch, plain_state, plain_records, plain_out;
+    bool timing = false;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};   // before triage, after triage, after search
+    ~GrandSession() {
+        for (hipEvent_t e : ev) if (e) (void)hipEventDestroy(e);
+    }
 };
 
 namespace grand {
@@ -557,6 +562,7 @@ int run_plain(GrandSession *s, hipStream_t st, int num_shots) {
                     HIPX(grand_launch_plain_ml_reduce(s->plain_records.p, grid, s->plain_out.p, st));
                     HIPX(hipMemcpyAsync(&rec, s->plain_out.p, sizeof(rec), hipMemcpyDeviceToHost, st));
                     HIPX(hipStreamSynchronize(st));
+                    o.wops += Nw * (uint64_t)(w + 1);
                     if (rec.key == 0xFFFFFFFFu) continue;
                     const uint32_t b = (rec.key & 0x80000000u) ? (rec.key & 0x7FFFFFFFu) : ~rec.key;
                     float cost;
@@ -597,8 +603,10 @@ int run_plain(GrandSession *s, hipStream_t st, int num_shots) {
                         o.fw = w;
                         hit_rank = sh.best;
                         o.count += sh.best + 1;
+                        o.wops += (sh.best + 1) * (uint64_t)(w + 1);
                     } else {
                         o.count += Nw;
+                        o.wops += Nw * (uint64_t)(w + 1);
                     }
                 }
             }
@@ -614,7 +622,7 @@ int run_plain(GrandSession *s, hipStream_t st, int num_shots) {
 }
 
 int decode_device(GrandSession *s, const int *d_input, int input_errors, const int *d_logicals, int num_shots,
-                  GrandDecoderResult *d_results, unsigned *d_dec, unsigned *d_declog, unsigned long long *d_times,
+                  GrandDecoderResult *d_results, unsigned *d_dec, unsigned *d_declog, unsigned long long *d_stats,
                   hipStream_t st) {
     Runtime &R = rt();
     if (num_shots <= 0) return 0;
@@ -634,21 +642,25 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     c.results = d_results;
     c.dec = d_dec ? d_dec : s->ws.dec.as<uint32_t>();
     c.declog = d_declog ? d_declog : s->ws.declog.as<uint32_t>();
-    c.times = d_times;
+    c.stats = d_stats;
 
     // worklist length, claim cursor and error bits restart with every batch
     HIPX(grand_launch_fill(c.counters, 0u, 3, st));
     const int groups = (num_shots + 63) / 64;
     const int tgrid = std::max(1, std::min(groups, R.cu_count * 16));
+    if (s->timing) HIPX(hipEventRecord(s->ev[0], st));
     HIPX(grand_launch_triage(&code, &c, tgrid, st));
+    if (s->timing) HIPX(hipEventRecord(s->ev[1], st));
     if (s->ex.engine == GRAND_ENGINE_PLAIN) {
-        const int rc = run_plain(s, st, num_shots);
-        if (rc == -2) { (void)grand_launch_fill(c.counters + 2, kErrTamOverflow, 1, st); return 0; }
+        int rc = run_plain(s, st, num_shots);
+        if (rc == -2) { (void)grand_launch_fill(c.counters + 2, kErrTamOverflow, 1, st); rc = 0; }
+        if (s->timing) HIPX(hipEventRecord(s->ev[2], st));
         return rc;
     }
     const int wpc = (s->ex.waves_per_cu > 0) ? s->ex.waves_per_cu : g_tuning.waves_per_cu;
     const int grid = std::max(1, std::min(R.cu_count * wpc, num_shots));
     HIPX(grand_launch_fused(&code, &c, grid, st));
+    if (s->timing) HIPX(hipEventRecord(s->ev[2], st));
     return 0;
 }
 
@@ -826,12 +838,12 @@ GrandSession *grand_session_create(int kind, const int *H, const int *H2, int n,
 
 int grand_session_decode_device(GrandSession *s, const int *d_input, int input_is_errors, const int *d_logicals,
                                 int num_shots, GrandDecoderResult *d_results, unsigned *d_decoded,
-                                unsigned *d_decoded_logicals, unsigned long long *d_times, void *stream) {
+                                unsigned *d_decoded_logicals, unsigned long long *d_stats, void *stream) {
     if (!s || (num_shots > 0 && (!d_input || !d_results))) return -1;
     if (s->family == kCLN && s->code->NL > 0 && num_shots > 0 && !d_logicals) return -1;
     hipStream_t st = stream ? (hipStream_t)stream : rt().stream;
     return decode_device(s, d_input, input_is_errors, d_logicals, num_shots, d_results, d_decoded, d_decoded_logicals,
-                         d_times, st);
+                         d_stats, st);
 }
 
 int grand_session_error_bits(GrandSession *s, void *stream) {
@@ -841,6 +853,26 @@ int grand_session_error_bits(GrandSession *s, void *stream) {
     if (hipMemcpyAsync(&bits, s->call.counters + 2, sizeof(unsigned), hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
     if (hipStreamSynchronize(st) != hipSuccess) return -1;
     return (int)bits;
+}
+
+int grand_session_enable_timing(GrandSession *s, int on) {
+    if (!s) return -1;
+    if (on && !s->ev[0])
+        for (hipEvent_t &e : s->ev)
+            if (hipEventCreate(&e) != hipSuccess) return -1;
+    s->timing = on != 0;
+    return 0;
+}
+
+int grand_session_last_timing(GrandSession *s, float *triage_ms, float *search_ms) {
+    if (!s || !s->timing) return -1;
+    if (hipEventSynchronize(s->ev[2]) != hipSuccess) return -1;
+    float a = 0.0f, b = 0.0f;
+    if (hipEventElapsedTime(&a, s->ev[0], s->ev[1]) != hipSuccess) return -1;
+    if (hipEventElapsedTime(&b, s->ev[1], s->ev[2]) != hipSuccess) return -1;
+    if (triage_ms) *triage_ms = a;
+    if (search_ms) *search_ms = b;
+    return 0;
 }
 
 double grand_session_ms_per_tick(GrandSession *s) { return s ? s->call.ms_per_tick : 0.0; }
@@ -855,6 +887,18 @@ size_t grand_session_footprint(GrandSession *s, int num_shots) {
 void grand_session_destroy(GrandSession *s) {
     std::lock_guard<std::recursive_mutex> lock(rt().mu);
     delete s;
+}
+
+// Compute units and peak engine clock of the current device (roofline inputs).
+int grand_device_info(int *cus, int *clock_khz, int *wall_clock_khz) {
+    Runtime &R = rt();
+    if (!R.ok) return -1;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, R.device) != hipSuccess) return -1;
+    if (cus) *cus = prop.multiProcessorCount;
+    if (clock_khz) *clock_khz = prop.clockRate;
+    if (wall_clock_khz) *wall_clock_khz = (int)(1.0 / R.ms_per_tick + 0.5);
+    return 0;
 }
 
 // Tuning knobs (written by the autotuner).

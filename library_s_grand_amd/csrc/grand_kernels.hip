@@ -284,10 +284,25 @@ __device__ __forceinline__ int scan_range_w(int w, const DevCode &code, const ui
 }
 
 // ------------------------------------------------------------ first-hit search
+// Sum of (weight + 1) over the first `upto` patterns of the concatenated classes.
+__device__ __forceinline__ uint64_t weighted_prefix(const uint64_t *N, const int *wl, int nw, uint64_t upto) {
+    uint64_t acc = 0, cs = 0;
+#pragma unroll
+    for (int k = 0; k < kMaxW; ++k) {
+        if (k < nw && upto > cs) {
+            const uint64_t take = (upto - cs) < N[k] ? (upto - cs) : N[k];
+            acc += take * (uint64_t)(wl[k] + 1);
+        }
+        if (k < nw) cs += N[k];
+    }
+    return acc;
+}
+
 // Weight list wl[0..nw) searched in order over A[0..tam); on a hit writes the
 // decoded columns to win and returns true; count gets the patterns tested.
 __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, const uint32_t *s, uint32_t t0, int tam,
-                               const int *wl, int nw, const WaveLds &L, int *win, int &fw, unsigned long long &count) {
+                               const int *wl, int nw, const WaveLds &L, int *win, int &fw, unsigned long long &count,
+                               unsigned long long &wops) {
     const int lane = lane_id();
     uint64_t N[kMaxW];
     uint64_t total = 0;
@@ -354,11 +369,13 @@ __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, const u
                 win[l] = (l < hit_w) ? L.A[cl] : -1;
             }
             count += hit_idx + 1u;
+            wops += weighted_prefix(N, wl, nw, hit_idx + 1u);
             return true;
         }
         while (kc < nw && pos >= class_start + N[kc]) { class_start += N[kc]; ++kc; }
     }
     count += total;
+    wops += weighted_prefix(N, wl, nw, total);
     return false;
 }
 
@@ -367,7 +384,7 @@ __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, const u
 // in order, stop before weight w when a best exists and prefix[w] >= best, keep
 // the first strictly cheapest match (fp32 costs summed in pattern order).
 __device__ bool ml_wave(const DevCode &code, const DevCall &call, const uint32_t *s, uint32_t t0, int tam,
-                        const WaveLds &L, int *win, int &fw, unsigned long long &count) {
+                        const WaveLds &L, int *win, int &fw, unsigned long long &count, unsigned long long &wops) {
     const int lane = lane_id();
     sort_by_llr(code, L, tam);
     const int max_w = call.W < tam ? call.W : tam;
@@ -382,6 +399,7 @@ __device__ bool ml_wave(const DevCode &code, const DevCall &call, const uint32_t
         if (have && prefix[w] >= best) break;
         const uint64_t Nw = ncr(code.nCr, tam, w);
         count += Nw;
+        wops += Nw * (uint64_t)(w + 1);
         uint64_t pos = 0;
         while (pos < Nw) {
             const uint64_t rem = Nw - pos;
@@ -538,7 +556,11 @@ __device__ void finalize_shot(const DevCode &code, const DevCall &call, int shot
     const double t_search = (double)(o.t1 - o.t0) * call.ms_per_tick;
     write_result(&call.results[shot], o.found, o.found ? o.fw : 0, ow, deg, eq, o.tam, o.count, o.hibrido,
                  o.iteration, t_search, o.found ? t_search : 0.0);
-    if (call.times && lane == 0) { call.times[2 * (size_t)shot] = o.t0; call.times[2 * (size_t)shot + 1] = o.t1; }
+    if (call.stats && lane == 0) {
+        call.stats[3 * (size_t)shot] = o.t0;
+        call.stats[3 * (size_t)shot + 1] = o.t1;
+        call.stats[3 * (size_t)shot + 2] = o.wops;
+    }
 }
 
 // The whole per-shot pipeline in one wave (grand_decoder.cu:1284-1343 and the
@@ -551,7 +573,7 @@ __device__ void decode_shot_wave(const DevCode &code, const DevCall &call, int s
     for (int d = 0; d < kMaxWords; ++d) s[d] = (d < code.D) ? call.s_packed[(size_t)shot * code.D + d] : 0u;
     const uint32_t t0 = call.s_sig[shot];
 
-    o.found = 0; o.fw = 0; o.count = 0; o.hibrido = -2;
+    o.found = 0; o.fw = 0; o.count = 0; o.wops = 0; o.hibrido = -2;
 #pragma unroll
     for (int l = 0; l < kMaxW; ++l) o.win[l] = -1;
     int it = 0, tam = 0, tam_old = 0;
@@ -570,11 +592,11 @@ __device__ void decode_shot_wave(const DevCode &code, const DevCall &call, int s
         o.hibrido = (br > 0) ? (o.hibrido == -1 ? 1 : 0) : -1;
         bool hit;
         if (call.kind == kML) {
-            hit = ml_wave(code, call, s, t0, tam, L, o.win, o.fw, o.count);
+            hit = ml_wave(code, call, s, t0, tam, L, o.win, o.fw, o.count, o.wops);
         } else {
             int wl[2 * kMaxW];
             const int nw = weight_list(call, tam, wl);
-            hit = first_hit_wave(code, call, s, t0, tam, wl, nw, L, o.win, o.fw, o.count);
+            hit = first_hit_wave(code, call, s, t0, tam, wl, nw, L, o.win, o.fw, o.count, o.wops);
         }
         if (hit) { o.found = 1; break; }
     }
@@ -685,7 +707,11 @@ __global__ __launch_bounds__(64) void triage_kernel(DevCode code, DevCall call) 
                 for (int wd = lane; wd < code.M32; wd += 64) call.dec[(size_t)shot * code.M32 + wd] = 0u;
                 if (call.family == kCLN)
                     for (int wd = lane; wd < code.NL32; wd += 64) call.declog[(size_t)shot * code.NL32 + wd] = 0u;
-                if (call.times && lane == 0) { call.times[2 * (size_t)shot] = 0; call.times[2 * (size_t)shot + 1] = 0; }
+                if (call.stats && lane == 0) {
+                    call.stats[3 * (size_t)shot] = 0;
+                    call.stats[3 * (size_t)shot + 1] = 0;
+                    call.stats[3 * (size_t)shot + 2] = 0;
+                }
             }
             if (lane == j) my_nz = nz;
         }

@@ -65,7 +65,7 @@ struct DevCall {
     GrandDecoderResult *results;      // [shots]
     uint32_t *dec;                    // [shots][M32] decoded pattern
     uint32_t *declog;                 // [shots][NL32] decoded logicals (cln)
-    unsigned long long *times;        // [shots][2] claim / completion timestamps, or null
+    unsigned long long *stats;        // [shots][3] claim tick, completion tick, weighted pattern count; or null
 };
 
 // Search outcome of one shot, handed from a search engine to the finalizer.
@@ -73,6 +73,7 @@ struct ShotOutcome {
     int found, fw, iteration, tam, hibrido;
     int win[kMaxW];                   // decoded columns (ascending within the pattern order)
     unsigned long long count;
+    unsigned long long wops;          // sum over tested patterns of (weight + 1)
     unsigned long long t0, t1;        // device timestamps
 };
 
