@@ -33,15 +33,16 @@ extern "C" {
 size_t grand_fused_lds_bytes(int cap, int ml);
 hipError_t grand_launch_fill(unsigned *p, unsigned v, int n, hipStream_t st);
 hipError_t grand_launch_triage(const grand::DevCode *code, const grand::DevCall *call, int grid, hipStream_t st);
-hipError_t grand_launch_fused(const grand::DevCode *code, const grand::DevCall *call, int grid, hipStream_t st);
+hipError_t grand_launch_fused(const grand::DevCode *code, const grand::DevCall *call, const grand::DevCall *hcall,
+                              int grid, hipStream_t st);
 hipError_t grand_launch_expand(const uint32_t *packed, int words, int cols, int rows, int *out, hipStream_t st);
 size_t grand_plain_state_bytes();
 size_t grand_plain_record_bytes();
-hipError_t grand_launch_plain_active(const grand::DevCode *code, const grand::DevCall *call, int shot, int thr,
-                                     int sort_tam, void *scratch, void *state, hipStream_t st);
-hipError_t grand_launch_plain_class(const grand::DevCode *code, const grand::DevCall *call, int shot, int w, int tam,
-                                    int R, unsigned long long nchunks, int grid, int ml, void *scratch, void *state,
-                                    void *records, hipStream_t st);
+hipError_t grand_launch_plain_active(const grand::DevCode *code, const grand::DevCall *call, int cap, int shot,
+                                     int thr, int sort_tam, void *scratch, void *state, hipStream_t st);
+hipError_t grand_launch_plain_class(const grand::DevCode *code, const grand::DevCall *call, int cap, int shot, int w,
+                                    int tam, int R, unsigned long long nchunks, int grid, int ml, void *scratch,
+                                    void *state, void *records, hipStream_t st);
 hipError_t grand_launch_plain_ml_reduce(void *records, int nrec, void *out, hipStream_t st);
 hipError_t grand_launch_plain_finalize(const grand::DevCode *code, const grand::DevCall *call, int shot,
                                        const grand::ShotOutcome *o, void *scratch, unsigned long long hit_rank,
@@ -199,6 +200,7 @@ struct CodeEntry {
     uint64_t key = 0;
     DevCode dev{};
     DevMem mem;
+    DevMem dcode;                     // device copy of `dev`
     int family = kCC;
     int n = 0, m = 0, NL = 0;
     size_t bytes = 0;
@@ -321,6 +323,9 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
     dc.Lp = (const uint32_t *)(base + segs[iL].off);
     dc.llr = (const float *)(base + segs[iLlr].off);
     dc.nCr = rt().nCr.as<unsigned long long>();
+    if (!ce->dcode.ensure(sizeof(DevCode)) ||
+        !hip_check(hipMemcpy(ce->dcode.p, &dc, sizeof(DevCode), hipMemcpyHostToDevice), "hipMemcpy DevCode"))
+        return nullptr;
     return ce;
 }
 
@@ -347,20 +352,22 @@ void remember_code(const std::shared_ptr<CodeEntry> &e) {
 
 // ================================================================= sessions
 struct Workspace {
-    DevMem s_packed, e_packed, lg_packed, s_sig, worklist, counters, dec, declog;
+    DevMem s_packed, e_packed, lg_packed, s_sig, worklist, counters, dec, declog, tasks, task_arrays;
     int shots = 0;
-    bool ensure(const DevCode &c, int shots_) {
+    bool ensure(const DevCode &c, int shots_, int cap) {
         const size_t S = (size_t)std::max(shots_, 1);
         if (!s_packed.ensure(S * c.D * 4) || !e_packed.ensure(S * c.M32 * 4) || !lg_packed.ensure(S * c.NL32 * 4) ||
             !s_sig.ensure(S * 4) || !worklist.ensure(S * 4) || !counters.ensure(64) || !dec.ensure(S * c.M32 * 4) ||
-            !declog.ensure(S * c.NL32 * 4))
+            !declog.ensure(S * c.NL32 * 4) || !tasks.ensure(sizeof(TaskSlot) * kTaskSlots) ||
+            !task_arrays.ensure((size_t)kTaskSlots * 3 * std::max(cap, 1) * 4))
             return false;
         shots = shots_;
         return true;
     }
-    size_t bytes_for(const DevCode &c, int shots_) const {
+    size_t bytes_for(const DevCode &c, int shots_, int cap) const {
         const size_t S = (size_t)std::max(shots_, 1);
-        return S * 4 * ((size_t)c.D + 2 * c.M32 + 2 * c.NL32 + 2) + 64;
+        return S * 4 * ((size_t)c.D + 2 * c.M32 + 2 * c.NL32 + 2) + 64 + sizeof(TaskSlot) * kTaskSlots +
+               (size_t)kTaskSlots * 3 * std::max(cap, 1) * 4;
     }
 };
 
@@ -375,6 +382,7 @@ struct GrandSession {
     GrandDecoderExConfig ex{};
     grand::Workspace ws;
     grand::DevMem plain_scratch, plain_state, plain_records, plain_out;
+    grand::DevMem dcall;              // device copy of `call`, refreshed before every batch
     bool timing = false;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};   // before triage, after triage, after search
     ~GrandSession() {
@@ -395,7 +403,7 @@ int kind_search(int kind) {
 }
 
 // Autotuned defaults (see tune_* in the Python package); overridable per call.
-struct Tuning { int waves_per_cu = 8; int claim_batch = 1; int rmax = 32; };
+struct Tuning { int waves_per_cu = 8; int claim_batch = 1; int rmax = 32; int split_threshold = 8192; };
 Tuning g_tuning;
 
 // Validates exactly as the reference driver for `kind` would, then applies the
@@ -498,6 +506,8 @@ GrandSession *create_session(int kind, const int *H, const int *H2, int n, int m
 int run_plain(GrandSession *s, hipStream_t st, int num_shots) {
     const DevCode &code = s->code->dev;
     DevCall &call = s->call;
+    const DevCode *dcode = s->code->dcode.as<DevCode>();
+    const DevCall *dcall = s->dcall.as<DevCall>();
     unsigned nwork = 0;
     HIPX(hipMemcpyAsync(&nwork, call.counters, sizeof(unsigned), hipMemcpyDeviceToHost, st));
     HIPX(hipStreamSynchronize(st));
@@ -526,7 +536,7 @@ int run_plain(GrandSession *s, hipStream_t st, int num_shots) {
         while (it < code.max_soma && it < 3) {
             const int thr = code.max_soma - it;
             ++it;
-            HIPX(grand_launch_plain_active(&code, &call, shot, thr, 0, s->plain_scratch.p, s->plain_state.p, st));
+            HIPX(grand_launch_plain_active(dcode, dcall, cap, shot, thr, 0, s->plain_scratch.p, s->plain_state.p, st));
             HIPX(hipMemcpyAsync(&sh, s->plain_state.p, sizeof(sh), hipMemcpyDeviceToHost, st));
             HIPX(hipStreamSynchronize(st));
             tam = sh.tam_new;
@@ -540,7 +550,7 @@ int run_plain(GrandSession *s, hipStream_t st, int num_shots) {
             o.hibrido = (br > 0) ? (o.hibrido == -1 ? 1 : 0) : -1;
 
             if (call.kind == kML) {
-                HIPX(grand_launch_plain_active(&code, &call, shot, thr, tam, s->plain_scratch.p, s->plain_state.p, st));
+                HIPX(grand_launch_plain_active(dcode, dcall, cap, shot, thr, tam, s->plain_scratch.p, s->plain_state.p, st));
                 HIPX(hipMemcpyAsync(&sh, s->plain_state.p, sizeof(sh), hipMemcpyDeviceToHost, st));
                 P.resize(tam);
                 HIPX(hipMemcpyAsync(P.data(), s->plain_scratch.as<int>() + 3 * cap, sizeof(int) * tam,
@@ -557,8 +567,8 @@ int run_plain(GrandSession *s, hipStream_t st, int num_shots) {
                     const int R = (int)std::min<uint64_t>(std::max<uint64_t>(per, 1), (uint64_t)rmax);
                     const uint64_t nchunks = (Nw + 64ull * R - 1) / (64ull * R);
                     const int grid = (int)std::min<uint64_t>(nchunks, (uint64_t)kRecords);
-                    HIPX(grand_launch_plain_class(&code, &call, shot, w, tam, R, nchunks, grid, 1, s->plain_scratch.p,
-                                                  s->plain_state.p, s->plain_records.p, st));
+                    HIPX(grand_launch_plain_class(dcode, dcall, cap, shot, w, tam, R, nchunks, grid, 1,
+                                                  s->plain_scratch.p, s->plain_state.p, s->plain_records.p, st));
                     HIPX(grand_launch_plain_ml_reduce(s->plain_records.p, grid, s->plain_out.p, st));
                     HIPX(hipMemcpyAsync(&rec, s->plain_out.p, sizeof(rec), hipMemcpyDeviceToHost, st));
                     HIPX(hipStreamSynchronize(st));
@@ -594,8 +604,8 @@ int run_plain(GrandSession *s, hipStream_t st, int num_shots) {
                     const int R = (int)std::min<uint64_t>(std::max<uint64_t>(per, 1), (uint64_t)rmax);
                     const uint64_t nchunks = (Nw + 64ull * R - 1) / (64ull * R);
                     const int grid = (int)std::min<uint64_t>(nchunks, (uint64_t)kRecords);
-                    HIPX(grand_launch_plain_class(&code, &call, shot, w, tam, R, nchunks, grid, 0, s->plain_scratch.p,
-                                                  s->plain_state.p, s->plain_records.p, st));
+                    HIPX(grand_launch_plain_class(dcode, dcall, cap, shot, w, tam, R, nchunks, grid, 0,
+                                                  s->plain_scratch.p, s->plain_state.p, s->plain_records.p, st));
                     HIPX(hipMemcpyAsync(&sh, s->plain_state.p, sizeof(sh), hipMemcpyDeviceToHost, st));
                     HIPX(hipStreamSynchronize(st));
                     if (sh.best != ~0ull) {
@@ -615,7 +625,7 @@ int run_plain(GrandSession *s, hipStream_t st, int num_shots) {
         o.found = found ? 1 : 0;
         o.iteration = it;
         o.tam = tam;
-        HIPX(grand_launch_plain_finalize(&code, &call, shot, &o, s->plain_scratch.p, hit_rank, st));
+        HIPX(grand_launch_plain_finalize(dcode, dcall, shot, &o, s->plain_scratch.p, hit_rank, st));
     }
     (void)num_shots;
     return 0;
@@ -627,7 +637,7 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     Runtime &R = rt();
     if (num_shots <= 0) return 0;
     const DevCode &code = s->code->dev;
-    if (!s->ws.ensure(code, num_shots)) return -1;
+    if (!s->ws.ensure(code, num_shots, s->call.cap)) return -1;
     DevCall &c = s->call;
     c.num_shots = num_shots;
     c.input_errors = (s->family == kCC) ? (input_errors ? 1 : 0) : 0;
@@ -643,13 +653,28 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     c.dec = d_dec ? d_dec : s->ws.dec.as<uint32_t>();
     c.declog = d_declog ? d_declog : s->ws.declog.as<uint32_t>();
     c.stats = d_stats;
+    c.tasks = s->ws.tasks.as<TaskSlot>();
+    c.task_arrays = s->ws.task_arrays.as<uint32_t>();
+    c.task_counters = s->ws.counters.as<unsigned>() + 4;
+    {
+        const int st_ex = s->ex.split_threshold;
+        c.split_threshold = st_ex < 0 ? 0ull
+                          : st_ex > 0 ? (unsigned long long)st_ex
+                          : (unsigned long long)g_tuning.split_threshold;
+    }
 
-    // worklist length, claim cursor and error bits restart with every batch
-    HIPX(grand_launch_fill(c.counters, 0u, 3, st));
+    // worklist length, claim cursor, error bits, task counters and the task
+    // table restart with every batch
+    HIPX(grand_launch_fill(c.counters, 0u, 8, st));
+    HIPX(grand_launch_fill(reinterpret_cast<unsigned *>(c.tasks), 0u, (int)(sizeof(TaskSlot) * kTaskSlots / 4), st));
     const int groups = (num_shots + 63) / 64;
     const int tgrid = std::max(1, std::min(groups, R.cu_count * 16));
+    if (!s->dcall.ensure(sizeof(DevCall))) return -1;
+    HIPX(hipMemcpyAsync(s->dcall.p, &c, sizeof(DevCall), hipMemcpyHostToDevice, st));
+    const DevCode *dcode = s->code->dcode.as<DevCode>();
+    const DevCall *dcall = s->dcall.as<DevCall>();
     if (s->timing) HIPX(hipEventRecord(s->ev[0], st));
-    HIPX(grand_launch_triage(&code, &c, tgrid, st));
+    HIPX(grand_launch_triage(dcode, dcall, tgrid, st));
     if (s->timing) HIPX(hipEventRecord(s->ev[1], st));
     if (s->ex.engine == GRAND_ENGINE_PLAIN) {
         int rc = run_plain(s, st, num_shots);
@@ -659,7 +684,7 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     }
     const int wpc = (s->ex.waves_per_cu > 0) ? s->ex.waves_per_cu : g_tuning.waves_per_cu;
     const int grid = std::max(1, std::min(R.cu_count * wpc, num_shots));
-    HIPX(grand_launch_fused(&code, &c, grid, st));
+    HIPX(grand_launch_fused(dcode, dcall, &c, grid, st));
     if (s->timing) HIPX(hipEventRecord(s->ev[2], st));
     return 0;
 }
@@ -881,7 +906,7 @@ int grand_session_logical_words(GrandSession *s) { return s ? s->code->dev.NL32 
 
 size_t grand_session_footprint(GrandSession *s, int num_shots) {
     if (!s) return 0;
-    return s->code->bytes + s->ws.bytes_for(s->code->dev, num_shots);
+    return s->code->bytes + s->ws.bytes_for(s->code->dev, num_shots, s->call.cap);
 }
 
 void grand_session_destroy(GrandSession *s) {
@@ -902,16 +927,18 @@ int grand_device_info(int *cus, int *clock_khz, int *wall_clock_khz) {
 }
 
 // Tuning knobs (written by the autotuner).
-void grand_set_tuning(int waves_per_cu, int claim_batch, int rmax) {
+void grand_set_tuning(int waves_per_cu, int claim_batch, int rmax, int split_threshold) {
     if (waves_per_cu > 0) g_tuning.waves_per_cu = waves_per_cu;
     if (claim_batch > 0) g_tuning.claim_batch = claim_batch;
     if (rmax > 0) g_tuning.rmax = rmax;
+    if (split_threshold > 0) g_tuning.split_threshold = split_threshold;
 }
 
-void grand_get_tuning(int *waves_per_cu, int *claim_batch, int *rmax) {
+void grand_get_tuning(int *waves_per_cu, int *claim_batch, int *rmax, int *split_threshold) {
     if (waves_per_cu) *waves_per_cu = g_tuning.waves_per_cu;
     if (claim_batch) *claim_batch = g_tuning.claim_batch;
     if (rmax) *rmax = g_tuning.rmax;
+    if (split_threshold) *split_threshold = g_tuning.split_threshold;
 }
 
 }  // extern "C"

@@ -127,12 +127,14 @@ struct WaveLds {
     uint32_t *sigP;
     float *llrP;
     uint32_t *ew;      // packed error words of the shot (triage / finalize), kMaxM / 32
+    uint32_t *sw;      // syndrome words of the shot being searched, kMaxWords
 };
 
 __device__ __forceinline__ WaveLds carve_lds(uint32_t *base, int cap, bool ml) {
     WaveLds L;
     L.ew = base;
-    uint32_t *p = base + kMaxM / 32;
+    L.sw = base + kMaxM / 32;
+    uint32_t *p = L.sw + kMaxWords;
     L.A = (int *)p; p += cap;
     L.sigA = p; p += cap;
     if (ml) {
@@ -149,15 +151,13 @@ __device__ __forceinline__ WaveLds carve_lds(uint32_t *base, int cap, bool ml) {
 // Exact check of a candidate: s ^ XOR_l column(cols[l]) == 0 over all D words.
 __device__ __forceinline__ bool exact_match(const DevCode &code, const uint32_t *s, const int *cols, int w) {
     bool ok = true;
+#pragma unroll 1
+    for (int d = 0; d < code.D; ++d) {
+        uint32_t x = s[d];
 #pragma unroll
-    for (int d = 0; d < kMaxWords; ++d) {
-        if (d < code.D) {
-            uint32_t x = s[d];
-#pragma unroll
-            for (int l = 0; l < kMaxW; ++l)
-                if (l < w) x ^= code.Hcm[(size_t)cols[l] * code.D + d];
-            ok = ok && (x == 0u);
-        }
+        for (int l = 0; l < kMaxW; ++l)
+            if (l < w) x ^= code.Hcm[(size_t)cols[l] * code.D + d];
+        ok = ok && (x == 0u);
     }
     return ok;
 }
@@ -173,9 +173,8 @@ __device__ int build_active(const DevCode &code, const uint32_t *s, int thr, con
         int score = kNegScore;
         if (c < code.m) {
             int sc = 0;
-#pragma unroll
-            for (int d = 0; d < kMaxWords; ++d)
-                if (d < code.D) sc += __popc(code.Hcw[(size_t)d * code.m + c] & s[d]);
+#pragma unroll 1
+            for (int d = 0; d < code.D; ++d) sc += __popc(code.Hcw[(size_t)d * code.m + c] & s[d]);
             score = sc + code.off[c];
         }
         const bool on = score >= thr;
@@ -211,13 +210,19 @@ __device__ void sort_by_llr(const DevCode &code, const WaveLds &L, int tam) {
 
 // --------------------------------------------------------------- lane scanners
 // A lane walks `len` consecutive ranks of one weight-W class starting at rank0.
-// MODE 0: stop at the first exact match, return its offset (or -1).
-// MODE 1 (ML): visit all, keep the first strictly cheapest match (cost, offset).
+// MODE 0: stop at the first exact match.  MODE 1 (ML): visit all, keep the
+// first strictly cheapest match.  Returns the offset of the hit within the
+// lane's range (-1 = none) and its fp32 cost.
+struct ScanHit {
+    int j;
+    float cost;
+};
+
 template <int W, int MODE>
-__device__ __forceinline__ int scan_range(const DevCode &code, const uint32_t *s, uint32_t t0, uint64_t rank0, int len,
-                                          int tam, const int *act, const uint32_t *sg, const float *llr, int *best_c,
-                                          float &best_cost) {
-    if (len <= 0) return -1;
+__device__ __forceinline__ ScanHit scan_range(const DevCode &code, const uint32_t *s, uint32_t t0, uint64_t rank0,
+                                              int len, int tam, const int *act, const uint32_t *sg, const float *llr) {
+    ScanHit h{-1, 0.0f};
+    if (len <= 0) return h;
     int c[W];
     unrank_t<W>(rank0, tam, code.nCr, c);
     uint32_t acc[W];    // acc[l] = t0 ^ sg[c0] ^ ... ^ sg[c_l]
@@ -225,7 +230,6 @@ __device__ __forceinline__ int scan_range(const DevCode &code, const uint32_t *s
 #pragma unroll
     for (int l = 0; l < W - 1; ++l) { a ^= sg[c[l]]; acc[l] = a; }
     uint32_t pre = a;   // prefix of the first W-1 elements
-    int found = -1;
     for (int j = 0; j < len; ++j) {
         const uint32_t x = pre ^ sg[c[W - 1]];
         if (x == 0u) {
@@ -233,19 +237,11 @@ __device__ __forceinline__ int scan_range(const DevCode &code, const uint32_t *s
 #pragma unroll
             for (int l = 0; l < kMaxW; ++l) cols[l] = (l < W) ? act[c[l]] : 0;
             if (exact_match(code, s, cols, W)) {
-                if (MODE == 0) { found = j;
-#pragma unroll
-                    for (int l = 0; l < W; ++l) best_c[l] = c[l];
-                    break;
-                }
+                if (MODE == 0) { h.j = j; break; }
                 float cost = 0.0f;
 #pragma unroll
                 for (int l = 0; l < W; ++l) cost = __fadd_rn(cost, llr[c[l]]);
-                if (found < 0 || cost < best_cost) {
-                    found = j; best_cost = cost;
-#pragma unroll
-                    for (int l = 0; l < W; ++l) best_c[l] = c[l];
-                }
+                if (h.j < 0 || cost < h.cost) { h.j = j; h.cost = cost; }
             }
         }
         if (j + 1 == len) break;
@@ -266,21 +262,190 @@ __device__ __forceinline__ int scan_range(const DevCode &code, const uint32_t *s
             pre = (W >= 2) ? acc[(W >= 2) ? W - 2 : 0] : t0;
         }
     }
-    return found;
+    return h;
 }
 
 template <int MODE>
-__device__ __forceinline__ int scan_range_w(int w, const DevCode &code, const uint32_t *s, uint32_t t0, uint64_t rank0,
-                                            int len, int tam, const int *act, const uint32_t *sg, const float *llr,
-                                            int *best_c, float &best_cost) {
+__device__ __attribute__((noinline)) ScanHit scan_range_w(int w, const DevCode &code, const uint32_t *s, uint32_t t0,
+                                                          uint64_t rank0, int len, int tam, const int *act,
+                                                          const uint32_t *sg, const float *llr) {
     switch (w) {
-        case 1: return scan_range<1, MODE>(code, s, t0, rank0, len, tam, act, sg, llr, best_c, best_cost);
-        case 2: return scan_range<2, MODE>(code, s, t0, rank0, len, tam, act, sg, llr, best_c, best_cost);
-        case 3: return scan_range<3, MODE>(code, s, t0, rank0, len, tam, act, sg, llr, best_c, best_cost);
-        case 4: return scan_range<4, MODE>(code, s, t0, rank0, len, tam, act, sg, llr, best_c, best_cost);
-        case 5: return scan_range<5, MODE>(code, s, t0, rank0, len, tam, act, sg, llr, best_c, best_cost);
-        default: return scan_range<6, MODE>(code, s, t0, rank0, len, tam, act, sg, llr, best_c, best_cost);
+        case 1: return scan_range<1, MODE>(code, s, t0, rank0, len, tam, act, sg, llr);
+        case 2: return scan_range<2, MODE>(code, s, t0, rank0, len, tam, act, sg, llr);
+        case 3: return scan_range<3, MODE>(code, s, t0, rank0, len, tam, act, sg, llr);
+        case 4: return scan_range<4, MODE>(code, s, t0, rank0, len, tam, act, sg, llr);
+        case 5: return scan_range<5, MODE>(code, s, t0, rank0, len, tam, act, sg, llr);
+        default: return scan_range<6, MODE>(code, s, t0, rank0, len, tam, act, sg, llr);
     }
+}
+
+// ---------------------------------------------------------------- shared tasks
+// ctl word: generation (20 bits) | nchunks (22 bits) | next chunk (22 bits).
+constexpr unsigned long long kNextMask = (1ull << 22) - 1;
+constexpr unsigned kMaxChunks = (1u << 22) - 1;
+__device__ __forceinline__ unsigned ctl_next(unsigned long long c) { return (unsigned)(c & kNextMask); }
+__device__ __forceinline__ unsigned ctl_nch(unsigned long long c) { return (unsigned)((c >> 22) & kNextMask); }
+__device__ __forceinline__ unsigned ctl_gen(unsigned long long c) { return (unsigned)(c >> 44); }
+
+template <class T>
+__device__ __forceinline__ T ld_relaxed(T *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+template <class T>
+__device__ __forceinline__ T ld_acquire(T *p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
+
+__device__ __forceinline__ float key_to_float(uint32_t key) {
+    const uint32_t b = (key & 0x80000000u) ? (key & 0x7FFFFFFFu) : ~key;
+    return __uint_as_float(b);
+}
+
+// Lane 0 claims the next chunk of slot T; returns it (broadcast) or ~0u.
+__device__ __forceinline__ unsigned claim_chunk(TaskSlot *T, unsigned *open_count, unsigned &gen) {
+    unsigned ch = ~0u, g = 0;
+    if (lane_id() == 0) {
+        unsigned long long snap = ld_acquire(&T->ctl);
+        while (ctl_next(snap) < ctl_nch(snap)) {
+            const unsigned long long prev = atomicCAS(&T->ctl, snap, snap + 1ull);
+            if (prev == snap) {
+                ch = ctl_next(snap);
+                g = ctl_gen(snap);
+                if (ch + 1u == ctl_nch(snap)) atomicSub(open_count, 1u);
+                break;
+            }
+            snap = prev;
+        }
+    }
+    gen = (unsigned)bcast_i((int)g, 0);
+    return (unsigned)bcast_i((int)ch, 0);
+}
+
+// One chunk (64 lanes x R ranks) of a shared class; the class's arrays are in
+// this wave's LDS.  First-hit: atomicMin of the lowest matching rank, chunks
+// past a known hit are skipped.  ML: atomicMin of (cost key << 32 | rank).
+__device__ void task_chunk(const DevCode &code, TaskSlot *T, const uint32_t *s, uint32_t t0, int w, int tam, int R,
+                           uint64_t Nw, bool ml, unsigned ch, const int *act, const uint32_t *sg, const float *llr) {
+    const int lane = lane_id();
+    const uint64_t base = (uint64_t)ch * 64ull * (uint64_t)R;
+    if (!ml && base > ld_relaxed(&T->best)) return;
+    const uint64_t lstart = base + (uint64_t)lane * (uint64_t)R;
+    const int len = lstart >= Nw ? 0 : (int)((Nw - lstart) < (uint64_t)R ? (Nw - lstart) : (uint64_t)R);
+    if (!ml) {
+        const int j = scan_range_w<0>(w, code, s, t0, lstart, len, tam, act, sg, nullptr).j;
+        const uint64_t b = ballot(j >= 0);
+        if (b) {
+            const int hl = __ffsll((long long)b) - 1;
+            const uint64_t r = base + (uint64_t)hl * (uint64_t)R + (uint64_t)bcast_i(j, hl);
+            if (lane == 0) atomicMin(&T->best, (unsigned long long)r);
+        }
+    } else {
+        const ScanHit h = scan_range_w<1>(w, code, s, t0, lstart, len, tam, act, sg, llr);
+        const uint32_t key = (h.j >= 0) ? order_key(h.cost) : 0xFFFFFFFFu;
+        const uint32_t kmin = wave_min(key);
+        const uint64_t b = ballot(h.j >= 0 && key == kmin);
+        if (b) {
+            const int bl = __ffsll((long long)b) - 1;
+            const uint64_t r = (uint64_t)__shfl((long long)(lstart + (uint64_t)(h.j < 0 ? 0 : h.j)), bl, 64);
+            if (lane == 0) atomicMin(&T->best, ((unsigned long long)kmin << 32) | r);
+        }
+    }
+}
+
+// Owner side: publish the weight-w class over act/sg/llr (this wave's LDS) as a
+// shared task, work on it, wait for every chunk and return the class result in
+// best (~0 = no match).  Returns false when no slot is free (caller searches alone).
+__device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, const uint32_t *s, uint32_t t0, int w,
+                           int tam, uint64_t Nw, bool ml, const int *act, const uint32_t *sg, const float *llr,
+                           unsigned long long &best) {
+    const int lane = lane_id();
+    int R = call.rmax;
+    const uint64_t per = 64ull * (uint64_t)kMaxChunks;
+    if (Nw / per >= (uint64_t)R) R = (int)(Nw / per + 1);
+    const unsigned nch = (unsigned)((Nw + 64ull * R - 1) / (64ull * R));
+    int slot = -1;
+    if (lane == 0) {
+        const int start = (int)((blockIdx.x * 131u) & (kTaskSlots - 1));
+        for (int i = 0; i < kTaskSlots; ++i) {
+            const int k = (start + i) & (kTaskSlots - 1);
+            if (ld_relaxed(&call.tasks[k].busy) == 0u && atomicCAS(&call.tasks[k].busy, 0u, 1u) == 0u) { slot = k; break; }
+        }
+    }
+    slot = bcast_i(slot, 0);
+    if (slot < 0) return false;
+    TaskSlot *T = &call.tasks[slot];
+    uint32_t *arr = call.task_arrays + (size_t)slot * 3 * call.cap;
+    for (int i = lane; i < tam; i += 64) {
+        arr[i] = (uint32_t)act[i];
+        arr[call.cap + i] = sg[i];
+        if (ml) arr[2 * call.cap + i] = __float_as_uint(llr[i]);
+    }
+    const unsigned gen = (ctl_gen(ld_relaxed(&T->ctl)) + 1u) & 0xFFFFFu;
+    if (lane == 0) {
+        T->gen = gen; T->nchunks = nch; T->shot = shot; T->w = w; T->tam = tam; T->R = R; T->ml = ml ? 1 : 0;
+        T->t0 = t0; T->nw = Nw; T->done = 0u; T->best = ~0ull;
+#pragma unroll
+        for (int d = 0; d < kMaxWords; ++d) T->s[d] = s[d];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane == 0) {
+        __hip_atomic_exchange(&T->ctl, ((unsigned long long)gen << 44) | ((unsigned long long)nch << 22),
+                              __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        atomicAdd(&call.task_counters[0], 1u);
+    }
+    for (;;) {
+        unsigned g;
+        const unsigned ch = claim_chunk(T, &call.task_counters[0], g);
+        if (ch == ~0u) break;
+        task_chunk(code, T, s, t0, w, tam, R, Nw, ml, ch, act, sg, llr);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (lane == 0) __hip_atomic_fetch_add(&T->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    while (ld_acquire(&T->done) < nch) __builtin_amdgcn_s_sleep(1);
+    best = ld_acquire(&T->best);
+    if (lane == 0) __hip_atomic_store(&T->busy, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+// Helper side: claim one chunk of any open task and process it.  The task's
+// arrays are copied into this wave's LDS once per (slot, generation).
+__device__ bool help_task(const DevCode &code, const DevCall &call, const WaveLds &L, int &cslot, unsigned &cgen) {
+    const int lane = lane_id();
+    if (__builtin_amdgcn_readfirstlane(ld_relaxed(&call.task_counters[0])) == 0u) return false;
+    int slot = -1;
+    const int start = (int)((blockIdx.x * 37u) & (kTaskSlots - 1));
+    for (int b0 = 0; b0 < kTaskSlots && slot < 0; b0 += 64) {
+        const int k = (start + b0 + lane) & (kTaskSlots - 1);
+        const unsigned long long c = ld_relaxed(&call.tasks[k].ctl);
+        const uint64_t b = ballot(ctl_next(c) < ctl_nch(c));
+        if (b) slot = bcast_i(k, __ffsll((long long)b) - 1);
+    }
+    if (slot < 0) return false;
+    TaskSlot *T = &call.tasks[slot];
+    unsigned gen;
+    const unsigned ch = claim_chunk(T, &call.task_counters[0], gen);
+    if (ch == ~0u) return true;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const int w = ld_relaxed(&T->w), tam = ld_relaxed(&T->tam), R = ld_relaxed(&T->R);
+    const bool ml = ld_relaxed(&T->ml) != 0;
+    const uint64_t Nw = ld_relaxed(&T->nw);
+    const uint32_t t0 = ld_relaxed(&T->t0);
+    if (lane < kMaxWords) L.sw[lane] = ld_relaxed(&T->s[lane]);
+    __syncthreads();
+    const uint32_t *s = L.sw;
+    int *act = ml ? L.P : L.A;
+    uint32_t *sg = ml ? L.sigP : L.sigA;
+    if (cslot != slot || cgen != gen) {
+        const uint32_t *arr = call.task_arrays + (size_t)slot * 3 * call.cap;
+        for (int i = lane; i < tam; i += 64) {
+            act[i] = (int)arr[i];
+            sg[i] = arr[call.cap + i];
+            if (ml) L.llrP[i] = __uint_as_float(arr[2 * call.cap + i]);
+        }
+        __syncthreads();
+        cslot = slot;
+        cgen = gen;
+    }
+    task_chunk(code, T, s, t0, w, tam, R, Nw, ml, ch, act, sg, ml ? L.llrP : nullptr);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane == 0) __hip_atomic_fetch_add(&T->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
 }
 
 // ------------------------------------------------------------ first-hit search
@@ -300,9 +465,9 @@ __device__ __forceinline__ uint64_t weighted_prefix(const uint64_t *N, const int
 
 // Weight list wl[0..nw) searched in order over A[0..tam); on a hit writes the
 // decoded columns to win and returns true; count gets the patterns tested.
-__device__ bool first_hit_wave(const DevCode &code, const DevCall &call, const uint32_t *s, uint32_t t0, int tam,
-                               const int *wl, int nw, const WaveLds &L, int *win, int &fw, unsigned long long &count,
-                               unsigned long long &wops) {
+__device__ bool first_hit_wave(const DevCode &code, const DevCall &call, int shot, const uint32_t *s, uint32_t t0,
+                               int tam, const int *wl, int nw, const WaveLds &L, int *win, int &fw,
+                               unsigned long long &count, unsigned long long &wops) {
     const int lane = lane_id();
     uint64_t N[kMaxW];
     uint64_t total = 0;
@@ -321,7 +486,17 @@ __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, const u
         int hit_lane = -1, hit_w = 0;
         uint64_t hit_idx = 0;
         int c[kMaxW] = {0, 0, 0, 0, 0, 0};
-        if (rem_class <= 128u) {
+        unsigned long long shared_best;
+        if (call.split_threshold && pos == class_start && N[kc] > call.split_threshold &&
+            run_shared(code, call, shot, s, t0, wl[kc], tam, N[kc], false, L.A, L.sigA, nullptr, shared_best)) {
+            if (shared_best != ~0ull) {
+                hit_lane = 0;
+                hit_w = wl[kc];
+                hit_idx = class_start + shared_best;
+                unrank_dyn(shared_best, tam, hit_w, code.nCr, c);
+            }
+            pos = class_end;
+        } else if (rem_class <= 128u) {
             // one pattern per lane, 64 consecutive flattened indices (may span classes)
             const uint64_t idx = pos + (uint64_t)lane;
             bool cand = false;
@@ -350,14 +525,15 @@ __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, const u
             uint64_t r64 = rem_class / 64u;
             const int R = (int)(r64 < (uint64_t)rmax ? r64 : (uint64_t)rmax);   // >= 2 here
             const uint64_t start = pos + (uint64_t)lane * (uint64_t)R;
-            float dummy = 0.0f;
             const int w = wl[kc];
-            const int j = scan_range_w<0>(w, code, s, t0, start - class_start, R, tam, L.A, L.sigA, nullptr, c, dummy);
+            const int j = scan_range_w<0>(w, code, s, t0, start - class_start, R, tam, L.A, L.sigA, nullptr).j;
             const uint64_t b = ballot(j >= 0);
             if (b) {
-                hit_lane = __ffsll((long long)b) - 1;
-                hit_idx = pos + (uint64_t)hit_lane * (uint64_t)R + (uint64_t)bcast_i(j, hit_lane);
+                hit_lane = 0;
+                const int hl = __ffsll((long long)b) - 1;
+                hit_idx = pos + (uint64_t)hl * (uint64_t)R + (uint64_t)bcast_i(j, hl);
                 hit_w = w;
+                unrank_dyn(hit_idx - class_start, tam, w, code.nCr, c);   // same values in every lane
             }
             pos += 64ull * (uint64_t)R;
         }
@@ -383,7 +559,7 @@ __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, const u
 // grand_decoder.cu:590-666 over the LLR-sorted active list P: weights 1..max_w
 // in order, stop before weight w when a best exists and prefix[w] >= best, keep
 // the first strictly cheapest match (fp32 costs summed in pattern order).
-__device__ bool ml_wave(const DevCode &code, const DevCall &call, const uint32_t *s, uint32_t t0, int tam,
+__device__ bool ml_wave(const DevCode &code, const DevCall &call, int shot, const uint32_t *s, uint32_t t0, int tam,
                         const WaveLds &L, int *win, int &fw, unsigned long long &count, unsigned long long &wops) {
     const int lane = lane_id();
     sort_by_llr(code, L, tam);
@@ -400,6 +576,23 @@ __device__ bool ml_wave(const DevCode &code, const DevCall &call, const uint32_t
         const uint64_t Nw = ncr(code.nCr, tam, w);
         count += Nw;
         wops += Nw * (uint64_t)(w + 1);
+        unsigned long long shared_best;
+        if (call.split_threshold && Nw > call.split_threshold && Nw < (1ull << 32) &&
+            run_shared(code, call, shot, s, t0, w, tam, Nw, true, L.P, L.sigP, L.llrP, shared_best)) {
+            if (shared_best != ~0ull) {
+                const float cb = key_to_float((uint32_t)(shared_best >> 32));
+                if (!have || cb < best) {
+                    int c[kMaxW] = {0, 0, 0, 0, 0, 0};
+                    unrank_dyn(shared_best & 0xFFFFFFFFull, tam, w, code.nCr, c);
+                    have = true;
+                    best = cb;
+                    fw = w;
+#pragma unroll
+                    for (int l = 0; l < kMaxW; ++l) win[l] = (l < w) ? L.P[c[l]] : -1;
+                }
+            }
+            continue;
+        }
         uint64_t pos = 0;
         while (pos < Nw) {
             const uint64_t rem = Nw - pos;
@@ -407,24 +600,22 @@ __device__ bool ml_wave(const DevCode &code, const DevCall &call, const uint32_t
             const int R = (int)(r64 < (uint64_t)call.rmax ? r64 : (uint64_t)call.rmax);
             const uint64_t lstart = (uint64_t)lane * (uint64_t)R;
             const int len = lstart >= rem ? 0 : (int)((rem - lstart) < (uint64_t)R ? (rem - lstart) : (uint64_t)R);
-            int c[kMaxW] = {0, 0, 0, 0, 0, 0};
-            float cost = 0.0f;
-            const int j = scan_range_w<1>(w, code, s, t0, pos + lstart, len, tam, L.P, L.sigP, L.llrP, c, cost);
-            const uint32_t key = (j >= 0) ? order_key(cost) : 0xFFFFFFFFu;
+            const ScanHit h = scan_range_w<1>(w, code, s, t0, pos + lstart, len, tam, L.P, L.sigP, L.llrP);
+            const uint32_t key = (h.j >= 0) ? order_key(h.cost) : 0xFFFFFFFFu;
             const uint32_t kmin = wave_min(key);
-            const uint64_t b = ballot(j >= 0 && key == kmin);
+            const uint64_t b = ballot(h.j >= 0 && key == kmin);
             if (b) {
                 const int bl = __ffsll((long long)b) - 1;
-                const float cbest = bcast_f(cost, bl);
+                const float cbest = bcast_f(h.cost, bl);
                 if (!have || cbest < best) {
+                    const uint64_t r = (uint64_t)__shfl((long long)(pos + lstart + (uint64_t)(h.j < 0 ? 0 : h.j)), bl, 64);
+                    int c[kMaxW] = {0, 0, 0, 0, 0, 0};
+                    unrank_dyn(r, tam, w, code.nCr, c);
                     have = true;
                     best = cbest;
                     fw = w;
 #pragma unroll
-                    for (int l = 0; l < kMaxW; ++l) {
-                        const int cl = bcast_i(c[l], bl);
-                        win[l] = (l < w) ? L.P[cl] : -1;
-                    }
+                    for (int l = 0; l < kMaxW; ++l) win[l] = (l < w) ? L.P[c[l]] : -1;
                 }
             }
             pos += 64ull * (uint64_t)R;
@@ -568,9 +759,10 @@ __device__ void finalize_shot(const DevCode &code, const DevCall &call, int shot
 __device__ void decode_shot_wave(const DevCode &code, const DevCall &call, int shot, const WaveLds &L) {
     ShotOutcome o;
     o.t0 = now_ticks();
-    uint32_t s[kMaxWords];
-#pragma unroll
-    for (int d = 0; d < kMaxWords; ++d) s[d] = (d < code.D) ? call.s_packed[(size_t)shot * code.D + d] : 0u;
+    if (lane_id() < kMaxWords)
+        L.sw[lane_id()] = (lane_id() < code.D) ? call.s_packed[(size_t)shot * code.D + lane_id()] : 0u;
+    __syncthreads();
+    const uint32_t *s = L.sw;
     const uint32_t t0 = call.s_sig[shot];
 
     o.found = 0; o.fw = 0; o.count = 0; o.wops = 0; o.hibrido = -2;
@@ -592,11 +784,11 @@ __device__ void decode_shot_wave(const DevCode &code, const DevCall &call, int s
         o.hibrido = (br > 0) ? (o.hibrido == -1 ? 1 : 0) : -1;
         bool hit;
         if (call.kind == kML) {
-            hit = ml_wave(code, call, s, t0, tam, L, o.win, o.fw, o.count, o.wops);
+            hit = ml_wave(code, call, shot, s, t0, tam, L, o.win, o.fw, o.count, o.wops);
         } else {
             int wl[2 * kMaxW];
             const int nw = weight_list(call, tam, wl);
-            hit = first_hit_wave(code, call, s, t0, tam, wl, nw, L, o.win, o.fw, o.count, o.wops);
+            hit = first_hit_wave(code, call, shot, s, t0, tam, wl, nw, L, o.win, o.fw, o.count, o.wops);
         }
         if (hit) { o.found = 1; break; }
     }
@@ -618,7 +810,9 @@ __device__ void decode_shot_wave(const DevCode &code, const DevCall &call, int s
 
 // One wave per shot, 64 shots per group so that nonzero shots are appended to
 // the worklist with one atomic per group.
-__global__ __launch_bounds__(64) void triage_kernel(DevCode code, DevCall call) {
+__global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ codep, const DevCall *__restrict__ callp) {
+    const DevCode &code = *codep;
+    const DevCall &call = *callp;
     extern __shared__ uint32_t lds_raw[];
     const int lane = lane_id();
     uint32_t *ew = lds_raw;     // kMaxM / 32 words
@@ -727,18 +921,38 @@ __global__ __launch_bounds__(64) void triage_kernel(DevCode code, DevCall call) 
 }
 
 // Persistent search kernel: waves claim worklist entries in batches until none remain.
-__global__ __launch_bounds__(64) void fused_kernel(DevCode code, DevCall call) {
+__global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ codep, const DevCall *__restrict__ callp) {
+    const DevCode &code = *codep;
+    const DevCall &call = *callp;
     extern __shared__ uint32_t lds_raw[];
     const WaveLds L = carve_lds(lds_raw, call.cap, call.kind == kML);
     const unsigned total = call.counters[0];
     const unsigned B = (unsigned)call.claim_batch;
+    int cslot = -1;
+    unsigned cgen = 0;
     for (;;) {
-        unsigned first = 0;
-        if (lane_id() == 0) first = atomicAdd(&call.counters[1], B);
+        // shared classes of deep shots first: they bound the batch's completion time
+        if (call.split_threshold && help_task(code, call, L, cslot, cgen)) continue;
+        unsigned first = total;
+        if (lane_id() == 0) {
+            first = ld_relaxed(&call.counters[1]);
+            if (first < total) first = atomicAdd(&call.counters[1], B);
+        }
         first = (unsigned)bcast_i((int)first, 0);
-        if (first >= total) break;
-        const unsigned last = (first + B < total) ? first + B : total;
-        for (unsigned i = first; i < last; ++i) decode_shot_wave(code, call, call.worklist[i], L);
+        if (first < total) {
+            if (lane_id() == 0) atomicAdd(&call.task_counters[1], 1u);
+            const unsigned last = (first + B < total) ? first + B : total;
+            for (unsigned i = first; i < last; ++i) decode_shot_wave(code, call, call.worklist[i], L);
+            cslot = -1;
+            if (lane_id() == 0) atomicSub(&call.task_counters[1], 1u);
+            continue;
+        }
+        if (!call.split_threshold) break;
+        // queue drained: stay while shots are in flight, to help their shared classes
+        const unsigned inflight = __builtin_amdgcn_readfirstlane(ld_relaxed(&call.task_counters[1]));
+        const unsigned open = __builtin_amdgcn_readfirstlane(ld_relaxed(&call.task_counters[0]));
+        if (inflight == 0u && open == 0u) break;
+        __builtin_amdgcn_s_sleep(2);
     }
 }
 
@@ -753,11 +967,13 @@ struct PlainState {
 
 // Builds A for threshold thr into the shot's global scratch (and, for ML with
 // sort_tam > 0, the LLR-sorted copy with its prefix sums).
-__global__ __launch_bounds__(64) void plain_active_kernel(DevCode code, DevCall call, int shot, int thr, int sort_tam,
+__global__ __launch_bounds__(64) void plain_active_kernel(const DevCode *__restrict__ codep, const DevCall *__restrict__ callp, int shot, int thr, int sort_tam,
                                                           int *gA, uint32_t *gsig, float *gllr, int *gP, uint32_t *gsigP,
                                                           float *gllrP, PlainState *st) {
+    const DevCode &code = *codep;
+    const DevCall &call = *callp;
     WaveLds L;
-    L.A = gA; L.sigA = gsig; L.llrA = gllr; L.P = gP; L.sigP = gsigP; L.llrP = gllrP; L.ew = nullptr;
+    L.A = gA; L.sigA = gsig; L.llrA = gllr; L.P = gP; L.sigP = gsigP; L.llrP = gllrP; L.ew = nullptr; L.sw = nullptr;
     if (sort_tam > 0) {
         sort_by_llr(code, L, sort_tam);
         if (lane_id() == 0) {
@@ -767,9 +983,7 @@ __global__ __launch_bounds__(64) void plain_active_kernel(DevCode code, DevCall 
         }
         return;
     }
-    uint32_t s[kMaxWords];
-#pragma unroll
-    for (int d = 0; d < kMaxWords; ++d) s[d] = (d < code.D) ? call.s_packed[(size_t)shot * code.D + d] : 0u;
+    const uint32_t *s = call.s_packed + (size_t)shot * code.D;
     const int tam = build_active(code, s, thr, L, call.cap);
     if (lane_id() == 0) { st->tam_new = tam; st->best = ~0ull; }
 }
@@ -783,28 +997,25 @@ struct MlRecord {
     int c[kMaxW];
 };
 
-__global__ __launch_bounds__(64) void plain_class_kernel(DevCode code, DevCall call, int shot, int w, int tam, int R,
+__global__ __launch_bounds__(64) void plain_class_kernel(const DevCode *__restrict__ codep, const DevCall *__restrict__ callp, int shot, int w, int tam, int R,
                                                          unsigned long long nchunks, int ml, const int *gA,
                                                          const uint32_t *gsig, const float *gllr, PlainState *st,
                                                          MlRecord *rec) {
+    const DevCode &code = *codep;
+    const DevCall &call = *callp;
     const int lane = lane_id();
-    uint32_t s[kMaxWords];
-#pragma unroll
-    for (int d = 0; d < kMaxWords; ++d) s[d] = (d < code.D) ? call.s_packed[(size_t)shot * code.D + d] : 0u;
+    const uint32_t *s = call.s_packed + (size_t)shot * code.D;
     const uint32_t t0 = call.s_sig[shot];
     const uint64_t Nw = ncr(code.nCr, tam, w);
     uint32_t bkey = 0xFFFFFFFFu;
     uint64_t brank = 0;
-    int bc[kMaxW] = {0, 0, 0, 0, 0, 0};
     for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
         const uint64_t base = ch * 64ull * (uint64_t)R;
         if (!ml && base > __hip_atomic_load(&st->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
         const uint64_t lstart = base + (uint64_t)lane * (uint64_t)R;
         const int len = lstart >= Nw ? 0 : (int)((Nw - lstart) < (uint64_t)R ? (Nw - lstart) : (uint64_t)R);
-        int c[kMaxW];
-        float cost = 0.0f;
         if (!ml) {
-            const int j = scan_range_w<0>(w, code, s, t0, lstart, len, tam, gA, gsig, nullptr, c, cost);
+            const int j = scan_range_w<0>(w, code, s, t0, lstart, len, tam, gA, gsig, nullptr).j;
             const uint64_t b = ballot(j >= 0);
             if (b) {
                 const int hl = __ffsll((long long)b) - 1;
@@ -813,26 +1024,27 @@ __global__ __launch_bounds__(64) void plain_class_kernel(DevCode code, DevCall c
                 break;
             }
         } else {
-            const int j = scan_range_w<1>(w, code, s, t0, lstart, len, tam, gA, gsig, gllr, c, cost);
-            const uint32_t key = (j >= 0) ? order_key(cost) : 0xFFFFFFFFu;
+            const ScanHit h = scan_range_w<1>(w, code, s, t0, lstart, len, tam, gA, gsig, gllr);
+            const uint32_t key = (h.j >= 0) ? order_key(h.cost) : 0xFFFFFFFFu;
             const uint32_t kmin = wave_min(key);
-            const uint64_t b = ballot(j >= 0 && key == kmin);
+            const uint64_t b = ballot(h.j >= 0 && key == kmin);
             if (b && kmin < bkey) {   // chunks visited in rank order: strict keeps the earliest
                 const int bl = __ffsll((long long)b) - 1;
                 bkey = kmin;
-                brank = lstart + (uint64_t)bcast_i(j, bl);
-                brank = (uint64_t)__shfl((long long)brank, bl, 64);
-#pragma unroll
-                for (int l = 0; l < kMaxW; ++l) bc[l] = bcast_i(c[l], bl);
+                brank = (uint64_t)__shfl((long long)(lstart + (uint64_t)(h.j < 0 ? 0 : h.j)), bl, 64);
             }
         }
     }
-    if (ml && lane == 0) {
-        MlRecord r;
-        r.key = bkey; r.pad = 0; r.rank = brank;
+    if (ml) {
+        int bc[kMaxW] = {0, 0, 0, 0, 0, 0};
+        if (bkey != 0xFFFFFFFFu) unrank_dyn(brank, tam, w, code.nCr, bc);
+        if (lane == 0) {
+            MlRecord r;
+            r.key = bkey; r.pad = 0; r.rank = brank;
 #pragma unroll
-        for (int l = 0; l < kMaxW; ++l) r.c[l] = bc[l];
-        rec[blockIdx.x] = r;
+            for (int l = 0; l < kMaxW; ++l) r.c[l] = bc[l];
+            rec[blockIdx.x] = r;
+        }
     }
 }
 
@@ -848,8 +1060,10 @@ __global__ __launch_bounds__(64) void plain_ml_reduce_kernel(const MlRecord *rec
 }
 
 // Rebuilds the hit of a first-hit class (rank -> columns) for the finalizer.
-__global__ __launch_bounds__(64) void plain_finalize_kernel(DevCode code, DevCall call, int shot, ShotOutcome o,
-                                                            const int *gA, unsigned long long hit_rank) {
+__global__ __launch_bounds__(64) void plain_finalize_kernel(const DevCode *__restrict__ codep, const DevCall *__restrict__ callp, int shot,
+                                                            ShotOutcome o, const int *gA, unsigned long long hit_rank) {
+    const DevCode &code = *codep;
+    const DevCall &call = *callp;
     if (o.found && o.win[0] < 0) {
         int c[kMaxW];
         unrank_dyn(hit_rank, o.tam, o.fw, code.nCr, c);
@@ -884,12 +1098,14 @@ __global__ void fill_u32_kernel(unsigned *p, unsigned v, int n) {
 }  // namespace grand
 
 // ----------------------------------------------------------- launch wrappers
+// `code`/`call` are device copies of the descriptors; `hcall` is the host copy
+// (launch geometry only).
 extern "C" {
 using grand::DevCall;
 using grand::DevCode;
 
 size_t grand_fused_lds_bytes(int cap, int ml) {
-    return sizeof(uint32_t) * ((size_t)grand::kMaxM / 32 + (size_t)cap * (ml ? 6 : 2));
+    return sizeof(uint32_t) * ((size_t)grand::kMaxM / 32 + grand::kMaxWords + (size_t)cap * (ml ? 6 : 2));
 }
 
 hipError_t grand_launch_fill(unsigned *p, unsigned v, int n, hipStream_t st) {
@@ -898,13 +1114,13 @@ hipError_t grand_launch_fill(unsigned *p, unsigned v, int n, hipStream_t st) {
 }
 
 hipError_t grand_launch_triage(const DevCode *code, const DevCall *call, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(grand::triage_kernel, dim3(grid), dim3(64), sizeof(uint32_t) * (grand::kMaxM / 32), st, *code, *call);
+    hipLaunchKernelGGL(grand::triage_kernel, dim3(grid), dim3(64), sizeof(uint32_t) * (grand::kMaxM / 32), st, code, call);
     return hipGetLastError();
 }
 
-hipError_t grand_launch_fused(const DevCode *code, const DevCall *call, int grid, hipStream_t st) {
-    const size_t lds = grand_fused_lds_bytes(call->cap, call->kind == grand::kML);
-    hipLaunchKernelGGL(grand::fused_kernel, dim3(grid), dim3(64), lds, st, *code, *call);
+hipError_t grand_launch_fused(const DevCode *code, const DevCall *call, const DevCall *hcall, int grid, hipStream_t st) {
+    const size_t lds = grand_fused_lds_bytes(hcall->cap, hcall->kind == grand::kML);
+    hipLaunchKernelGGL(grand::fused_kernel, dim3(grid), dim3(64), lds, st, code, call);
     return hipGetLastError();
 }
 
@@ -920,31 +1136,29 @@ hipError_t grand_launch_expand(const uint32_t *packed, int words, int cols, int 
 size_t grand_plain_state_bytes() { return sizeof(grand::PlainState); }
 size_t grand_plain_record_bytes() { return sizeof(grand::MlRecord); }
 
-hipError_t grand_launch_plain_active(const DevCode *code, const DevCall *call, int shot, int thr, int sort_tam,
-                                     void *scratch, void *state, hipStream_t st) {
-    const int cap = call->cap;
+hipError_t grand_launch_plain_active(const DevCode *code, const DevCall *call, int cap, int shot, int thr,
+                                     int sort_tam, void *scratch, void *state, hipStream_t st) {
     int *gA = (int *)scratch;
     uint32_t *gsig = (uint32_t *)(gA + cap);
     float *gllr = (float *)(gsig + cap);
     int *gP = (int *)(gllr + cap);
     uint32_t *gsigP = (uint32_t *)(gP + cap);
     float *gllrP = (float *)(gsigP + cap);
-    hipLaunchKernelGGL(grand::plain_active_kernel, dim3(1), dim3(64), 0, st, *code, *call, shot, thr, sort_tam, gA,
+    hipLaunchKernelGGL(grand::plain_active_kernel, dim3(1), dim3(64), 0, st, code, call, shot, thr, sort_tam, gA,
                        gsig, gllr, gP, gsigP, gllrP, (grand::PlainState *)state);
     return hipGetLastError();
 }
 
-hipError_t grand_launch_plain_class(const DevCode *code, const DevCall *call, int shot, int w, int tam, int R,
+hipError_t grand_launch_plain_class(const DevCode *code, const DevCall *call, int cap, int shot, int w, int tam, int R,
                                     unsigned long long nchunks, int grid, int ml, void *scratch, void *state,
                                     void *records, hipStream_t st) {
-    const int cap = call->cap;
     int *gA = (int *)scratch;
     uint32_t *gsig = (uint32_t *)(gA + cap);
     float *gllr = (float *)(gsig + cap);
     int *gP = (int *)(gllr + cap);
     uint32_t *gsigP = (uint32_t *)(gP + cap);
     float *gllrP = (float *)(gsigP + cap);
-    hipLaunchKernelGGL(grand::plain_class_kernel, dim3(grid), dim3(64), 0, st, *code, *call, shot, w, tam, R, nchunks,
+    hipLaunchKernelGGL(grand::plain_class_kernel, dim3(grid), dim3(64), 0, st, code, call, shot, w, tam, R, nchunks,
                        ml, ml ? gP : gA, ml ? gsigP : gsig, gllrP, (grand::PlainState *)state,
                        (grand::MlRecord *)records);
     return hipGetLastError();
@@ -959,7 +1173,7 @@ hipError_t grand_launch_plain_ml_reduce(void *records, int nrec, void *out, hipS
 hipError_t grand_launch_plain_finalize(const DevCode *code, const DevCall *call, int shot,
                                        const grand::ShotOutcome *o, void *scratch, unsigned long long hit_rank,
                                        hipStream_t st) {
-    hipLaunchKernelGGL(grand::plain_finalize_kernel, dim3(1), dim3(64), 0, st, *code, *call, shot, *o,
+    hipLaunchKernelGGL(grand::plain_finalize_kernel, dim3(1), dim3(64), 0, st, code, call, shot, *o,
                        (const int *)scratch, hit_rank);
     return hipGetLastError();
 }

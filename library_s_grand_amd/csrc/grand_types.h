@@ -40,6 +40,24 @@ struct DevCode {
     const unsigned long long *nCr;    // [(kMaxTam + 1) * kNcrStride]
 };
 
+// A weight class too large for one wave, shared through the task table.  The
+// owner wave publishes it; any wave claims chunks of 64 * R ranks with a CAS on
+// ctl = (generation << 32) | next_chunk, so a slot reused for a later task can
+// never hand out a chunk of the earlier one.
+constexpr int kTaskSlots = 1024;
+struct TaskSlot {
+    unsigned long long ctl;           // (gen << 32) | next chunk to claim
+    unsigned long long best;          // first-hit: lowest matching rank; ML: (cost key << 32) | rank
+    unsigned done;                    // chunks finished
+    unsigned busy;                    // 0 free, 1 owned
+    unsigned gen;                     // generation the fields below belong to
+    unsigned nchunks;
+    int shot, w, tam, R, ml;
+    uint32_t t0;                      // signature of the syndrome
+    uint32_t s[kMaxWords];            // syndrome words
+    unsigned long long nw;            // class size C(tam, w)
+};
+
 // Per-call parameters and per-shot buffers.
 struct DevCall {
     int family, kind;
@@ -66,6 +84,10 @@ struct DevCall {
     uint32_t *dec;                    // [shots][M32] decoded pattern
     uint32_t *declog;                 // [shots][NL32] decoded logicals (cln)
     unsigned long long *stats;        // [shots][3] claim tick, completion tick, weighted pattern count; or null
+    TaskSlot *tasks;                  // [kTaskSlots]
+    uint32_t *task_arrays;            // [kTaskSlots][3][cap]: active list, signatures, sorted LLRs
+    unsigned *task_counters;          // [0] open tasks, [1] shots in flight
+    unsigned long long split_threshold;  // classes larger than this are shared (0 = never)
 };
 
 // Search outcome of one shot, handed from a search engine to the finalizer.

@@ -32,8 +32,8 @@ def bind_session(lib):
     lib.grand_session_footprint.restype = ctypes.c_size_t
     lib.grand_session_destroy.argtypes = [_VP]
     lib.grand_device_info.argtypes = [ctypes.POINTER(ctypes.c_int)] * 3
-    lib.grand_set_tuning.argtypes = [ctypes.c_int] * 3
-    lib.grand_get_tuning.argtypes = [ctypes.POINTER(ctypes.c_int)] * 3
+    lib.grand_set_tuning.argtypes = [ctypes.c_int] * 4
+    lib.grand_get_tuning.argtypes = [ctypes.POINTER(ctypes.c_int)] * 4
     lib._grand_session_bound = True
     return lib
 

@@ -101,3 +101,38 @@ def test_plain_engine_ml(oracle, product, tmp_path):
     o = T.run_cln(oracle, G.KIND_ML_CLN, H, L, det, lg, cfg=cfg, file=lf)
     p = T.run_cln(product, G.KIND_ML_CLN, H, L, det, lg, cfg=cfg, file=lf, ex=G.make_ex(engine=G.GRAND_ENGINE_PLAIN))
     T.assert_same(o, p, "plain ML")
+
+
+# ---------------------------------------------------------------- shared classes
+# split_threshold = 1 shares every weight class through the task table, 200
+# shares the larger ones, -1 never shares: all must equal the oracle.
+@pytest.mark.parametrize("split", [1, 200, -1])
+def test_shared_classes_cc(oracle, product, split, tmp_path):
+    hx, hz, e = T.cc_workload(144, 3000, "loguniform", seed=12)
+    pf = T.phase_file(tmp_path)
+    cfg = G.make_config(max_error_weight=4)
+    ex = G.make_ex(split_threshold=split)
+    o = T.run_cc(oracle, G.KIND_PROB_CC, hx, hz, errors=e, cfg=cfg, prob_file=pf)
+    p = T.run_cc(product, G.KIND_PROB_CC, hx, hz, errors=e, cfg=cfg, prob_file=pf, ex=ex)
+    T.assert_same(o, p, f"shared prob_cc split={split}")
+
+
+@pytest.mark.parametrize("split", [1, 500, -1])
+def test_shared_classes_ml(oracle, product, split, tmp_path):
+    H, L, pr, det, lg = T.dem_workload(400, seed=13)
+    lf = T.llr_file(tmp_path, pr)
+    cfg = G.make_config(max_error_weight=4, max_size_active_positions=32)
+    ex = G.make_ex(split_threshold=split)
+    o = T.run_cln(oracle, G.KIND_ML_CLN, H, L, det, lg, cfg=cfg, file=lf)
+    p = T.run_cln(product, G.KIND_ML_CLN, H, L, det, lg, cfg=cfg, file=lf, ex=ex)
+    T.assert_same(o, p, f"shared ML split={split}")
+
+
+@pytest.mark.parametrize("split", [1, 300])
+def test_shared_classes_cln_deep(oracle, product, split):
+    H, L, pr, det, lg = T.dem_workload(800, seed=14)
+    cfg = G.make_config(max_error_weight=4, max_size_active_positions=40)
+    ex = G.make_ex(split_threshold=split)
+    o = T.run_cln(oracle, G.KIND_CLN, H, L, det, lg, cfg=cfg)
+    p = T.run_cln(product, G.KIND_CLN, H, L, det, lg, cfg=cfg, ex=ex)
+    T.assert_same(o, p, f"shared cln split={split}")
