@@ -383,6 +383,7 @@ struct GrandSession {
     grand::Workspace ws;
     grand::DevMem plain_scratch, plain_state, plain_records, plain_out;
     grand::DevMem dcall;              // device copy of `call`, refreshed before every batch
+    grand::DevMem dbg;                // chunk trace buffer (GRAND_DEBUG_TASKS)
     bool timing = false;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};   // before triage, after triage, after search
     ~GrandSession() {
@@ -653,6 +654,15 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     c.dec = d_dec ? d_dec : s->ws.dec.as<uint32_t>();
     c.declog = d_declog ? d_declog : s->ws.declog.as<uint32_t>();
     c.stats = d_stats;
+    c.dbg = nullptr;
+    c.dbg_cap = 0;
+    if (const char *dbg = getenv("GRAND_DEBUG_TASKS")) {
+        if (dbg[0] && s->dbg.ensure(4u << 20)) {
+            c.dbg = s->dbg.as<unsigned>();
+            c.dbg_cap = (4u << 20) / 4;
+            HIPX(grand_launch_fill(c.dbg, 0u, 1, st));
+        }
+    }
     c.tasks = s->ws.tasks.as<TaskSlot>();
     c.task_arrays = s->ws.task_arrays.as<uint32_t>();
     c.task_counters = s->ws.counters.as<unsigned>() + 4;
@@ -768,6 +778,22 @@ int run_abi(int kind, const int *H, const int *H2, int n, int m, int NL, char ty
     HIPX(hipMemcpyAsync(&bits, s->call.counters + 2, sizeof(unsigned), hipMemcpyDeviceToHost, st));
     HIPX(hipStreamSynchronize(st));
     (void)plain;
+    if (const char *dbg = getenv("GRAND_DEBUG_TASKS")) {
+        if (dbg[0] && s->call.dbg) {
+            std::vector<unsigned> h(s->call.dbg_cap);
+            if (hipMemcpy(h.data(), s->call.dbg, h.size() * 4, hipMemcpyDeviceToHost) == hipSuccess) {
+                FILE *f = fopen(dbg, "w");
+                if (f) {
+                    const unsigned nrec = std::min<unsigned>(h[0], s->call.dbg_cap / 16 - 2);
+                    for (unsigned i = 0; i < nrec; ++i) {
+                        for (int k = 0; k < 16; ++k) fprintf(f, "%u ", h[16 * (i + 1) + k]);
+                        fprintf(f, "\n");
+                    }
+                    fclose(f);
+                }
+            }
+        }
+    }
     if (report_error_bits(bits)) return -1;
     if (print_enabled())
         for (int i = 0; i < num_shots; ++i) print_shot(family, family == kCC && input_errors, i, results[i]);

@@ -56,6 +56,13 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v) {
 }
 
 __device__ __forceinline__ int bcast_i(int v, int src) { return __shfl(v, src, 64); }
+// Value of the first active lane as a wave-uniform (scalar) value; used so that
+// loop exits and branches after lane-0 atomics depend only on uniform values.
+__device__ __forceinline__ unsigned uni(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ unsigned long long uni64(unsigned long long v) {
+    const unsigned lo = uni((unsigned)v), hi = uni((unsigned)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
 __device__ __forceinline__ uint32_t bcast_u(uint32_t v, int src) { return (uint32_t)__shfl((int)v, src, 64); }
 __device__ __forceinline__ float bcast_f(float v, int src) { return __shfl(v, src, 64); }
 
@@ -297,34 +304,36 @@ __device__ __forceinline__ float key_to_float(uint32_t key) {
     return __uint_as_float(b);
 }
 
-// Lane 0 claims the next chunk of slot T; returns it (broadcast) or ~0u.
+// Claims the next chunk of slot T for this wave; returns it or ~0u.  Lane 0
+// issues the CAS; every exit depends only on wave-uniform values.
 __device__ __forceinline__ unsigned claim_chunk(TaskSlot *T, unsigned *open_count, unsigned &gen) {
-    unsigned ch = ~0u, g = 0;
-    if (lane_id() == 0) {
-        unsigned long long snap = ld_acquire(&T->ctl);
-        while (ctl_next(snap) < ctl_nch(snap)) {
-            const unsigned long long prev = atomicCAS(&T->ctl, snap, snap + 1ull);
-            if (prev == snap) {
-                ch = ctl_next(snap);
-                g = ctl_gen(snap);
-                if (ch + 1u == ctl_nch(snap)) atomicSub(open_count, 1u);
-                break;
-            }
-            snap = prev;
+    const bool leader = lane_id() == 0;
+    unsigned long long snap = uni64(ld_acquire(&T->ctl));
+    for (;;) {
+        if (ctl_next(snap) >= ctl_nch(snap)) { gen = 0; return ~0u; }
+        unsigned long long prev = 0;
+        if (leader) prev = atomicCAS(&T->ctl, snap, snap + 1ull);
+        prev = uni64(prev);
+        if (prev == snap) {
+            const unsigned ch = ctl_next(snap);
+            if (leader && ch + 1u == ctl_nch(snap)) atomicSub(open_count, 1u);
+            gen = ctl_gen(snap);
+            return ch;
         }
+        snap = prev;
     }
-    gen = (unsigned)bcast_i((int)g, 0);
-    return (unsigned)bcast_i((int)ch, 0);
 }
 
 // One chunk (64 lanes x R ranks) of a shared class; the class's arrays are in
 // this wave's LDS.  First-hit: atomicMin of the lowest matching rank, chunks
 // past a known hit are skipped.  ML: atomicMin of (cost key << 32 | rank).
-__device__ void task_chunk(const DevCode &code, TaskSlot *T, const uint32_t *s, uint32_t t0, int w, int tam, int R,
-                           uint64_t Nw, bool ml, unsigned ch, const int *act, const uint32_t *sg, const float *llr) {
+__device__ unsigned long long task_chunk(const DevCode &code, TaskSlot *T, const uint32_t *s, uint32_t t0, int w,
+                                        int tam, int R, uint64_t Nw, bool ml, unsigned ch, const int *act,
+                                        const uint32_t *sg, const float *llr) {
     const int lane = lane_id();
     const uint64_t base = (uint64_t)ch * 64ull * (uint64_t)R;
-    if (!ml && base > ld_relaxed(&T->best)) return;
+    unsigned long long out = ~0ull;
+    if (!ml && base > ld_relaxed(&T->best)) return out - 1;
     const uint64_t lstart = base + (uint64_t)lane * (uint64_t)R;
     const int len = lstart >= Nw ? 0 : (int)((Nw - lstart) < (uint64_t)R ? (Nw - lstart) : (uint64_t)R);
     if (!ml) {
@@ -334,6 +343,7 @@ __device__ void task_chunk(const DevCode &code, TaskSlot *T, const uint32_t *s, 
             const int hl = __ffsll((long long)b) - 1;
             const uint64_t r = base + (uint64_t)hl * (uint64_t)R + (uint64_t)bcast_i(j, hl);
             if (lane == 0) atomicMin(&T->best, (unsigned long long)r);
+            out = r;
         }
     } else {
         const ScanHit h = scan_range_w<1>(w, code, s, t0, lstart, len, tam, act, sg, llr);
@@ -344,8 +354,24 @@ __device__ void task_chunk(const DevCode &code, TaskSlot *T, const uint32_t *s, 
             const int bl = __ffsll((long long)b) - 1;
             const uint64_t r = (uint64_t)__shfl((long long)(lstart + (uint64_t)(h.j < 0 ? 0 : h.j)), bl, 64);
             if (lane == 0) atomicMin(&T->best, ((unsigned long long)kmin << 32) | r);
+            out = ((unsigned long long)kmin << 32) | r;
         }
     }
+    return out;
+}
+
+// Debug trace of shared chunks (enabled by the host only for diagnosis).
+__device__ void trace_chunk(const DevCall &call, int who, int slot, unsigned gen, unsigned ch, int w, int tam, int R,
+                            uint64_t Nw, uint32_t t0, const uint32_t *s, const int *act, const uint32_t *sg,
+                            unsigned long long res) {
+    const unsigned active = (unsigned)__popcll(ballot(true));
+    if (!call.dbg || lane_id() != 0) return;
+    const unsigned i = atomicAdd(&call.dbg[0], 1u);
+    if (16u * (i + 1u) + 16u > call.dbg_cap) return;
+    unsigned *r = call.dbg + 16u * (i + 1u);
+    r[0] = (unsigned)who; r[1] = (unsigned)slot; r[2] = gen; r[3] = ch; r[4] = (unsigned)w; r[5] = (unsigned)tam;
+    r[6] = (unsigned)R; r[7] = (unsigned)Nw; r[8] = t0; r[9] = s[0]; r[10] = (unsigned)act[0]; r[11] = sg[0];
+    r[12] = (unsigned)res; r[13] = (unsigned)(res >> 32); r[14] = active; r[15] = sg[1];
 }
 
 // Owner side: publish the weight-w class over act/sg/llr (this wave's LDS) as a
@@ -359,15 +385,21 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
     const uint64_t per = 64ull * (uint64_t)kMaxChunks;
     if (Nw / per >= (uint64_t)R) R = (int)(Nw / per + 1);
     const unsigned nch = (unsigned)((Nw + 64ull * R - 1) / (64ull * R));
+    // find a free slot: 64 candidates per step, lane 0 takes the first free one
     int slot = -1;
-    if (lane == 0) {
-        const int start = (int)((blockIdx.x * 131u) & (kTaskSlots - 1));
-        for (int i = 0; i < kTaskSlots; ++i) {
-            const int k = (start + i) & (kTaskSlots - 1);
-            if (ld_relaxed(&call.tasks[k].busy) == 0u && atomicCAS(&call.tasks[k].busy, 0u, 1u) == 0u) { slot = k; break; }
+    const int start = (int)((blockIdx.x * 131u) & (kTaskSlots - 1));
+    for (int b0 = 0; b0 < kTaskSlots; b0 += 64) {
+        const int k = (start + b0 + lane) & (kTaskSlots - 1);
+        uint64_t freem = ballot(ld_relaxed(&call.tasks[k].busy) == 0u);
+        while (freem) {
+            const int pick = (int)uni((unsigned)(start + b0 + __ffsll((long long)freem) - 1) & (kTaskSlots - 1));
+            unsigned old = 1u;
+            if (lane == 0) old = atomicCAS(&call.tasks[pick].busy, 0u, 1u);
+            if (uni(old) == 0u) { slot = pick; break; }
+            freem &= freem - 1;
         }
+        if (slot >= 0) break;
     }
-    slot = bcast_i(slot, 0);
     if (slot < 0) return false;
     TaskSlot *T = &call.tasks[slot];
     uint32_t *arr = call.task_arrays + (size_t)slot * 3 * call.cap;
@@ -393,12 +425,13 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
         unsigned g;
         const unsigned ch = claim_chunk(T, &call.task_counters[0], g);
         if (ch == ~0u) break;
-        task_chunk(code, T, s, t0, w, tam, R, Nw, ml, ch, act, sg, llr);
+        const unsigned long long res = task_chunk(code, T, s, t0, w, tam, R, Nw, ml, ch, act, sg, llr);
+        trace_chunk(call, 0, slot, g, ch, w, tam, R, Nw, t0, s, act, sg, res);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         if (lane == 0) __hip_atomic_fetch_add(&T->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
-    while (ld_acquire(&T->done) < nch) __builtin_amdgcn_s_sleep(1);
-    best = ld_acquire(&T->best);
+    while (uni(ld_acquire(&T->done)) < nch) __builtin_amdgcn_s_sleep(1);
+    best = uni64(ld_acquire(&T->best));
     if (lane == 0) __hip_atomic_store(&T->busy, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     return true;
 }
@@ -410,11 +443,11 @@ __device__ bool help_task(const DevCode &code, const DevCall &call, const WaveLd
     if (__builtin_amdgcn_readfirstlane(ld_relaxed(&call.task_counters[0])) == 0u) return false;
     int slot = -1;
     const int start = (int)((blockIdx.x * 37u) & (kTaskSlots - 1));
-    for (int b0 = 0; b0 < kTaskSlots && slot < 0; b0 += 64) {
+    for (int b0 = 0; b0 < kTaskSlots; b0 += 64) {
         const int k = (start + b0 + lane) & (kTaskSlots - 1);
         const unsigned long long c = ld_relaxed(&call.tasks[k].ctl);
         const uint64_t b = ballot(ctl_next(c) < ctl_nch(c));
-        if (b) slot = bcast_i(k, __ffsll((long long)b) - 1);
+        if (b) { slot = (int)uni((unsigned)(start + b0 + __ffsll((long long)b) - 1) & (kTaskSlots - 1)); break; }
     }
     if (slot < 0) return false;
     TaskSlot *T = &call.tasks[slot];
@@ -442,7 +475,8 @@ __device__ bool help_task(const DevCode &code, const DevCall &call, const WaveLd
         cslot = slot;
         cgen = gen;
     }
-    task_chunk(code, T, s, t0, w, tam, R, Nw, ml, ch, act, sg, ml ? L.llrP : nullptr);
+    const unsigned long long res = task_chunk(code, T, s, t0, w, tam, R, Nw, ml, ch, act, sg, ml ? L.llrP : nullptr);
+    trace_chunk(call, 1, slot, gen, ch, w, tam, R, Nw, t0, s, act, sg, res);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     if (lane == 0) __hip_atomic_fetch_add(&T->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     return true;
@@ -913,7 +947,7 @@ __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ 
         if (mask) {
             unsigned pos = 0;
             if (lane == 0) pos = atomicAdd(&call.counters[0], (unsigned)__popcll(mask));
-            pos = (unsigned)bcast_i((int)pos, 0);
+            pos = uni(pos);
             if (my_nz) call.worklist[pos + lanes_below(mask)] = base + lane;
         }
     }
@@ -933,12 +967,12 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
     for (;;) {
         // shared classes of deep shots first: they bound the batch's completion time
         if (call.split_threshold && help_task(code, call, L, cslot, cgen)) continue;
-        unsigned first = total;
-        if (lane_id() == 0) {
-            first = ld_relaxed(&call.counters[1]);
-            if (first < total) first = atomicAdd(&call.counters[1], B);
+        unsigned first = uni(ld_relaxed(&call.counters[1]));
+        if (first < total) {
+            unsigned got = 0;
+            if (lane_id() == 0) got = atomicAdd(&call.counters[1], B);
+            first = uni(got);
         }
-        first = (unsigned)bcast_i((int)first, 0);
         if (first < total) {
             if (lane_id() == 0) atomicAdd(&call.task_counters[1], 1u);
             const unsigned last = (first + B < total) ? first + B : total;

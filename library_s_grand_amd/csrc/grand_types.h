@@ -88,6 +88,8 @@ struct DevCall {
     uint32_t *task_arrays;            // [kTaskSlots][3][cap]: active list, signatures, sorted LLRs
     unsigned *task_counters;          // [0] open tasks, [1] shots in flight
     unsigned long long split_threshold;  // classes larger than this are shared (0 = never)
+    unsigned *dbg;                    // optional chunk trace (debug builds of a run), [0] = records
+    unsigned dbg_cap;
 };
 
 // Search outcome of one shot, handed from a search engine to the finalizer.
