@@ -30,6 +30,10 @@
 
 namespace grand {
 
+// Pointers into the workgroup's LDS carry the LDS address space, so every access
+// through them is a ds_read / ds_write rather than a flat access.
+#define LDS __attribute__((address_space(3)))
+
 // ------------------------------------------------------------------ wave helpers
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
@@ -127,28 +131,32 @@ __device__ __forceinline__ void unrank_dyn(uint64_t r, int tam, int w, const uns
 // Per-wave LDS layout (cap entries each): A (active columns), sig of A, and for
 // ML the LLR-sorted copies P / sigP / llrP plus the unsorted llr of A.
 struct WaveLds {
-    int *A;
-    uint32_t *sigA;
-    float *llrA;
-    int *P;
-    uint32_t *sigP;
-    float *llrP;
-    uint32_t *ew;      // packed error words of the shot (triage / finalize), kMaxM / 32
-    uint32_t *sw;      // syndrome words of the shot being searched, kMaxWords
+    int LDS *A;
+    uint32_t LDS *sigA;
+    float LDS *llrA;
+    int LDS *P;
+    uint32_t LDS *sigP;
+    float LDS *llrP;
+    uint32_t LDS *ew;  // packed error words of the shot (triage), kMaxM / 32
+    uint32_t LDS *sw;  // syndrome words of the shot being searched, kMaxWords
 };
 
-__device__ __forceinline__ WaveLds carve_lds(uint32_t *base, int cap, bool ml) {
+extern __shared__ uint32_t lds_raw[];
+__device__ __forceinline__ uint32_t LDS *lds_base() { return (uint32_t LDS *)lds_raw; }
+
+__device__ __forceinline__ WaveLds carve_lds(int cap, bool ml) {
+    uint32_t LDS *base = lds_base();
     WaveLds L;
     L.ew = base;
     L.sw = base + kMaxM / 32;
-    uint32_t *p = L.sw + kMaxWords;
-    L.A = (int *)p; p += cap;
+    uint32_t LDS *p = L.sw + kMaxWords;
+    L.A = (int LDS *)p; p += cap;
     L.sigA = p; p += cap;
     if (ml) {
-        L.llrA = (float *)p; p += cap;
-        L.P = (int *)p; p += cap;
+        L.llrA = (float LDS *)p; p += cap;
+        L.P = (int LDS *)p; p += cap;
         L.sigP = p; p += cap;
-        L.llrP = (float *)p; p += cap;
+        L.llrP = (float LDS *)p; p += cap;
     } else {
         L.llrA = nullptr; L.P = nullptr; L.sigP = nullptr; L.llrP = nullptr;
     }
@@ -156,7 +164,7 @@ __device__ __forceinline__ WaveLds carve_lds(uint32_t *base, int cap, bool ml) {
 }
 
 // Exact check of a candidate: s ^ XOR_l column(cols[l]) == 0 over all D words.
-__device__ __forceinline__ bool exact_match(const DevCode &code, const uint32_t *s, const int *cols, int w) {
+__device__ __forceinline__ bool exact_match(const DevCode &code, const uint32_t LDS *s, const int *cols, int w) {
     bool ok = true;
 #pragma unroll 1
     for (int d = 0; d < code.D; ++d) {
@@ -172,7 +180,7 @@ __device__ __forceinline__ bool exact_match(const DevCode &code, const uint32_t 
 // Active set for threshold thr: ascending columns with score >= thr, where
 // score[c] = |supp(h_c) & supp(s)| + off[c]  (grand_decoder.cu:352-416, :495-504).
 // Writes the first min(tam, cap) entries; returns the full tam.
-__device__ int build_active(const DevCode &code, const uint32_t *s, int thr, const WaveLds &L, int cap) {
+__device__ int build_active(const DevCode &code, const uint32_t LDS *s, int thr, const WaveLds &L, int cap) {
     const int lane = lane_id();
     int tam = 0;
     for (int c0 = 0; c0 < code.m; c0 += 64) {
@@ -226,8 +234,9 @@ struct ScanHit {
 };
 
 template <int W, int MODE>
-__device__ __forceinline__ ScanHit scan_range(const DevCode &code, const uint32_t *s, uint32_t t0, uint64_t rank0,
-                                              int len, int tam, const int *act, const uint32_t *sg, const float *llr) {
+__device__ __forceinline__ ScanHit scan_range(const DevCode &code, const uint32_t LDS *s, uint32_t t0, uint64_t rank0,
+                                              int len, int tam, const int LDS *act, const uint32_t LDS *sg,
+                                              const float LDS *llr) {
     ScanHit h{-1, 0.0f};
     if (len <= 0) return h;
     int c[W];
@@ -273,9 +282,10 @@ __device__ __forceinline__ ScanHit scan_range(const DevCode &code, const uint32_
 }
 
 template <int MODE>
-__device__ __attribute__((noinline)) ScanHit scan_range_w(int w, const DevCode &code, const uint32_t *s, uint32_t t0,
-                                                          uint64_t rank0, int len, int tam, const int *act,
-                                                          const uint32_t *sg, const float *llr) {
+__device__ __attribute__((noinline)) ScanHit scan_range_w(int w, const DevCode &code, const uint32_t LDS *s,
+                                                          uint32_t t0, uint64_t rank0, int len, int tam,
+                                                          const int LDS *act, const uint32_t LDS *sg,
+                                                          const float LDS *llr) {
     switch (w) {
         case 1: return scan_range<1, MODE>(code, s, t0, rank0, len, tam, act, sg, llr);
         case 2: return scan_range<2, MODE>(code, s, t0, rank0, len, tam, act, sg, llr);
@@ -287,12 +297,18 @@ __device__ __attribute__((noinline)) ScanHit scan_range_w(int w, const DevCode &
 }
 
 // ---------------------------------------------------------------- shared tasks
-// ctl word: generation (20 bits) | nchunks (22 bits) | next chunk (22 bits).
-constexpr unsigned long long kNextMask = (1ull << 22) - 1;
-constexpr unsigned kMaxChunks = (1u << 22) - 1;
-__device__ __forceinline__ unsigned ctl_next(unsigned long long c) { return (unsigned)(c & kNextMask); }
-__device__ __forceinline__ unsigned ctl_nch(unsigned long long c) { return (unsigned)((c >> 22) & kNextMask); }
-__device__ __forceinline__ unsigned ctl_gen(unsigned long long c) { return (unsigned)(c >> 44); }
+// ctl word: generation (12 bits) | nchunks (20 bits) | next chunk (32 bits).
+// Chunks are claimed with one fetch-and-add on ctl: a returned `next` below
+// `nchunks` is a claim of that chunk of generation `gen`; anything else claims
+// nothing (the counter may run past nchunks by at most one step per wave, far
+// from carrying into the nchunks field).
+constexpr unsigned kMaxChunks = (1u << 20) - 1;
+__device__ __forceinline__ unsigned ctl_next(unsigned long long c) { return (unsigned)c; }
+__device__ __forceinline__ unsigned ctl_nch(unsigned long long c) { return (unsigned)((c >> 32) & 0xFFFFFu); }
+__device__ __forceinline__ unsigned ctl_gen(unsigned long long c) { return (unsigned)(c >> 52); }
+__device__ __forceinline__ unsigned long long make_ctl(unsigned gen, unsigned nch) {
+    return ((unsigned long long)(gen & 0xFFFu) << 52) | ((unsigned long long)nch << 32);
+}
 
 template <class T>
 __device__ __forceinline__ T ld_relaxed(T *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
@@ -305,31 +321,27 @@ __device__ __forceinline__ float key_to_float(uint32_t key) {
 }
 
 // Claims the next chunk of slot T for this wave; returns it or ~0u.  Lane 0
-// issues the CAS; every exit depends only on wave-uniform values.
+// issues the fetch-and-add; every exit depends only on wave-uniform values.
 __device__ __forceinline__ unsigned claim_chunk(TaskSlot *T, unsigned *open_count, unsigned &gen) {
-    const bool leader = lane_id() == 0;
-    unsigned long long snap = uni64(ld_acquire(&T->ctl));
-    for (;;) {
-        if (ctl_next(snap) >= ctl_nch(snap)) { gen = 0; return ~0u; }
-        unsigned long long prev = 0;
-        if (leader) prev = atomicCAS(&T->ctl, snap, snap + 1ull);
-        prev = uni64(prev);
-        if (prev == snap) {
-            const unsigned ch = ctl_next(snap);
-            if (leader && ch + 1u == ctl_nch(snap)) atomicSub(open_count, 1u);
-            gen = ctl_gen(snap);
-            return ch;
-        }
-        snap = prev;
-    }
+    gen = 0;
+    const unsigned long long snap = uni64(ld_relaxed(&T->ctl));
+    if (ctl_next(snap) >= ctl_nch(snap)) return ~0u;
+    unsigned long long old = 0;
+    if (lane_id() == 0) old = __hip_atomic_fetch_add(&T->ctl, 1ull, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    old = uni64(old);
+    if (ctl_next(old) >= ctl_nch(old)) return ~0u;
+    const unsigned ch = ctl_next(old);
+    if (lane_id() == 0 && ch + 1u == ctl_nch(old)) atomicSub(open_count, 1u);
+    gen = ctl_gen(old);
+    return ch;
 }
 
 // One chunk (64 lanes x R ranks) of a shared class; the class's arrays are in
 // this wave's LDS.  First-hit: atomicMin of the lowest matching rank, chunks
 // past a known hit are skipped.  ML: atomicMin of (cost key << 32 | rank).
-__device__ unsigned long long task_chunk(const DevCode &code, TaskSlot *T, const uint32_t *s, uint32_t t0, int w,
-                                        int tam, int R, uint64_t Nw, bool ml, unsigned ch, const int *act,
-                                        const uint32_t *sg, const float *llr) {
+__device__ unsigned long long task_chunk(const DevCode &code, TaskSlot *T, const uint32_t LDS *s, uint32_t t0, int w,
+                                        int tam, int R, uint64_t Nw, bool ml, unsigned ch, const int LDS *act,
+                                        const uint32_t LDS *sg, const float LDS *llr) {
     const int lane = lane_id();
     const uint64_t base = (uint64_t)ch * 64ull * (uint64_t)R;
     unsigned long long out = ~0ull;
@@ -362,7 +374,7 @@ __device__ unsigned long long task_chunk(const DevCode &code, TaskSlot *T, const
 
 // Debug trace of shared chunks (enabled by the host only for diagnosis).
 __device__ void trace_chunk(const DevCall &call, int who, int slot, unsigned gen, unsigned ch, int w, int tam, int R,
-                            uint64_t Nw, uint32_t t0, const uint32_t *s, const int *act, const uint32_t *sg,
+                            uint64_t Nw, uint32_t t0, const uint32_t LDS *s, const int LDS *act, const uint32_t LDS *sg,
                             unsigned long long res) {
     const unsigned active = (unsigned)__popcll(ballot(true));
     if (!call.dbg || lane_id() != 0) return;
@@ -371,15 +383,16 @@ __device__ void trace_chunk(const DevCall &call, int who, int slot, unsigned gen
     unsigned *r = call.dbg + 16u * (i + 1u);
     r[0] = (unsigned)who; r[1] = (unsigned)slot; r[2] = gen; r[3] = ch; r[4] = (unsigned)w; r[5] = (unsigned)tam;
     r[6] = (unsigned)R; r[7] = (unsigned)Nw; r[8] = t0; r[9] = s[0]; r[10] = (unsigned)act[0]; r[11] = sg[0];
-    r[12] = (unsigned)res; r[13] = (unsigned)(res >> 32); r[14] = active; r[15] = sg[1];
+    r[12] = (unsigned)res; r[13] = (unsigned)(res >> 32); r[14] = blockIdx.x | (active << 24);
+    r[15] = (unsigned)now_ticks();
 }
 
 // Owner side: publish the weight-w class over act/sg/llr (this wave's LDS) as a
 // shared task, work on it, wait for every chunk and return the class result in
 // best (~0 = no match).  Returns false when no slot is free (caller searches alone).
-__device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, const uint32_t *s, uint32_t t0, int w,
-                           int tam, uint64_t Nw, bool ml, const int *act, const uint32_t *sg, const float *llr,
-                           unsigned long long &best) {
+__device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, const uint32_t LDS *s, uint32_t t0,
+                           int w, int tam, uint64_t Nw, bool ml, const int LDS *act, const uint32_t LDS *sg,
+                           const float LDS *llr, unsigned long long &best) {
     const int lane = lane_id();
     int R = call.rmax;
     const uint64_t per = 64ull * (uint64_t)kMaxChunks;
@@ -408,17 +421,19 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
         arr[call.cap + i] = sg[i];
         if (ml) arr[2 * call.cap + i] = __float_as_uint(llr[i]);
     }
-    const unsigned gen = (ctl_gen(ld_relaxed(&T->ctl)) + 1u) & 0xFFFFFu;
+    const unsigned gen = (ctl_gen(ld_relaxed(&T->ctl)) + 1u) & 0xFFFu;
+    unsigned uid = 0;
+    if (lane == 0) uid = atomicAdd(&call.task_counters[2], 1u);
+    uid = uni(uid);
     if (lane == 0) {
-        T->gen = gen; T->nchunks = nch; T->shot = shot; T->w = w; T->tam = tam; T->R = R; T->ml = ml ? 1 : 0;
+        T->gen = uid; T->nchunks = nch; T->shot = shot; T->w = w; T->tam = tam; T->R = R; T->ml = ml ? 1 : 0;
         T->t0 = t0; T->nw = Nw; T->done = 0u; T->best = ~0ull;
 #pragma unroll
         for (int d = 0; d < kMaxWords; ++d) T->s[d] = s[d];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     if (lane == 0) {
-        __hip_atomic_exchange(&T->ctl, ((unsigned long long)gen << 44) | ((unsigned long long)nch << 22),
-                              __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_exchange(&T->ctl, make_ctl(gen, nch), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         atomicAdd(&call.task_counters[0], 1u);
     }
     for (;;) {
@@ -456,14 +471,15 @@ __device__ bool help_task(const DevCode &code, const DevCall &call, const WaveLd
     if (ch == ~0u) return true;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     const int w = ld_relaxed(&T->w), tam = ld_relaxed(&T->tam), R = ld_relaxed(&T->R);
+    gen = ld_relaxed(&T->gen);   // launch-unique task id: the LDS cache key
     const bool ml = ld_relaxed(&T->ml) != 0;
     const uint64_t Nw = ld_relaxed(&T->nw);
     const uint32_t t0 = ld_relaxed(&T->t0);
     if (lane < kMaxWords) L.sw[lane] = ld_relaxed(&T->s[lane]);
     __syncthreads();
-    const uint32_t *s = L.sw;
-    int *act = ml ? L.P : L.A;
-    uint32_t *sg = ml ? L.sigP : L.sigA;
+    const uint32_t LDS *s = L.sw;
+    int LDS *act = ml ? L.P : L.A;
+    uint32_t LDS *sg = ml ? L.sigP : L.sigA;
     if (cslot != slot || cgen != gen) {
         const uint32_t *arr = call.task_arrays + (size_t)slot * 3 * call.cap;
         for (int i = lane; i < tam; i += 64) {
@@ -499,7 +515,7 @@ __device__ __forceinline__ uint64_t weighted_prefix(const uint64_t *N, const int
 
 // Weight list wl[0..nw) searched in order over A[0..tam); on a hit writes the
 // decoded columns to win and returns true; count gets the patterns tested.
-__device__ bool first_hit_wave(const DevCode &code, const DevCall &call, int shot, const uint32_t *s, uint32_t t0,
+__device__ bool first_hit_wave(const DevCode &code, const DevCall &call, int shot, const uint32_t LDS *s, uint32_t t0,
                                int tam, const int *wl, int nw, const WaveLds &L, int *win, int &fw,
                                unsigned long long &count, unsigned long long &wops) {
     const int lane = lane_id();
@@ -531,11 +547,17 @@ __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, int sho
             }
             pos = class_end;
         } else if (rem_class <= 128u) {
-            // one pattern per lane, 64 consecutive flattened indices (may span classes)
+            // one pattern per lane, 64 consecutive flattened indices; may run into
+            // the following classes, but never into one that will be shared
+            uint64_t flat_end = class_end;
+            {
+                int k2 = kc + 1;
+                while (k2 < nw && (call.split_threshold == 0 || N[k2] <= call.split_threshold)) flat_end += N[k2++];
+            }
             const uint64_t idx = pos + (uint64_t)lane;
             bool cand = false;
             int w = 0;
-            if (idx < total) {
+            if (idx < flat_end) {
                 uint64_t cs = class_start;
                 int k = kc;
                 while (idx >= cs + N[k]) { cs += N[k]; ++k; }
@@ -554,7 +576,7 @@ __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, int sho
             const uint64_t b = ballot(cand);
             if (b) { hit_lane = __ffsll((long long)b) - 1; hit_idx = pos + (uint64_t)hit_lane; hit_w = bcast_i(w, hit_lane); }
             pos += 64;
-            if (pos > total) pos = total;
+            if (pos > flat_end) pos = flat_end;
         } else {
             uint64_t r64 = rem_class / 64u;
             const int R = (int)(r64 < (uint64_t)rmax ? r64 : (uint64_t)rmax);   // >= 2 here
@@ -593,7 +615,7 @@ __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, int sho
 // grand_decoder.cu:590-666 over the LLR-sorted active list P: weights 1..max_w
 // in order, stop before weight w when a best exists and prefix[w] >= best, keep
 // the first strictly cheapest match (fp32 costs summed in pattern order).
-__device__ bool ml_wave(const DevCode &code, const DevCall &call, int shot, const uint32_t *s, uint32_t t0, int tam,
+__device__ bool ml_wave(const DevCode &code, const DevCall &call, int shot, const uint32_t LDS *s, uint32_t t0, int tam,
                         const WaveLds &L, int *win, int &fw, unsigned long long &count, unsigned long long &wops) {
     const int lane = lane_id();
     sort_by_llr(code, L, tam);
@@ -796,7 +818,7 @@ __device__ void decode_shot_wave(const DevCode &code, const DevCall &call, int s
     if (lane_id() < kMaxWords)
         L.sw[lane_id()] = (lane_id() < code.D) ? call.s_packed[(size_t)shot * code.D + lane_id()] : 0u;
     __syncthreads();
-    const uint32_t *s = L.sw;
+    const uint32_t LDS *s = L.sw;
     const uint32_t t0 = call.s_sig[shot];
 
     o.found = 0; o.fw = 0; o.count = 0; o.wops = 0; o.hibrido = -2;
@@ -847,9 +869,8 @@ __device__ void decode_shot_wave(const DevCode &code, const DevCall &call, int s
 __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ codep, const DevCall *__restrict__ callp) {
     const DevCode &code = *codep;
     const DevCall &call = *callp;
-    extern __shared__ uint32_t lds_raw[];
     const int lane = lane_id();
-    uint32_t *ew = lds_raw;     // kMaxM / 32 words
+    uint32_t LDS *ew = lds_base();     // kMaxM / 32 words
     const int groups = (call.num_shots + 63) / 64;
     unsigned bad = 0;
     for (int g = blockIdx.x; g < groups; g += gridDim.x) {
@@ -958,8 +979,7 @@ __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ 
 __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ codep, const DevCall *__restrict__ callp) {
     const DevCode &code = *codep;
     const DevCall &call = *callp;
-    extern __shared__ uint32_t lds_raw[];
-    const WaveLds L = carve_lds(lds_raw, call.cap, call.kind == kML);
+    const WaveLds L = carve_lds(call.cap, call.kind == kML);
     const unsigned total = call.counters[0];
     const unsigned B = (unsigned)call.claim_batch;
     int cslot = -1;
@@ -1006,20 +1026,26 @@ __global__ __launch_bounds__(64) void plain_active_kernel(const DevCode *__restr
                                                           float *gllrP, PlainState *st) {
     const DevCode &code = *codep;
     const DevCall &call = *callp;
-    WaveLds L;
-    L.A = gA; L.sigA = gsig; L.llrA = gllr; L.P = gP; L.sigP = gsigP; L.llrP = gllrP; L.ew = nullptr; L.sw = nullptr;
+    const int lane = lane_id();
+    const WaveLds L = carve_lds(call.cap, true);
     if (sort_tam > 0) {
+        for (int i = lane; i < sort_tam; i += 64) { L.A[i] = gA[i]; L.sigA[i] = gsig[i]; }
+        __syncthreads();
         sort_by_llr(code, L, sort_tam);
-        if (lane_id() == 0) {
+        for (int i = lane; i < sort_tam; i += 64) { gP[i] = L.P[i]; gsigP[i] = L.sigP[i]; gllrP[i] = L.llrP[i]; }
+        if (lane == 0) {
             float p = 0.0f;
             st->prefix[0] = 0.0f;
-            for (int w = 1; w <= kMaxW; ++w) { if (w <= sort_tam) p = __fadd_rn(p, gllrP[w - 1]); st->prefix[w] = p; }
+            for (int w = 1; w <= kMaxW; ++w) { if (w <= sort_tam) p = __fadd_rn(p, L.llrP[w - 1]); st->prefix[w] = p; }
         }
         return;
     }
-    const uint32_t *s = call.s_packed + (size_t)shot * code.D;
-    const int tam = build_active(code, s, thr, L, call.cap);
-    if (lane_id() == 0) { st->tam_new = tam; st->best = ~0ull; }
+    if (lane < kMaxWords) L.sw[lane] = (lane < code.D) ? call.s_packed[(size_t)shot * code.D + lane] : 0u;
+    __syncthreads();
+    const int tam = build_active(code, L.sw, thr, L, call.cap);
+    const int keep = tam < call.cap ? tam : call.cap;
+    for (int i = lane; i < keep; i += 64) { gA[i] = L.A[i]; gsig[i] = L.sigA[i]; }
+    if (lane == 0) { st->tam_new = tam; st->best = ~0ull; }
 }
 
 // One wave per chunk of 64 * R ranks of the weight-w class (grid-stride).
@@ -1038,7 +1064,16 @@ __global__ __launch_bounds__(64) void plain_class_kernel(const DevCode *__restri
     const DevCode &code = *codep;
     const DevCall &call = *callp;
     const int lane = lane_id();
-    const uint32_t *s = call.s_packed + (size_t)shot * code.D;
+    // stage the class's arrays and the syndrome in LDS
+    const WaveLds L = carve_lds(call.cap, true);
+    if (lane < kMaxWords) L.sw[lane] = (lane < code.D) ? call.s_packed[(size_t)shot * code.D + lane] : 0u;
+    for (int i = lane; i < tam; i += 64) {
+        L.P[i] = gA[i];
+        L.sigP[i] = gsig[i];
+        if (ml) L.llrP[i] = gllr[i];
+    }
+    __syncthreads();
+    const uint32_t LDS *s = L.sw;
     const uint32_t t0 = call.s_sig[shot];
     const uint64_t Nw = ncr(code.nCr, tam, w);
     uint32_t bkey = 0xFFFFFFFFu;
@@ -1049,7 +1084,7 @@ __global__ __launch_bounds__(64) void plain_class_kernel(const DevCode *__restri
         const uint64_t lstart = base + (uint64_t)lane * (uint64_t)R;
         const int len = lstart >= Nw ? 0 : (int)((Nw - lstart) < (uint64_t)R ? (Nw - lstart) : (uint64_t)R);
         if (!ml) {
-            const int j = scan_range_w<0>(w, code, s, t0, lstart, len, tam, gA, gsig, nullptr).j;
+            const int j = scan_range_w<0>(w, code, s, t0, lstart, len, tam, L.P, L.sigP, nullptr).j;
             const uint64_t b = ballot(j >= 0);
             if (b) {
                 const int hl = __ffsll((long long)b) - 1;
@@ -1058,7 +1093,7 @@ __global__ __launch_bounds__(64) void plain_class_kernel(const DevCode *__restri
                 break;
             }
         } else {
-            const ScanHit h = scan_range_w<1>(w, code, s, t0, lstart, len, tam, gA, gsig, gllr);
+            const ScanHit h = scan_range_w<1>(w, code, s, t0, lstart, len, tam, L.P, L.sigP, L.llrP);
             const uint32_t key = (h.j >= 0) ? order_key(h.cost) : 0xFFFFFFFFu;
             const uint32_t kmin = wave_min(key);
             const uint64_t b = ballot(h.j >= 0 && key == kmin);
@@ -1178,8 +1213,8 @@ hipError_t grand_launch_plain_active(const DevCode *code, const DevCall *call, i
     int *gP = (int *)(gllr + cap);
     uint32_t *gsigP = (uint32_t *)(gP + cap);
     float *gllrP = (float *)(gsigP + cap);
-    hipLaunchKernelGGL(grand::plain_active_kernel, dim3(1), dim3(64), 0, st, code, call, shot, thr, sort_tam, gA,
-                       gsig, gllr, gP, gsigP, gllrP, (grand::PlainState *)state);
+    hipLaunchKernelGGL(grand::plain_active_kernel, dim3(1), dim3(64), grand_fused_lds_bytes(cap, 1), st, code, call,
+                       shot, thr, sort_tam, gA, gsig, gllr, gP, gsigP, gllrP, (grand::PlainState *)state);
     return hipGetLastError();
 }
 
@@ -1192,9 +1227,9 @@ hipError_t grand_launch_plain_class(const DevCode *code, const DevCall *call, in
     int *gP = (int *)(gllr + cap);
     uint32_t *gsigP = (uint32_t *)(gP + cap);
     float *gllrP = (float *)(gsigP + cap);
-    hipLaunchKernelGGL(grand::plain_class_kernel, dim3(grid), dim3(64), 0, st, code, call, shot, w, tam, R, nchunks,
-                       ml, ml ? gP : gA, ml ? gsigP : gsig, gllrP, (grand::PlainState *)state,
-                       (grand::MlRecord *)records);
+    hipLaunchKernelGGL(grand::plain_class_kernel, dim3(grid), dim3(64), grand_fused_lds_bytes(cap, 1), st, code, call,
+                       shot, w, tam, R, nchunks, ml, ml ? gP : gA, ml ? gsigP : gsig, gllrP,
+                       (grand::PlainState *)state, (grand::MlRecord *)records);
     return hipGetLastError();
 }
 
