@@ -787,8 +787,9 @@ __device__ void finalize_shot(const DevCode &code, const DevCall &call, int shot
         }
     } else if (call.family == kCLN) {
         const int NL32 = code.NL32;
+        const int nlw = (code.NL + 31) / 32;     // words that hold logicals (0 when n_logicos = 0)
         bool same = true;
-        for (int wd = lane; wd < NL32; wd += 64) {
+        for (int wd = lane; wd < nlw; wd += 64) {
             uint32_t le = 0;
             if (o.found) {
 #pragma unroll
