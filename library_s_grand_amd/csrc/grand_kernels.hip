@@ -128,8 +128,10 @@ __device__ __forceinline__ void unrank_dyn(uint64_t r, int tam, int w, const uns
 }
 
 // ------------------------------------------------------------- wave shot state
-// Per-wave LDS layout (cap entries each): A (active columns), sig of A, and for
-// ML the LLR-sorted copies P / sigP / llrP plus the unsorted llr of A.
+// Per-wave LDS layout (cap entries each): A (active columns) and their
+// signatures; for ML also the llr of A and the LLR-sorted copies P / sigP /
+// llrP; then the signature hash table (htcap = power of two >= 2 cap entries of
+// (index, signature)).
 struct WaveLds {
     int LDS *A;
     uint32_t LDS *sigA;
@@ -137,6 +139,7 @@ struct WaveLds {
     int LDS *P;
     uint32_t LDS *sigP;
     float LDS *llrP;
+    unsigned long long LDS *ht;   // (signature << 32) | position, empty = kHtEmpty
     uint32_t LDS *ew;  // packed error words of the shot (triage), kMaxM / 32
     uint32_t LDS *sw;  // syndrome words of the shot being searched, kMaxWords
 };
@@ -144,12 +147,19 @@ struct WaveLds {
 extern __shared__ uint32_t lds_raw[];
 __device__ __forceinline__ uint32_t LDS *lds_base() { return (uint32_t LDS *)lds_raw; }
 
+__host__ __device__ __forceinline__ int ht_capacity(int cap) {
+    int h = 64;
+    while (h < 2 * cap) h <<= 1;
+    return h;
+}
+
 __device__ __forceinline__ WaveLds carve_lds(int cap, bool ml) {
     uint32_t LDS *base = lds_base();
     WaveLds L;
     L.ew = base;
     L.sw = base + kMaxM / 32;
     uint32_t LDS *p = L.sw + kMaxWords;
+    L.ht = (unsigned long long LDS *)p; p += 2 * ht_capacity(cap);
     L.A = (int LDS *)p; p += cap;
     L.sigA = p; p += cap;
     if (ml) {
@@ -223,77 +233,164 @@ __device__ void sort_by_llr(const DevCode &code, const WaveLds &L, int tam) {
     __syncthreads();
 }
 
-// --------------------------------------------------------------- lane scanners
-// A lane walks `len` consecutive ranks of one weight-W class starting at rank0.
-// MODE 0: stop at the first exact match.  MODE 1 (ML): visit all, keep the
-// first strictly cheapest match.  Returns the offset of the hit within the
-// lane's range (-1 = none) and its fp32 cost.
-struct ScanHit {
-    int j;
+// Open-addressing table of (signature, position) for sg[0..tam): slot =
+// signature & mask, linear probing, at most half full.  Equal signatures
+// (equal columns) occupy successive slots of one probe chain.
+constexpr unsigned long long kHtEmpty = 0xFFFFFFFFull;
+__device__ unsigned build_ht(unsigned long long LDS *ht, const uint32_t LDS *sg, int tam) {
+    const int lane = lane_id();
+    const int hsz = ht_capacity(tam);
+    const unsigned mask = (unsigned)hsz - 1u;
+    for (int i = lane; i < hsz; i += 64) ht[i] = kHtEmpty;
+    __syncthreads();
+    for (int i = lane; i < tam; i += 64) {
+        const uint32_t key = sg[i];
+        const unsigned long long ent = ((unsigned long long)key << 32) | (unsigned)i;
+        unsigned h = key & mask;
+        for (;;) {
+            unsigned long long expect = kHtEmpty;
+            if (__hip_atomic_compare_exchange_strong(&ht[h], &expect, ent, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP))
+                break;
+            h = (h + 1u) & mask;
+        }
+    }
+    __syncthreads();
+    return mask;
+}
+
+// ------------------------------------------------------------ prefix scanners
+// A weight-w class is walked as its (w-1)-prefixes, the (w-1)-subsets of
+// [0, tam-1) in lexicographic order; for each prefix the positions x >
+// prefix_last with sig[x] == t0 ^ sig(prefix) come from one hash probe chain,
+// and each is confirmed exactly.  Lexicographic order of (prefix, x) is the
+// order of the w-subsets, so ranks, first-hit and ML tie-breaks are those of a
+// pattern-by-pattern walk.
+
+// Rank of the w-subset (c[0..WP-1], x) of [0, tam), w = WP + 1.
+template <int WP>
+__device__ __forceinline__ uint64_t rank_with_last(const int *c, int x, int tam, const unsigned long long *nCr) {
+    uint64_t r = 0;
+    int prev = -1;
+#pragma unroll
+    for (int i = 0; i < WP; ++i) {
+        const int k = WP + 1 - i;
+        r += nCr[(tam - 1 - prev) * kNcrStride + k] - nCr[(tam - c[i]) * kNcrStride + k];
+        prev = c[i];
+    }
+    return r + (uint64_t)(x - prev - 1);
+}
+
+struct PrefHit {
+    uint64_t rank;   // rank of the hit in its weight class (~0 = none)
     float cost;
+    int j;           // prefix offset of the hit within the lane's range
 };
 
-template <int W, int MODE>
-__device__ __forceinline__ ScanHit scan_range(const DevCode &code, const uint32_t LDS *s, uint32_t t0, uint64_t rank0,
-                                              int len, int tam, const int LDS *act, const uint32_t LDS *sg,
-                                              const float LDS *llr) {
-    ScanHit h{-1, 0.0f};
+template <int WP, int MODE>
+__device__ __forceinline__ PrefHit scan_prefix(const DevCode &code, const uint32_t LDS *s, uint32_t t0, uint64_t q0,
+                                               int len, int tam, const int LDS *act, const uint32_t LDS *sg,
+                                               const float LDS *llr, const unsigned long long LDS *ht, unsigned hmask) {
+    PrefHit h{~0ull, 0.0f, -1};
     if (len <= 0) return h;
-    int c[W];
-    unrank_t<W>(rank0, tam, code.nCr, c);
-    uint32_t acc[W];    // acc[l] = t0 ^ sg[c0] ^ ... ^ sg[c_l]
+    const int M = tam - 1;
+    int c[WP];
+    unrank_t<WP>(q0, M, code.nCr, c);
+    uint32_t acc[WP];   // acc[l] = t0 ^ sg[c0] ^ ... ^ sg[c_l]
     uint32_t a = t0;
 #pragma unroll
-    for (int l = 0; l < W - 1; ++l) { a ^= sg[c[l]]; acc[l] = a; }
-    uint32_t pre = a;   // prefix of the first W-1 elements
+    for (int l = 0; l < WP; ++l) { a ^= sg[c[l]]; acc[l] = a; }
     for (int j = 0; j < len; ++j) {
-        const uint32_t x = pre ^ sg[c[W - 1]];
-        if (x == 0u) {
-            int cols[kMaxW];
+        const uint32_t t = acc[WP - 1];
+        const int last = c[WP - 1];
+        unsigned pos = t & hmask;
+        for (;;) {
+            const unsigned long long e = ht[pos];
+            if (e == kHtEmpty) break;
+            const int x = (int)(uint32_t)e;
+            if ((uint32_t)(e >> 32) == t && x > last) {
+                int cols[kMaxW];
 #pragma unroll
-            for (int l = 0; l < kMaxW; ++l) cols[l] = (l < W) ? act[c[l]] : 0;
-            if (exact_match(code, s, cols, W)) {
-                if (MODE == 0) { h.j = j; break; }
-                float cost = 0.0f;
+                for (int l = 0; l < kMaxW; ++l) cols[l] = (l < WP) ? act[c[l]] : ((l == WP) ? act[x] : 0);
+                if (exact_match(code, s, cols, WP + 1)) {
+                    const uint64_t r = rank_with_last<WP>(c, x, tam, code.nCr);
+                    if (MODE == 0) {
+                        if (r < h.rank) { h.rank = r; h.j = j; }
+                    } else {
+                        float cost = 0.0f;
 #pragma unroll
-                for (int l = 0; l < W; ++l) cost = __fadd_rn(cost, llr[c[l]]);
-                if (h.j < 0 || cost < h.cost) { h.j = j; h.cost = cost; }
+                        for (int l = 0; l < WP; ++l) cost = __fadd_rn(cost, llr[c[l]]);
+                        cost = __fadd_rn(cost, llr[x]);
+                        if (h.j < 0 || cost < h.cost || (cost == h.cost && r < h.rank)) { h.rank = r; h.cost = cost; h.j = j; }
+                    }
+                }
             }
+            pos = (pos + 1u) & hmask;
         }
+        if (MODE == 0 && h.j >= 0) break;
         if (j + 1 == len) break;
-        // lexicographic successor
-        c[W - 1] += 1;
-        if (c[W - 1] >= tam) {
+        // lexicographic successor among WP-subsets of [0, M)
+        c[WP - 1] += 1;
+        if (c[WP - 1] < M) {
+            acc[WP - 1] = ((WP >= 2) ? acc[(WP >= 2) ? WP - 2 : 0] : t0) ^ sg[c[WP - 1]];
+        } else {
             int i = -1;
 #pragma unroll
-            for (int l = 0; l < W - 1; ++l) if (c[l] < tam - W + l) i = l;
+            for (int l = 0; l < WP - 1; ++l) if (c[l] < M - WP + l) i = l;
 #pragma unroll
-            for (int l = 0; l < W; ++l) {
+            for (int l = 0; l < WP; ++l) {
                 if (l == i) c[l] += 1;
                 else if (l > 0 && l > i) c[l] = c[l - 1] + 1;
             }
 #pragma unroll
-            for (int l = 0; l < W - 1; ++l)
-                if (l >= i) acc[l] = ((l == 0) ? t0 : acc[l - 1]) ^ sg[c[l]];
-            pre = (W >= 2) ? acc[(W >= 2) ? W - 2 : 0] : t0;
+            for (int l = 0; l < WP; ++l)
+                if (l >= i) acc[l] = ((l == 0) ? t0 : acc[(l == 0) ? 0 : l - 1]) ^ sg[c[l]];
         }
     }
     return h;
 }
 
 template <int MODE>
-__device__ __attribute__((noinline)) ScanHit scan_range_w(int w, const DevCode &code, const uint32_t LDS *s,
-                                                          uint32_t t0, uint64_t rank0, int len, int tam,
-                                                          const int LDS *act, const uint32_t LDS *sg,
-                                                          const float LDS *llr) {
+__device__ __attribute__((noinline)) PrefHit scan_prefix_w(int w, const DevCode &code, const uint32_t LDS *s,
+                                                           uint32_t t0, uint64_t q0, int len, int tam,
+                                                           const int LDS *act, const uint32_t LDS *sg,
+                                                           const float LDS *llr, const unsigned long long LDS *ht,
+                                                           unsigned hmask) {
     switch (w) {
-        case 1: return scan_range<1, MODE>(code, s, t0, rank0, len, tam, act, sg, llr);
-        case 2: return scan_range<2, MODE>(code, s, t0, rank0, len, tam, act, sg, llr);
-        case 3: return scan_range<3, MODE>(code, s, t0, rank0, len, tam, act, sg, llr);
-        case 4: return scan_range<4, MODE>(code, s, t0, rank0, len, tam, act, sg, llr);
-        case 5: return scan_range<5, MODE>(code, s, t0, rank0, len, tam, act, sg, llr);
-        default: return scan_range<6, MODE>(code, s, t0, rank0, len, tam, act, sg, llr);
+        case 2: return scan_prefix<1, MODE>(code, s, t0, q0, len, tam, act, sg, llr, ht, hmask);
+        case 3: return scan_prefix<2, MODE>(code, s, t0, q0, len, tam, act, sg, llr, ht, hmask);
+        case 4: return scan_prefix<3, MODE>(code, s, t0, q0, len, tam, act, sg, llr, ht, hmask);
+        case 5: return scan_prefix<4, MODE>(code, s, t0, q0, len, tam, act, sg, llr, ht, hmask);
+        default: return scan_prefix<5, MODE>(code, s, t0, q0, len, tam, act, sg, llr, ht, hmask);
     }
+}
+
+// Weight 1: every position with sig == t0, scanned by the wave 64 at a time.
+// MODE 0 returns the lowest matching position; MODE 1 the cheapest (ties: lowest).
+template <int MODE>
+__device__ PrefHit weight_one(const DevCode &code, const uint32_t LDS *s, uint32_t t0, int tam, const int LDS *act,
+                              const uint32_t LDS *sg, const float LDS *llr) {
+    const int lane = lane_id();
+    PrefHit best{~0ull, 0.0f, -1};
+    for (int i0 = 0; i0 < tam; i0 += 64) {
+        const int i = i0 + lane;
+        bool hit = false;
+        if (i < tam && sg[i] == t0) {
+            int cols[kMaxW] = {act[i], 0, 0, 0, 0, 0};
+            hit = exact_match(code, s, cols, 1);
+        }
+        const float cost = (hit && MODE == 1) ? __fadd_rn(0.0f, llr[i]) : 0.0f;
+        const uint32_t key = hit ? order_key(cost) : 0xFFFFFFFFu;
+        const uint32_t kmin = (MODE == 1) ? wave_min(key) : 0u;
+        const uint64_t b = ballot(hit && (MODE == 0 || key == kmin));
+        if (b) {
+            const int bl = __ffsll((long long)b) - 1;
+            const float cb = bcast_f(cost, bl);
+            if (best.j < 0 || (MODE == 1 && cb < best.cost)) { best.rank = (uint64_t)(i0 + bl); best.cost = cb; best.j = 0; }
+            if (MODE == 0) break;
+        }
+    }
+    return best;
 }
 
 // ---------------------------------------------------------------- shared tasks
@@ -336,35 +433,40 @@ __device__ __forceinline__ unsigned claim_chunk(TaskSlot *T, unsigned *open_coun
     return ch;
 }
 
-// One chunk (64 lanes x R ranks) of a shared class; the class's arrays are in
-// this wave's LDS.  First-hit: atomicMin of the lowest matching rank, chunks
-// past a known hit are skipped.  ML: atomicMin of (cost key << 32 | rank).
+// One chunk (64 lanes x R prefixes) of a shared class; the class's arrays and
+// hash table are in this wave's LDS.  First-hit: atomicMin of the lowest
+// matching rank (and of its prefix rank, so later chunks are skipped).  ML:
+// atomicMin of (cost key << 32 | rank).  Returns the chunk's result (~0 = none).
 __device__ unsigned long long task_chunk(const DevCode &code, TaskSlot *T, const uint32_t LDS *s, uint32_t t0, int w,
-                                        int tam, int R, uint64_t Nw, bool ml, unsigned ch, const int LDS *act,
-                                        const uint32_t LDS *sg, const float LDS *llr) {
+                                        int tam, int R, uint64_t NP, bool ml, unsigned ch, const int LDS *act,
+                                        const uint32_t LDS *sg, const float LDS *llr, const unsigned long long LDS *ht,
+                                        unsigned hmask) {
     const int lane = lane_id();
     const uint64_t base = (uint64_t)ch * 64ull * (uint64_t)R;
+    if (!ml && base > uni64(ld_relaxed(&T->best_q))) return ~0ull - 1;
+    const uint64_t q0 = base + (uint64_t)lane * (uint64_t)R;
+    const int len = q0 >= NP ? 0 : (int)((NP - q0) < (uint64_t)R ? (NP - q0) : (uint64_t)R);
+    const PrefHit h = ml ? scan_prefix_w<1>(w, code, s, t0, q0, len, tam, act, sg, llr, ht, hmask)
+                         : scan_prefix_w<0>(w, code, s, t0, q0, len, tam, act, sg, llr, ht, hmask);
     unsigned long long out = ~0ull;
-    if (!ml && base > ld_relaxed(&T->best)) return out - 1;
-    const uint64_t lstart = base + (uint64_t)lane * (uint64_t)R;
-    const int len = lstart >= Nw ? 0 : (int)((Nw - lstart) < (uint64_t)R ? (Nw - lstart) : (uint64_t)R);
     if (!ml) {
-        const int j = scan_range_w<0>(w, code, s, t0, lstart, len, tam, act, sg, nullptr).j;
-        const uint64_t b = ballot(j >= 0);
+        const uint64_t b = ballot(h.j >= 0);
         if (b) {
             const int hl = __ffsll((long long)b) - 1;
-            const uint64_t r = base + (uint64_t)hl * (uint64_t)R + (uint64_t)bcast_i(j, hl);
-            if (lane == 0) atomicMin(&T->best, (unsigned long long)r);
+            const uint64_t r = (uint64_t)__shfl((long long)h.rank, hl, 64);
+            const uint64_t q = q0 + (uint64_t)(h.j < 0 ? 0 : h.j);
+            const uint64_t qh = (uint64_t)__shfl((long long)q, hl, 64);
+            if (lane == 0) { atomicMin(&T->best, (unsigned long long)r); atomicMin(&T->best_q, (unsigned long long)qh); }
             out = r;
         }
     } else {
-        const ScanHit h = scan_range_w<1>(w, code, s, t0, lstart, len, tam, act, sg, llr);
         const uint32_t key = (h.j >= 0) ? order_key(h.cost) : 0xFFFFFFFFu;
         const uint32_t kmin = wave_min(key);
+        // among lanes with the minimal key, the lowest rank (lanes hold ascending ranges)
         const uint64_t b = ballot(h.j >= 0 && key == kmin);
         if (b) {
             const int bl = __ffsll((long long)b) - 1;
-            const uint64_t r = (uint64_t)__shfl((long long)(lstart + (uint64_t)(h.j < 0 ? 0 : h.j)), bl, 64);
+            const uint64_t r = (uint64_t)__shfl((long long)h.rank, bl, 64);
             if (lane == 0) atomicMin(&T->best, ((unsigned long long)kmin << 32) | r);
             out = ((unsigned long long)kmin << 32) | r;
         }
@@ -374,15 +476,15 @@ __device__ unsigned long long task_chunk(const DevCode &code, TaskSlot *T, const
 
 // Debug trace of shared chunks (enabled by the host only for diagnosis).
 __device__ void trace_chunk(const DevCall &call, int who, int slot, unsigned gen, unsigned ch, int w, int tam, int R,
-                            uint64_t Nw, uint32_t t0, const uint32_t LDS *s, const int LDS *act, const uint32_t LDS *sg,
-                            unsigned long long res) {
+                            uint64_t NP, uint32_t t0, const uint32_t LDS *s, const int LDS *act,
+                            const uint32_t LDS *sg, unsigned long long res) {
     const unsigned active = (unsigned)__popcll(ballot(true));
     if (!call.dbg || lane_id() != 0) return;
     const unsigned i = atomicAdd(&call.dbg[0], 1u);
     if (16u * (i + 1u) + 16u > call.dbg_cap) return;
     unsigned *r = call.dbg + 16u * (i + 1u);
     r[0] = (unsigned)who; r[1] = (unsigned)slot; r[2] = gen; r[3] = ch; r[4] = (unsigned)w; r[5] = (unsigned)tam;
-    r[6] = (unsigned)R; r[7] = (unsigned)Nw; r[8] = t0; r[9] = s[0]; r[10] = (unsigned)act[0]; r[11] = sg[0];
+    r[6] = (unsigned)R; r[7] = (unsigned)NP; r[8] = t0; r[9] = s[0]; r[10] = (unsigned)act[0]; r[11] = sg[0];
     r[12] = (unsigned)res; r[13] = (unsigned)(res >> 32); r[14] = blockIdx.x | (active << 24);
     r[15] = (unsigned)now_ticks();
 }
@@ -391,13 +493,14 @@ __device__ void trace_chunk(const DevCall &call, int who, int slot, unsigned gen
 // shared task, work on it, wait for every chunk and return the class result in
 // best (~0 = no match).  Returns false when no slot is free (caller searches alone).
 __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, const uint32_t LDS *s, uint32_t t0,
-                           int w, int tam, uint64_t Nw, bool ml, const int LDS *act, const uint32_t LDS *sg,
-                           const float LDS *llr, unsigned long long &best) {
+                           int w, int tam, uint64_t Nw, uint64_t NP, bool ml, const int LDS *act,
+                           const uint32_t LDS *sg, const float LDS *llr, const unsigned long long LDS *ht, unsigned hmask,
+                           unsigned long long &best) {
     const int lane = lane_id();
     int R = call.rmax;
     const uint64_t per = 64ull * (uint64_t)kMaxChunks;
-    if (Nw / per >= (uint64_t)R) R = (int)(Nw / per + 1);
-    const unsigned nch = (unsigned)((Nw + 64ull * R - 1) / (64ull * R));
+    if (NP / per >= (uint64_t)R) R = (int)(NP / per + 1);
+    const unsigned nch = (unsigned)((NP + 64ull * R - 1) / (64ull * R));
     // find a free slot: 64 candidates per step, lane 0 takes the first free one
     int slot = -1;
     const int start = (int)((blockIdx.x * 131u) & (kTaskSlots - 1));
@@ -427,7 +530,7 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
     uid = uni(uid);
     if (lane == 0) {
         T->gen = uid; T->nchunks = nch; T->shot = shot; T->w = w; T->tam = tam; T->R = R; T->ml = ml ? 1 : 0;
-        T->t0 = t0; T->nw = Nw; T->done = 0u; T->best = ~0ull;
+        T->t0 = t0; T->nw = Nw; T->np = NP; T->done = 0u; T->best = ~0ull; T->best_q = ~0ull;
 #pragma unroll
         for (int d = 0; d < kMaxWords; ++d) T->s[d] = s[d];
     }
@@ -440,8 +543,8 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
         unsigned g;
         const unsigned ch = claim_chunk(T, &call.task_counters[0], g);
         if (ch == ~0u) break;
-        const unsigned long long res = task_chunk(code, T, s, t0, w, tam, R, Nw, ml, ch, act, sg, llr);
-        trace_chunk(call, 0, slot, g, ch, w, tam, R, Nw, t0, s, act, sg, res);
+        const unsigned long long res = task_chunk(code, T, s, t0, w, tam, R, NP, ml, ch, act, sg, llr, ht, hmask);
+        trace_chunk(call, 0, slot, uid, ch, w, tam, R, NP, t0, s, act, sg, res);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         if (lane == 0) __hip_atomic_fetch_add(&T->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -452,8 +555,8 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
 }
 
 // Helper side: claim one chunk of any open task and process it.  The task's
-// arrays are copied into this wave's LDS once per (slot, generation).
-__device__ bool help_task(const DevCode &code, const DevCall &call, const WaveLds &L, int &cslot, unsigned &cgen) {
+// arrays are copied into this wave's LDS and its hash table built once per task.
+__device__ bool help_task(const DevCode &code, const DevCall &call, const WaveLds &L, int &ctask, unsigned &hmask) {
     const int lane = lane_id();
     if (__builtin_amdgcn_readfirstlane(ld_relaxed(&call.task_counters[0])) == 0u) return false;
     int slot = -1;
@@ -466,21 +569,21 @@ __device__ bool help_task(const DevCode &code, const DevCall &call, const WaveLd
     }
     if (slot < 0) return false;
     TaskSlot *T = &call.tasks[slot];
-    unsigned gen;
-    const unsigned ch = claim_chunk(T, &call.task_counters[0], gen);
+    unsigned g;
+    const unsigned ch = claim_chunk(T, &call.task_counters[0], g);
     if (ch == ~0u) return true;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     const int w = ld_relaxed(&T->w), tam = ld_relaxed(&T->tam), R = ld_relaxed(&T->R);
-    gen = ld_relaxed(&T->gen);   // launch-unique task id: the LDS cache key
+    const unsigned uid = uni(ld_relaxed(&T->gen));
     const bool ml = ld_relaxed(&T->ml) != 0;
-    const uint64_t Nw = ld_relaxed(&T->nw);
+    const uint64_t NP = ld_relaxed(&T->np);
     const uint32_t t0 = ld_relaxed(&T->t0);
     if (lane < kMaxWords) L.sw[lane] = ld_relaxed(&T->s[lane]);
     __syncthreads();
     const uint32_t LDS *s = L.sw;
     int LDS *act = ml ? L.P : L.A;
     uint32_t LDS *sg = ml ? L.sigP : L.sigA;
-    if (cslot != slot || cgen != gen) {
+    if (ctask != (int)uid) {
         const uint32_t *arr = call.task_arrays + (size_t)slot * 3 * call.cap;
         for (int i = lane; i < tam; i += 64) {
             act[i] = (int)arr[i];
@@ -488,17 +591,46 @@ __device__ bool help_task(const DevCode &code, const DevCall &call, const WaveLd
             if (ml) L.llrP[i] = __uint_as_float(arr[2 * call.cap + i]);
         }
         __syncthreads();
-        cslot = slot;
-        cgen = gen;
+        hmask = build_ht(L.ht, sg, tam);
+        ctask = (int)uid;
     }
-    const unsigned long long res = task_chunk(code, T, s, t0, w, tam, R, Nw, ml, ch, act, sg, ml ? L.llrP : nullptr);
-    trace_chunk(call, 1, slot, gen, ch, w, tam, R, Nw, t0, s, act, sg, res);
+    const unsigned long long res =
+        task_chunk(code, T, s, t0, w, tam, R, NP, ml, ch, act, sg, ml ? L.llrP : nullptr, L.ht, hmask);
+    trace_chunk(call, 1, slot, uid, ch, w, tam, R, NP, t0, s, act, sg, res);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     if (lane == 0) __hip_atomic_fetch_add(&T->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     return true;
 }
 
-// ------------------------------------------------------------ first-hit search
+// ------------------------------------------------------------ class searches
+constexpr uint64_t kFlatMax = 128;   // classes up to this size go pattern by pattern, one per lane
+
+// First matching rank in the weight-w class over act/sg (hash table ht), or ~0.
+__device__ uint64_t class_first_hit(const DevCode &code, const DevCall &call, int shot, const uint32_t LDS *s,
+                                    uint32_t t0, int w, int tam, uint64_t Nw, const int LDS *act,
+                                    const uint32_t LDS *sg, const unsigned long long LDS *ht, unsigned hmask) {
+    const int lane = lane_id();
+    if (w == 1) return weight_one<0>(code, s, t0, tam, act, sg, nullptr).rank;
+    const uint64_t NP = ncr(code.nCr, tam - 1, w - 1);
+    unsigned long long shared_best;
+    if (call.split_threshold && NP > call.split_threshold &&
+        run_shared(code, call, shot, s, t0, w, tam, Nw, NP, false, act, sg, nullptr, ht, hmask, shared_best))
+        return shared_best;
+    uint64_t pos = 0;
+    while (pos < NP) {
+        const uint64_t rem = NP - pos;
+        const uint64_t r64 = (rem + 63u) / 64u;
+        const int R = (int)(r64 < (uint64_t)call.rmax ? r64 : (uint64_t)call.rmax);
+        const uint64_t q0 = pos + (uint64_t)lane * (uint64_t)R;
+        const int len = q0 >= NP ? 0 : (int)((NP - q0) < (uint64_t)R ? (NP - q0) : (uint64_t)R);
+        const PrefHit h = scan_prefix_w<0>(w, code, s, t0, q0, len, tam, act, sg, nullptr, ht, hmask);
+        const uint64_t b = ballot(h.j >= 0);
+        if (b) return (uint64_t)__shfl((long long)h.rank, __ffsll((long long)b) - 1, 64);
+        pos += 64ull * (uint64_t)R;
+    }
+    return ~0ull;
+}
+
 // Sum of (weight + 1) over the first `upto` patterns of the concatenated classes.
 __device__ __forceinline__ uint64_t weighted_prefix(const uint64_t *N, const int *wl, int nw, uint64_t upto) {
     uint64_t acc = 0, cs = 0;
@@ -513,12 +645,16 @@ __device__ __forceinline__ uint64_t weighted_prefix(const uint64_t *N, const int
     return acc;
 }
 
-// Weight list wl[0..nw) searched in order over A[0..tam); on a hit writes the
-// decoded columns to win and returns true; count gets the patterns tested.
+// ------------------------------------------------------------ first-hit search
+// Weight list wl[0..nw) searched in order over A[0..tam) (grand_decoder.cu:295-350,
+// :534-588); on a hit writes the decoded columns to win and returns true; count
+// gets the patterns tested up to and including the hit.
 __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, int shot, const uint32_t LDS *s, uint32_t t0,
                                int tam, const int *wl, int nw, const WaveLds &L, int *win, int &fw,
                                unsigned long long &count, unsigned long long &wops) {
     const int lane = lane_id();
+    unsigned hmask = 0;          // hash table of A, built before the first class that needs it
+    bool have_table = false;
     uint64_t N[kMaxW];
     uint64_t total = 0;
 #pragma unroll
@@ -526,85 +662,76 @@ __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, int sho
         N[k] = (k < nw) ? ncr(code.nCr, tam, wl[k]) : 0ull;
         total += N[k];
     }
-    uint64_t pos = 0, class_start = 0;
+    uint64_t class_start = 0;
     int kc = 0;
-    while (kc < nw && N[kc] == 0) ++kc;
-    const int rmax = call.rmax;
-    while (pos < total) {
-        const uint64_t class_end = class_start + N[kc];
-        const uint64_t rem_class = class_end - pos;
-        int hit_lane = -1, hit_w = 0;
-        uint64_t hit_idx = 0;
-        int c[kMaxW] = {0, 0, 0, 0, 0, 0};
-        unsigned long long shared_best;
-        if (call.split_threshold && pos == class_start && N[kc] > call.split_threshold &&
-            run_shared(code, call, shot, s, t0, wl[kc], tam, N[kc], false, L.A, L.sigA, nullptr, shared_best)) {
-            if (shared_best != ~0ull) {
-                hit_lane = 0;
-                hit_w = wl[kc];
-                hit_idx = class_start + shared_best;
-                unrank_dyn(shared_best, tam, hit_w, code.nCr, c);
-            }
-            pos = class_end;
-        } else if (rem_class <= 128u) {
-            // one pattern per lane, 64 consecutive flattened indices; may run into
-            // the following classes, but never into one that will be shared
-            uint64_t flat_end = class_end;
-            {
-                int k2 = kc + 1;
-                while (k2 < nw && (call.split_threshold == 0 || N[k2] <= call.split_threshold)) flat_end += N[k2++];
-            }
-            const uint64_t idx = pos + (uint64_t)lane;
-            bool cand = false;
-            int w = 0;
-            if (idx < flat_end) {
-                uint64_t cs = class_start;
-                int k = kc;
-                while (idx >= cs + N[k]) { cs += N[k]; ++k; }
-                w = wl[k];
-                unrank_dyn(idx - cs, tam, w, code.nCr, c);
-                uint32_t x = t0;
+    bool hit = false;
+    uint64_t hit_idx = 0;
+    int hit_w = 0;
+    int c[kMaxW] = {0, 0, 0, 0, 0, 0};
+    int hit_lane = 0;
+    while (kc < nw && !hit) {
+        if (N[kc] == 0) { ++kc; continue; }
+        if (N[kc] <= kFlatMax) {
+            // small classes: one pattern per lane over the flattened index of this
+            // and the following small classes
+            uint64_t flat_end = class_start + N[kc];
+            int kend = kc + 1;
+            while (kend < nw && N[kend] <= kFlatMax) flat_end += N[kend++];
+            for (uint64_t pos = class_start; pos < flat_end && !hit; pos += 64) {
+                const uint64_t idx = pos + (uint64_t)lane;
+                bool cand = false;
+                int w = 0;
+                if (idx < flat_end) {
+                    uint64_t cs = class_start;
+                    int k = kc;
+                    while (idx >= cs + N[k]) { cs += N[k]; ++k; }
+                    w = wl[k];
+                    unrank_dyn(idx - cs, tam, w, code.nCr, c);
+                    uint32_t x = t0;
 #pragma unroll
-                for (int l = 0; l < kMaxW; ++l) if (l < w) x ^= L.sigA[c[l]];
-                if (x == 0u) {
-                    int cols[kMaxW];
+                    for (int l = 0; l < kMaxW; ++l) if (l < w) x ^= L.sigA[c[l]];
+                    if (x == 0u) {
+                        int cols[kMaxW];
 #pragma unroll
-                    for (int l = 0; l < kMaxW; ++l) cols[l] = (l < w) ? L.A[c[l]] : 0;
-                    cand = exact_match(code, s, cols, w);
+                        for (int l = 0; l < kMaxW; ++l) cols[l] = (l < w) ? L.A[c[l]] : 0;
+                        cand = exact_match(code, s, cols, w);
+                    }
+                }
+                const uint64_t b = ballot(cand);
+                if (b) {
+                    hit = true;
+                    hit_lane = __ffsll((long long)b) - 1;
+                    hit_idx = pos + (uint64_t)hit_lane;
+                    hit_w = bcast_i(w, hit_lane);
                 }
             }
-            const uint64_t b = ballot(cand);
-            if (b) { hit_lane = __ffsll((long long)b) - 1; hit_idx = pos + (uint64_t)hit_lane; hit_w = bcast_i(w, hit_lane); }
-            pos += 64;
-            if (pos > flat_end) pos = flat_end;
+            if (!hit) { class_start = flat_end; kc = kend; }
         } else {
-            uint64_t r64 = rem_class / 64u;
-            const int R = (int)(r64 < (uint64_t)rmax ? r64 : (uint64_t)rmax);   // >= 2 here
-            const uint64_t start = pos + (uint64_t)lane * (uint64_t)R;
             const int w = wl[kc];
-            const int j = scan_range_w<0>(w, code, s, t0, start - class_start, R, tam, L.A, L.sigA, nullptr).j;
-            const uint64_t b = ballot(j >= 0);
-            if (b) {
+            if (!have_table && w > 1) { hmask = build_ht(L.ht, L.sigA, tam); have_table = true; }
+            const uint64_t r = class_first_hit(code, call, shot, s, t0, w, tam, N[kc], L.A, L.sigA, L.ht, hmask);
+            if (r != ~0ull) {
+                hit = true;
                 hit_lane = 0;
-                const int hl = __ffsll((long long)b) - 1;
-                hit_idx = pos + (uint64_t)hl * (uint64_t)R + (uint64_t)bcast_i(j, hl);
                 hit_w = w;
-                unrank_dyn(hit_idx - class_start, tam, w, code.nCr, c);   // same values in every lane
+                hit_idx = class_start + r;
+                unrank_dyn(r, tam, w, code.nCr, c);   // same values in every lane
+            } else {
+                class_start += N[kc];
+                ++kc;
             }
-            pos += 64ull * (uint64_t)R;
         }
-        if (hit_lane >= 0) {
-            fw = hit_w;
+    }
+    if (hit) {
+        fw = hit_w;
 #pragma unroll
-            for (int l = 0; l < kMaxW; ++l) {
-                const int cl = bcast_i(c[l], hit_lane);
-                win[l] = (l < hit_w) ? L.A[cl] : -1;
-            }
-            count += hit_idx + 1u;
-            wops += weighted_prefix(N, wl, nw, hit_idx + 1u);
-            return true;
+        for (int l = 0; l < kMaxW; ++l) {
+            const int cl = bcast_i(c[l], hit_lane);
+            win[l] = (l < hit_w) ? L.A[cl] : -1;
         }
-        while (kc < nw && pos >= class_start + N[kc]) { class_start += N[kc]; ++kc; }
+        count += hit_idx + 1u;
+        wops += weighted_prefix(N, wl, nw, hit_idx + 1u);
+        return true;
     }
     count += total;
     wops += weighted_prefix(N, wl, nw, total);
@@ -619,6 +746,7 @@ __device__ bool ml_wave(const DevCode &code, const DevCall &call, int shot, cons
                         const WaveLds &L, int *win, int &fw, unsigned long long &count, unsigned long long &wops) {
     const int lane = lane_id();
     sort_by_llr(code, L, tam);
+    const unsigned hmask = build_ht(L.ht, L.sigP, tam);
     const int max_w = call.W < tam ? call.W : tam;
     float prefix[kMaxW + 1];
     prefix[0] = 0.0f;
@@ -632,49 +760,56 @@ __device__ bool ml_wave(const DevCode &code, const DevCall &call, int shot, cons
         const uint64_t Nw = ncr(code.nCr, tam, w);
         count += Nw;
         wops += Nw * (uint64_t)(w + 1);
-        unsigned long long shared_best;
-        if (call.split_threshold && Nw > call.split_threshold && Nw < (1ull << 32) &&
-            run_shared(code, call, shot, s, t0, w, tam, Nw, true, L.P, L.sigP, L.llrP, shared_best)) {
-            if (shared_best != ~0ull) {
-                const float cb = key_to_float((uint32_t)(shared_best >> 32));
-                if (!have || cb < best) {
-                    int c[kMaxW] = {0, 0, 0, 0, 0, 0};
-                    unrank_dyn(shared_best & 0xFFFFFFFFull, tam, w, code.nCr, c);
-                    have = true;
-                    best = cb;
-                    fw = w;
-#pragma unroll
-                    for (int l = 0; l < kMaxW; ++l) win[l] = (l < w) ? L.P[c[l]] : -1;
+        bool chit = false;
+        float cbest = 0.0f;
+        uint64_t crank = 0;
+        if (w == 1) {
+            const PrefHit h = weight_one<1>(code, s, t0, tam, L.P, L.sigP, L.llrP);
+            if (h.j >= 0) { chit = true; cbest = h.cost; crank = h.rank; }
+        } else {
+            const uint64_t NP = ncr(code.nCr, tam - 1, w - 1);
+            unsigned long long shared_best;
+            if (call.split_threshold && NP > call.split_threshold && Nw < (1ull << 32) &&
+                run_shared(code, call, shot, s, t0, w, tam, Nw, NP, true, L.P, L.sigP, L.llrP, L.ht, hmask,
+                           shared_best)) {
+                if (shared_best != ~0ull) {
+                    chit = true;
+                    cbest = key_to_float((uint32_t)(shared_best >> 32));
+                    crank = shared_best & 0xFFFFFFFFull;
+                }
+            } else {
+                uint64_t pos = 0;
+                while (pos < NP) {
+                    const uint64_t rem = NP - pos;
+                    const uint64_t r64 = (rem + 63u) / 64u;
+                    const int R = (int)(r64 < (uint64_t)call.rmax ? r64 : (uint64_t)call.rmax);
+                    const uint64_t q0 = pos + (uint64_t)lane * (uint64_t)R;
+                    const int len = q0 >= NP ? 0 : (int)((NP - q0) < (uint64_t)R ? (NP - q0) : (uint64_t)R);
+                    const PrefHit h = scan_prefix_w<1>(w, code, s, t0, q0, len, tam, L.P, L.sigP, L.llrP, L.ht, hmask);
+                    const uint32_t key = (h.j >= 0) ? order_key(h.cost) : 0xFFFFFFFFu;
+                    const uint32_t kmin = wave_min(key);
+                    const uint64_t b = ballot(h.j >= 0 && key == kmin);
+                    if (b) {
+                        const int bl = __ffsll((long long)b) - 1;
+                        const float cb = bcast_f(h.cost, bl);
+                        if (!chit || cb < cbest) {   // chunks in rank order: strict keeps the earliest
+                            chit = true;
+                            cbest = cb;
+                            crank = (uint64_t)__shfl((long long)h.rank, bl, 64);
+                        }
+                    }
+                    pos += 64ull * (uint64_t)R;
                 }
             }
-            continue;
         }
-        uint64_t pos = 0;
-        while (pos < Nw) {
-            const uint64_t rem = Nw - pos;
-            uint64_t r64 = (rem + 63u) / 64u;
-            const int R = (int)(r64 < (uint64_t)call.rmax ? r64 : (uint64_t)call.rmax);
-            const uint64_t lstart = (uint64_t)lane * (uint64_t)R;
-            const int len = lstart >= rem ? 0 : (int)((rem - lstart) < (uint64_t)R ? (rem - lstart) : (uint64_t)R);
-            const ScanHit h = scan_range_w<1>(w, code, s, t0, pos + lstart, len, tam, L.P, L.sigP, L.llrP);
-            const uint32_t key = (h.j >= 0) ? order_key(h.cost) : 0xFFFFFFFFu;
-            const uint32_t kmin = wave_min(key);
-            const uint64_t b = ballot(h.j >= 0 && key == kmin);
-            if (b) {
-                const int bl = __ffsll((long long)b) - 1;
-                const float cbest = bcast_f(h.cost, bl);
-                if (!have || cbest < best) {
-                    const uint64_t r = (uint64_t)__shfl((long long)(pos + lstart + (uint64_t)(h.j < 0 ? 0 : h.j)), bl, 64);
-                    int c[kMaxW] = {0, 0, 0, 0, 0, 0};
-                    unrank_dyn(r, tam, w, code.nCr, c);
-                    have = true;
-                    best = cbest;
-                    fw = w;
+        if (chit && (!have || cbest < best)) {
+            int c[kMaxW] = {0, 0, 0, 0, 0, 0};
+            unrank_dyn(crank, tam, w, code.nCr, c);
+            have = true;
+            best = cbest;
+            fw = w;
 #pragma unroll
-                    for (int l = 0; l < kMaxW; ++l) win[l] = (l < w) ? L.P[c[l]] : -1;
-                }
-            }
-            pos += 64ull * (uint64_t)R;
+            for (int l = 0; l < kMaxW; ++l) win[l] = (l < w) ? L.P[c[l]] : -1;
         }
     }
     return have;
@@ -983,11 +1118,11 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
     const WaveLds L = carve_lds(call.cap, call.kind == kML);
     const unsigned total = call.counters[0];
     const unsigned B = (unsigned)call.claim_batch;
-    int cslot = -1;
-    unsigned cgen = 0;
+    int ctask = -1;        // task whose arrays and hash table this wave's LDS holds
+    unsigned hmask = 0;
     for (;;) {
         // shared classes of deep shots first: they bound the batch's completion time
-        if (call.split_threshold && help_task(code, call, L, cslot, cgen)) continue;
+        if (call.split_threshold && help_task(code, call, L, ctask, hmask)) continue;
         unsigned first = uni(ld_relaxed(&call.counters[1]));
         if (first < total) {
             unsigned got = 0;
@@ -998,7 +1133,7 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
             if (lane_id() == 0) atomicAdd(&call.task_counters[1], 1u);
             const unsigned last = (first + B < total) ? first + B : total;
             for (unsigned i = first; i < last; ++i) decode_shot_wave(code, call, call.worklist[i], L);
-            cslot = -1;
+            ctask = -1;
             if (lane_id() == 0) atomicSub(&call.task_counters[1], 1u);
             continue;
         }
@@ -1008,6 +1143,79 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
         const unsigned open = __builtin_amdgcn_readfirstlane(ld_relaxed(&call.task_counters[0]));
         if (inflight == 0u && open == 0u) break;
         __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// --------------------------------------------------------------- lane scanners
+// A lane walks `len` consecutive ranks of one weight-W class starting at rank0.
+// MODE 0: stop at the first exact match.  MODE 1 (ML): visit all, keep the
+// first strictly cheapest match.  Returns the offset of the hit within the
+// lane's range (-1 = none) and its fp32 cost.
+struct ScanHit {
+    int j;
+    float cost;
+};
+
+template <int W, int MODE>
+__device__ __forceinline__ ScanHit scan_range(const DevCode &code, const uint32_t LDS *s, uint32_t t0, uint64_t rank0,
+                                              int len, int tam, const int LDS *act, const uint32_t LDS *sg,
+                                              const float LDS *llr) {
+    ScanHit h{-1, 0.0f};
+    if (len <= 0) return h;
+    int c[W];
+    unrank_t<W>(rank0, tam, code.nCr, c);
+    uint32_t acc[W];    // acc[l] = t0 ^ sg[c0] ^ ... ^ sg[c_l]
+    uint32_t a = t0;
+#pragma unroll
+    for (int l = 0; l < W - 1; ++l) { a ^= sg[c[l]]; acc[l] = a; }
+    uint32_t pre = a;   // prefix of the first W-1 elements
+    for (int j = 0; j < len; ++j) {
+        const uint32_t x = pre ^ sg[c[W - 1]];
+        if (x == 0u) {
+            int cols[kMaxW];
+#pragma unroll
+            for (int l = 0; l < kMaxW; ++l) cols[l] = (l < W) ? act[c[l]] : 0;
+            if (exact_match(code, s, cols, W)) {
+                if (MODE == 0) { h.j = j; break; }
+                float cost = 0.0f;
+#pragma unroll
+                for (int l = 0; l < W; ++l) cost = __fadd_rn(cost, llr[c[l]]);
+                if (h.j < 0 || cost < h.cost) { h.j = j; h.cost = cost; }
+            }
+        }
+        if (j + 1 == len) break;
+        // lexicographic successor
+        c[W - 1] += 1;
+        if (c[W - 1] >= tam) {
+            int i = -1;
+#pragma unroll
+            for (int l = 0; l < W - 1; ++l) if (c[l] < tam - W + l) i = l;
+#pragma unroll
+            for (int l = 0; l < W; ++l) {
+                if (l == i) c[l] += 1;
+                else if (l > 0 && l > i) c[l] = c[l - 1] + 1;
+            }
+#pragma unroll
+            for (int l = 0; l < W - 1; ++l)
+                if (l >= i) acc[l] = ((l == 0) ? t0 : acc[l - 1]) ^ sg[c[l]];
+            pre = (W >= 2) ? acc[(W >= 2) ? W - 2 : 0] : t0;
+        }
+    }
+    return h;
+}
+
+template <int MODE>
+__device__ __attribute__((noinline)) ScanHit scan_range_w(int w, const DevCode &code, const uint32_t LDS *s,
+                                                          uint32_t t0, uint64_t rank0, int len, int tam,
+                                                          const int LDS *act, const uint32_t LDS *sg,
+                                                          const float LDS *llr) {
+    switch (w) {
+        case 1: return scan_range<1, MODE>(code, s, t0, rank0, len, tam, act, sg, llr);
+        case 2: return scan_range<2, MODE>(code, s, t0, rank0, len, tam, act, sg, llr);
+        case 3: return scan_range<3, MODE>(code, s, t0, rank0, len, tam, act, sg, llr);
+        case 4: return scan_range<4, MODE>(code, s, t0, rank0, len, tam, act, sg, llr);
+        case 5: return scan_range<5, MODE>(code, s, t0, rank0, len, tam, act, sg, llr);
+        default: return scan_range<6, MODE>(code, s, t0, rank0, len, tam, act, sg, llr);
     }
 }
 
@@ -1175,7 +1383,8 @@ using grand::DevCall;
 using grand::DevCode;
 
 size_t grand_fused_lds_bytes(int cap, int ml) {
-    return sizeof(uint32_t) * ((size_t)grand::kMaxM / 32 + grand::kMaxWords + (size_t)cap * (ml ? 6 : 2));
+    return sizeof(uint32_t) * ((size_t)grand::kMaxM / 32 + grand::kMaxWords + 2 * (size_t)grand::ht_capacity(cap) +
+                               (size_t)cap * (ml ? 6 : 2));
 }
 
 hipError_t grand_launch_fill(unsigned *p, unsigned v, int n, hipStream_t st) {

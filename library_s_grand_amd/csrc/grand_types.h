@@ -41,21 +41,22 @@ struct DevCode {
 };
 
 // A weight class too large for one wave, shared through the task table.  The
-// owner wave publishes it; any wave claims chunks of 64 * R ranks with a CAS on
-// ctl = (generation << 32) | next_chunk, so a slot reused for a later task can
-// never hand out a chunk of the earlier one.
+// owner wave publishes it; any wave claims chunks of 64 * R prefixes with one
+// fetch-and-add on ctl = (generation << 52) | (nchunks << 32) | next_chunk.
 constexpr int kTaskSlots = 1024;
 struct TaskSlot {
-    unsigned long long ctl;           // (gen << 32) | next chunk to claim
+    unsigned long long ctl;           // generation | nchunks | next chunk to claim
     unsigned long long best;          // first-hit: lowest matching rank; ML: (cost key << 32) | rank
+    unsigned long long best_q;        // first-hit: prefix rank of that hit (later chunks are skipped)
+    unsigned long long nw;            // class size C(tam, w)
+    unsigned long long np;            // prefixes C(tam - 1, w - 1)
     unsigned done;                    // chunks finished
     unsigned busy;                    // 0 free, 1 owned
-    unsigned gen;                     // generation the fields below belong to
+    unsigned gen;                     // launch-unique task id (helpers' LDS cache key)
     unsigned nchunks;
     int shot, w, tam, R, ml;
     uint32_t t0;                      // signature of the syndrome
     uint32_t s[kMaxWords];            // syndrome words
-    unsigned long long nw;            // class size C(tam, w)
 };
 
 // Per-call parameters and per-shot buffers.
