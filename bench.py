@@ -76,6 +76,19 @@ def cpu_baseline(hx, hz, errors, prob_file, fused_res):
             bad == 0, o)
 
 
+def measured_traffic():
+    """HBM bytes per launch of each kernel, from the committed rocprofv3 PMC summary
+    (FETCH_SIZE / WRITE_SIZE runs scaled by a copy kernel of known size)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_bb144_prob_cc_100k_shots.md")
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic":
+                out[f[1]] = int(f[2])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +97,9 @@ def main():
     ap.add_argument("--shots", type=int, default=SHOTS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-engine leg")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run a device-to-device copy of this many MiB twice (a kernel of known traffic "
+                         "for scaling hardware byte counters)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this decoder shards shots across GPUs with no exchange, and only the "
@@ -110,6 +126,16 @@ def main():
 
     def step():
         sess.decode(d_err, True, d_res, d_decoded=d_dec, d_stats=d_stats, stream=stream)
+
+    if a.copy_calibration_mib > 0:
+        src = torch.ones(a.copy_calibration_mib << 18, dtype=torch.int32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"copy calibration: {a.copy_calibration_mib} MiB read + {a.copy_calibration_mib} MiB written per "
+              f"copy", file=sys.stderr)
+        del src, dst
 
     for _ in range(a.warmup):
         step()
@@ -145,6 +171,7 @@ def main():
     tri_bytes = shots * (hx.shape[1] * 4 + D32 * 4 + M32 * 4 + 4) + zero * (72 + M32 * 4) + int(searched.sum()) * 4
     achieved_bw = tri_bytes / (tri * 1e-3) / 1e9
 
+    traffic = measured_traffic() if shots == SHOTS else {}
     out = {
         "metric": METRIC, "value": shots / (ms_per_step * 1e-3), "unit": "shots/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
@@ -162,13 +189,14 @@ def main():
         "patterns_per_s": patterns / (ms_per_step * 1e-3),
         "kernel_ms": {"triage": tri, "search": srch},
         "roofline": {"bound": "valu", "achieved": achieved_int, "peak": peak_int, "unit": "TOP/s",
-                     "frac": achieved_int / peak_int, "traffic": None,
+                     "frac": achieved_int / peak_int, "traffic": traffic.get("grand::fused_kernel"),
                      "kernel": "grand::fused_kernel",
                      "basis": f"(w+1)*ceil(n/32) int32 ops per tested pattern, {wops} weighted patterns "
                               f"x {D32} words per launch / mean search-kernel time; peak = {info['cus']} CU x 64 "
                               f"lanes x {info['clock_khz'] / 1e6:.2f} GHz"},
         "roofline_triage": {"bound": "hbm", "achieved": achieved_bw, "peak": 8000.0, "unit": "GB/s",
-                            "frac": achieved_bw / 8000.0, "traffic": None, "kernel": "grand::triage_kernel",
+                            "frac": achieved_bw / 8000.0, "traffic": traffic.get("grand::triage_kernel"),
+                            "kernel": "grand::triage_kernel",
                             "bytes_per_launch": tri_bytes},
     }
     if not a.no_plain:

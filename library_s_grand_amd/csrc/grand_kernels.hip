@@ -15,10 +15,13 @@
 // Candidate test.  Every column c carries a 32-bit signature sig[c] = M h_c where
 // M is a fixed random GF(2) matrix (32 x n); the map is linear, so a pattern
 // {c_1..c_w} can only satisfy H e = s if sig(s) ^ sig[c_1] ^ ... ^ sig[c_w] == 0.
-// Lanes walk their combinations keeping prefix XORs of signatures (one LDS read
-// and one XOR per pattern); a signature hit is confirmed exactly against the
-// bit-packed columns.  Results are therefore exact; the signature only decides
-// which patterns get the full D-word comparison.
+// A weight-w class is walked as its (w-1)-prefixes: lanes keep prefix XORs of
+// signatures, and the last columns that complete a prefix are looked up in an
+// LDS hash table of the active signatures (C(tam-1, w-1) probes instead of
+// C(tam, w) pattern tests).  A signature hit is confirmed exactly against the
+// bit-packed columns, so results are exact; the signature only decides which
+// patterns get the full D-word comparison.  The plain engine keeps the
+// pattern-per-lane walk of the reference's GPU kernels.
 //
 // Enumeration order and counts follow the reference CPU search
 // (grand_decoder.cu:264-350, :534-666): weight classes in list order, ranks in
