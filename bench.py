@@ -13,7 +13,9 @@ Reported beside the fused engine:
   * the integer-VALU roofline of the search kernel and the HBM roofline of the
     triage kernel, from HIP events recorded on the decode stream,
   * the CPU oracle (a restatement of the reference CPU search) timed on 1 core,
-  * a bit-exactness check of the fused results against the oracle.
+  * a bit-exactness check of the fused results against the oracle,
+  * the serving loop on a mixed BB72/BB144/BB288 Poisson request stream
+    (request latency p50 / p99, served shots/s).
 """
 from __future__ import annotations
 
@@ -97,6 +99,7 @@ def main():
     ap.add_argument("--shots", type=int, default=SHOTS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-engine leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop leg")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB twice (a kernel of known traffic "
                          "for scaling hardware byte counters)")
@@ -222,6 +225,9 @@ def main():
         out["cpu_baseline"] = cb
         out["bit_exact_vs_oracle"] = ok
     sess.close()
+    if not a.no_serving:
+        from library_s_grand_amd import serving
+        out["serving"] = serving.measure(rate_hz=2000.0, n_requests=4000)
     print(json.dumps(out))
 
 

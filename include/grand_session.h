@@ -58,6 +58,14 @@ int grand_session_decode_device(GrandSession *s, const int *d_input, int input_i
  * supported 1024 positions); synchronizes `stream`. */
 int grand_session_error_bits(GrandSession *s, void *stream);
 
+/* Queues a copy of the error-bit word of the last decode into *host_dst
+ * (pinned host memory) on `stream`, without waiting. */
+int grand_session_error_bits_async(GrandSession *s, unsigned *host_dst, void *stream);
+
+/* Allocates the scratch of a num_shots batch now, so that later decodes of up
+ * to that many shots allocate nothing.  Returns 0, or -1 when out of memory. */
+int grand_session_reserve(GrandSession *s, int num_shots);
+
 /* HIP-event timing of the kernels of each decode, recorded on the decode's
  * stream: triage (pack + syndrome + worklist) and search (the persistent
  * kernel, or the whole plain loop).  last_timing waits for the last decode. */

@@ -32,6 +32,7 @@
 extern "C" {
 size_t grand_fused_lds_bytes(int cap, int ml);
 hipError_t grand_launch_fill(unsigned *p, unsigned v, int n, hipStream_t st);
+hipError_t grand_launch_prep(const grand::DevCall *hcall, grand::DevCall *dcall, unsigned *counters, hipStream_t st);
 hipError_t grand_launch_triage(const grand::DevCode *code, const grand::DevCall *call, int grid, hipStream_t st);
 hipError_t grand_launch_fused(const grand::DevCode *code, const grand::DevCall *call, const grand::DevCall *hcall,
                               int grid, hipStream_t st);
@@ -354,8 +355,10 @@ void remember_code(const std::shared_ptr<CodeEntry> &e) {
 struct Workspace {
     DevMem s_packed, e_packed, lg_packed, s_sig, worklist, counters, dec, declog, tasks, task_arrays;
     int shots = 0;
+    bool tasks_fresh = false;   // the task table was just allocated and must be cleared once
     bool ensure(const DevCode &c, int shots_, int cap) {
         const size_t S = (size_t)std::max(shots_, 1);
+        if (tasks.bytes == 0) tasks_fresh = true;
         if (!s_packed.ensure(S * c.D * 4) || !e_packed.ensure(S * c.M32 * 4) || !lg_packed.ensure(S * c.NL32 * 4) ||
             !s_sig.ensure(S * 4) || !worklist.ensure(S * 4) || !counters.ensure(64) || !dec.ensure(S * c.M32 * 4) ||
             !declog.ensure(S * c.NL32 * 4) || !tasks.ensure(sizeof(TaskSlot) * kTaskSlots) ||
@@ -673,14 +676,18 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
                           : (unsigned long long)g_tuning.split_threshold;
     }
 
-    // worklist length, claim cursor, error bits, task counters and the task
-    // table restart with every batch
-    HIPX(grand_launch_fill(c.counters, 0u, 8, st));
-    HIPX(grand_launch_fill(reinterpret_cast<unsigned *>(c.tasks), 0u, (int)(sizeof(TaskSlot) * kTaskSlots / 4), st));
+    // The task table is cleared once when allocated: every launch leaves all
+    // slots idle (each owner waits for its chunks and frees its slot).  The
+    // worklist length, claim cursor, error bits and task counters restart with
+    // every batch, in the launch that installs the batch descriptor.
+    if (s->ws.tasks_fresh) {
+        HIPX(grand_launch_fill(reinterpret_cast<unsigned *>(c.tasks), 0u, (int)(sizeof(TaskSlot) * kTaskSlots / 4), st));
+        s->ws.tasks_fresh = false;
+    }
     const int groups = (num_shots + 63) / 64;
     const int tgrid = std::max(1, std::min(groups, R.cu_count * 16));
     if (!s->dcall.ensure(sizeof(DevCall))) return -1;
-    HIPX(hipMemcpyAsync(s->dcall.p, &c, sizeof(DevCall), hipMemcpyHostToDevice, st));
+    HIPX(grand_launch_prep(&c, s->dcall.as<DevCall>(), c.counters, st));
     const DevCode *dcode = s->code->dcode.as<DevCode>();
     const DevCall *dcall = s->dcall.as<DevCall>();
     if (s->timing) HIPX(hipEventRecord(s->ev[0], st));
@@ -904,6 +911,21 @@ int grand_session_error_bits(GrandSession *s, void *stream) {
     if (hipMemcpyAsync(&bits, s->call.counters + 2, sizeof(unsigned), hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
     if (hipStreamSynchronize(st) != hipSuccess) return -1;
     return (int)bits;
+}
+
+int grand_session_error_bits_async(GrandSession *s, unsigned *host_dst, void *stream) {
+    if (!s || !host_dst || !s->call.counters) return -1;
+    hipStream_t st = stream ? (hipStream_t)stream : rt().stream;
+    return hipMemcpyAsync(host_dst, s->call.counters + 2, sizeof(unsigned), hipMemcpyDeviceToHost, st) == hipSuccess
+               ? 0 : -1;
+}
+
+int grand_session_reserve(GrandSession *s, int num_shots) {
+    if (!s) return -1;
+    if (!s->ws.ensure(s->code->dev, std::max(num_shots, 1), s->call.cap)) return -1;
+    if (!s->dcall.ensure(sizeof(DevCall))) return -1;
+    s->call.counters = s->ws.counters.as<unsigned>();
+    return 0;
 }
 
 int grand_session_enable_timing(GrandSession *s, int on) {

@@ -1003,112 +1003,143 @@ __device__ void decode_shot_wave(const DevCode &code, const DevCall &call, int s
 
 // ================================================================== kernels
 
-// One wave per shot, 64 shots per group so that nonzero shots are appended to
-// the worklist with one atomic per group.
+// Record of a shot whose syndrome is zero, written by its own lane
+// (grand_decoder.cu:1223-1241, :1487-1499).
+__device__ __forceinline__ void write_zero_result(GrandDecoderResult *dst, int ow) {
+    uint32_t *p = reinterpret_cast<uint32_t *>(dst);
+#pragma unroll
+    for (int i = 0; i < 18; ++i) p[i] = 0u;
+    p[2] = (uint32_t)ow;            // original_weight
+    p[8] = (uint32_t)-2;            // hibrido: no search
+}
+
+// Stages a 64-shot x 64-column tile of int32 rows through LDS (coalesced loads,
+// one row per step) and returns this lane's 64 values packed into two words;
+// `bad` collects any value other than 0/1.
+__device__ __forceinline__ void pack_tile(const int *rows, int stride, int base, int cnt, int c0, int cols,
+                                          uint32_t LDS *tile, uint32_t &w0, uint32_t &w1, unsigned &bad) {
+    const int lane = lane_id();
+#pragma unroll 8
+    for (int j = 0; j < 64; ++j) {
+        int v = 0;
+        if (j < cnt && c0 + lane < cols) v = rows[(size_t)(base + j) * stride + c0 + lane];
+        tile[j * 65 + lane] = (uint32_t)v;
+    }
+    __syncthreads();
+    w0 = 0u; w1 = 0u;
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+        const uint32_t a = tile[lane * 65 + k], b = tile[lane * 65 + 32 + k];
+        bad |= (a | b) & ~1u;
+        w0 |= (a & 1u) << k;
+        w1 |= (b & 1u) << k;
+    }
+    __syncthreads();
+}
+
+// One lane per shot, 64 shots per wave: packs the input rows (errors or
+// syndromes, and logicals), forms s = H e and sig(s) as XORs over the set error
+// bits (both maps are GF(2)-linear), writes the result of zero-syndrome shots
+// and appends the others to the worklist with one atomic per 64 shots.
 __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ codep, const DevCall *__restrict__ callp) {
     const DevCode &code = *codep;
     const DevCall &call = *callp;
     const int lane = lane_id();
-    uint32_t LDS *ew = lds_base();     // kMaxM / 32 words
+    uint32_t LDS *tile = lds_base();   // [64][65]
     const int groups = (call.num_shots + 63) / 64;
     unsigned bad = 0;
     for (int g = blockIdx.x; g < groups; g += gridDim.x) {
         const int base = g * 64;
         const int cnt = (call.num_shots - base) < 64 ? (call.num_shots - base) : 64;
-        bool my_nz = false;
-        for (int j = 0; j < cnt; ++j) {
-            const int shot = base + j;
-            uint32_t sw[kMaxWords];
+        const bool valid = lane < cnt;
+        const int shot = base + lane;
+        uint32_t s[kMaxWords];
 #pragma unroll
-            for (int d = 0; d < kMaxWords; ++d) sw[d] = 0u;
-            int ow = 0;
-            uint32_t sg = 0;
-            if (call.input_errors) {
-                const int *row = call.input + (size_t)shot * code.m;
-                for (int c0 = 0; c0 < code.m; c0 += 64) {
-                    const int c = c0 + lane;
-                    const int v = (c < code.m) ? row[c] : 0;
-                    bad |= (unsigned)(v & ~1);
-                    const uint64_t b = ballot(v == 1);
-                    if (lane == 0) {
-                        ew[c0 >> 5] = (uint32_t)b;
-                        if ((c0 >> 5) + 1 < code.M32) ew[(c0 >> 5) + 1] = (uint32_t)(b >> 32);
-                    }
-                    ow += __popcll(b);
+        for (int d = 0; d < kMaxWords; ++d) s[d] = 0u;
+        uint32_t sg = 0;
+        int ow = 0;
+        if (call.input_errors) {
+            for (int c0 = 0; c0 < code.m; c0 += 64) {
+                uint32_t w[2];
+                pack_tile(call.input, code.m, base, cnt, c0, code.m, tile, w[0], w[1], bad);
+                const int wd = c0 >> 5;
+                if (valid) {
+                    call.e_packed[(size_t)shot * code.M32 + wd] = w[0];
+                    if (wd + 1 < code.M32) call.e_packed[(size_t)shot * code.M32 + wd + 1] = w[1];
                 }
-                __syncthreads();
-                for (int wd = lane; wd < code.M32; wd += 64) call.e_packed[(size_t)shot * code.M32 + wd] = ew[wd];
-                for (int r0 = 0; r0 < code.n; r0 += 64) {
-                    const int r = r0 + lane;
-                    uint32_t par = 0;
-                    if (r < code.n)
-                        for (int wd = 0; wd < code.M32; ++wd) par ^= __popc(code.Hrow[(size_t)r * code.M32 + wd] & ew[wd]);
-                    if (par & 1u) sg ^= code.rowsig[r];
-                    const uint64_t b = ballot(par & 1u);
+                ow += __popc(w[0]) + __popc(w[1]);
 #pragma unroll
-                    for (int d = 0; d < kMaxWords; ++d) {
-                        if (d == (r0 >> 5)) sw[d] = (uint32_t)b;
-                        if (d == (r0 >> 5) + 1) sw[d] = (uint32_t)(b >> 32);
-                    }
-                }
-                __syncthreads();
-            } else {
-                const int *row = call.input + (size_t)shot * code.n;
-                for (int r0 = 0; r0 < code.n; r0 += 64) {
-                    const int r = r0 + lane;
-                    const int v = (r < code.n) ? row[r] : 0;
-                    bad |= (unsigned)(v & ~1);
-                    if (v == 1) sg ^= code.rowsig[r];
-                    const uint64_t b = ballot(v == 1);
+                for (int h = 0; h < 2; ++h) {
+                    uint32_t bits = w[h];
+                    while (bits) {
+                        const int c = c0 + 32 * h + __ffs(bits) - 1;
+                        bits &= bits - 1u;
+                        sg ^= code.sig[c];
 #pragma unroll
-                    for (int d = 0; d < kMaxWords; ++d) {
-                        if (d == (r0 >> 5)) sw[d] = (uint32_t)b;
-                        if (d == (r0 >> 5) + 1) sw[d] = (uint32_t)(b >> 32);
+                        for (int d = 0; d < kMaxWords; ++d)
+                            if (d < code.D) s[d] ^= code.Hcm[(size_t)c * code.D + d];
                     }
                 }
             }
-            if (call.family == kCLN) {
-                const int *lrow = call.logicals + (size_t)shot * code.NL;
-                for (int q0 = 0; q0 < code.NL; q0 += 64) {
-                    const int q = q0 + lane;
-                    const int v = (q < code.NL) ? lrow[q] : 0;
-                    bad |= (unsigned)(v & ~1);
-                    const uint64_t b = ballot(v == 1);
-                    if (lane == 0) {
-                        call.lg_packed[(size_t)shot * code.NL32 + (q0 >> 5)] = (uint32_t)b;
-                        if ((q0 >> 5) + 1 < code.NL32) call.lg_packed[(size_t)shot * code.NL32 + (q0 >> 5) + 1] = (uint32_t)(b >> 32);
+        } else {
+            for (int r0 = 0; r0 < code.n; r0 += 64) {
+                uint32_t w[2];
+                pack_tile(call.input, code.n, base, cnt, r0, code.n, tile, w[0], w[1], bad);
+                const int wd = r0 >> 5;
+#pragma unroll
+                for (int d = 0; d < kMaxWords; ++d) {
+                    if (d == wd) s[d] = w[0];
+                    if (d == wd + 1) s[d] = w[1];
+                }
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    uint32_t bits = w[h];
+                    while (bits) {
+                        const int r = r0 + 32 * h + __ffs(bits) - 1;
+                        bits &= bits - 1u;
+                        sg ^= code.rowsig[r];
                     }
                 }
             }
-            int swt = 0;
-#pragma unroll
-            for (int d = 0; d < kMaxWords; ++d) swt += __popc(sw[d]);
-            sg = wave_xor(sg);   // signature of s: XOR of the row signatures of its set bits
-            if (lane < code.D) {
-#pragma unroll
-                for (int d = 0; d < kMaxWords; ++d) if (d == lane) call.s_packed[(size_t)shot * code.D + d] = sw[d];
-            }
-            if (lane == 0) call.s_sig[shot] = sg;
-            const bool nz = swt != 0;
-            if (!nz) {
-                write_result(&call.results[shot], 0, 0, (call.family == kCC) ? ow : 0, 0, 0, 0, 0ull, -2, 0, 0.0, 0.0);
-                for (int wd = lane; wd < code.M32; wd += 64) call.dec[(size_t)shot * code.M32 + wd] = 0u;
-                if (call.family == kCLN)
-                    for (int wd = lane; wd < code.NL32; wd += 64) call.declog[(size_t)shot * code.NL32 + wd] = 0u;
-                if (call.stats && lane == 0) {
-                    call.stats[3 * (size_t)shot] = 0;
-                    call.stats[3 * (size_t)shot + 1] = 0;
-                    call.stats[3 * (size_t)shot + 2] = 0;
-                }
-            }
-            if (lane == j) my_nz = nz;
         }
-        const uint64_t mask = ballot(my_nz);
+        if (call.family == kCLN) {
+            for (int q0 = 0; q0 < code.NL; q0 += 64) {
+                uint32_t w0, w1;
+                pack_tile(call.logicals, code.NL, base, cnt, q0, code.NL, tile, w0, w1, bad);
+                const int wd = q0 >> 5;
+                if (valid) {
+                    call.lg_packed[(size_t)shot * code.NL32 + wd] = w0;
+                    if (wd + 1 < code.NL32) call.lg_packed[(size_t)shot * code.NL32 + wd + 1] = w1;
+                }
+            }
+        }
+        uint32_t any = 0;
+#pragma unroll
+        for (int d = 0; d < kMaxWords; ++d) any |= s[d];
+        const bool nz = valid && any != 0u;
+        if (valid) {
+#pragma unroll
+            for (int d = 0; d < kMaxWords; ++d)
+                if (d < code.D) call.s_packed[(size_t)shot * code.D + d] = s[d];
+            call.s_sig[shot] = sg;
+        }
+        if (valid && !nz) {
+            write_zero_result(&call.results[shot], (call.family == kCC) ? ow : 0);
+            for (int wd = 0; wd < code.M32; ++wd) call.dec[(size_t)shot * code.M32 + wd] = 0u;
+            if (call.family == kCLN)
+                for (int wd = 0; wd < code.NL32; ++wd) call.declog[(size_t)shot * code.NL32 + wd] = 0u;
+            if (call.stats) {
+                call.stats[3 * (size_t)shot] = 0;
+                call.stats[3 * (size_t)shot + 1] = 0;
+                call.stats[3 * (size_t)shot + 2] = 0;
+            }
+        }
+        const uint64_t mask = ballot(nz);
         if (mask) {
             unsigned pos = 0;
             if (lane == 0) pos = atomicAdd(&call.counters[0], (unsigned)__popcll(mask));
             pos = uni(pos);
-            if (my_nz) call.worklist[pos + lanes_below(mask)] = base + lane;
+            if (nz) call.worklist[pos + lanes_below(mask)] = shot;
         }
     }
     if (ballot(bad != 0u) && lane == 0) atomicOr(&call.counters[2], kErrNonBinary);
@@ -1120,7 +1151,10 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
     const DevCall &call = *callp;
     const WaveLds L = carve_lds(call.cap, call.kind == kML);
     const unsigned total = call.counters[0];
-    const unsigned B = (unsigned)call.claim_batch;
+    // shots per claim: the tuned batch when the worklist is long, fewer when
+    // there are not several claims per wave (small batches finish sooner)
+    unsigned B = total / (4u * gridDim.x);
+    B = B < 1u ? 1u : (B > (unsigned)call.claim_batch ? (unsigned)call.claim_batch : B);
     int ctask = -1;        // task whose arrays and hash table this wave's LDS holds
     unsigned hmask = 0;
     for (;;) {
@@ -1371,6 +1405,16 @@ __global__ void expand_bits_kernel(const uint32_t *packed, int words, int cols, 
     }
 }
 
+// Installs a batch's descriptor (passed by value in the kernel arguments) in
+// device memory and clears the batch counters, in one launch.
+static_assert(sizeof(DevCall) % 4 == 0 && sizeof(DevCall) <= 3072, "DevCall must fit the kernel arguments");
+__global__ void prep_kernel(const DevCall c, DevCall *dst, unsigned *counters) {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(&c);
+    uint32_t *d = reinterpret_cast<uint32_t *>(dst);
+    for (int i = threadIdx.x; i < (int)(sizeof(DevCall) / 4); i += blockDim.x) d[i] = src[i];
+    if (threadIdx.x < 8) counters[threadIdx.x] = 0u;
+}
+
 __global__ void fill_u32_kernel(unsigned *p, unsigned v, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -1390,13 +1434,18 @@ size_t grand_fused_lds_bytes(int cap, int ml) {
                                (size_t)cap * (ml ? 6 : 2));
 }
 
+hipError_t grand_launch_prep(const DevCall *hcall, DevCall *dcall, unsigned *counters, hipStream_t st) {
+    hipLaunchKernelGGL(grand::prep_kernel, dim3(1), dim3(256), 0, st, *hcall, dcall, counters);
+    return hipGetLastError();
+}
+
 hipError_t grand_launch_fill(unsigned *p, unsigned v, int n, hipStream_t st) {
     hipLaunchKernelGGL(grand::fill_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, v, n);
     return hipGetLastError();
 }
 
 hipError_t grand_launch_triage(const DevCode *code, const DevCall *call, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(grand::triage_kernel, dim3(grid), dim3(64), sizeof(uint32_t) * (grand::kMaxM / 32), st, code, call);
+    hipLaunchKernelGGL(grand::triage_kernel, dim3(grid), dim3(64), sizeof(uint32_t) * 64 * 65, st, code, call);
     return hipGetLastError();
 }
 

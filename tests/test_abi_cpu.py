@@ -42,18 +42,22 @@ def test_header_compiles_as_c_and_layout(tmp_path):
 #include <stddef.h>
 #include "grand_decoder.h"
 #include "grand_session.h"
+#include "grand_server.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(GrandDecoderConfig), sizeof(GrandDecoderResult),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(GrandDecoderConfig), sizeof(GrandDecoderResult),
          offsetof(GrandDecoderResult, tam), offsetof(GrandDecoderResult, total_tested_combinations),
          offsetof(GrandDecoderResult, hibrido), offsetof(GrandDecoderResult, iteration),
-         offsetof(GrandDecoderResult, time_sum_ms), sizeof(GrandDecoderExConfig));
+         offsetof(GrandDecoderResult, time_sum_ms), sizeof(GrandDecoderExConfig), sizeof(GrandServerConfig),
+         offsetof(GrandServerConfig, hbm_budget_bytes));
   return 0;
 }''')
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(T.ROOT, "include"), str(src), "-o",
                     str(exe)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
-    assert got == [24, 72, 20, 24, 32, 36, 40, 32]
+    assert got == [24, 72, 20, 24, 32, 36, 40, 32, 24, 16]
+    from library_s_grand_amd.serving import GrandServerConfig
+    assert ctypes.sizeof(GrandServerConfig) == 24 and GrandServerConfig.hbm_budget_bytes.offset == 16
     assert ctypes.sizeof(G.GrandDecoderConfig) == 24 and ctypes.sizeof(G.GrandDecoderResult) == 72
     assert G.GrandDecoderResult.total_tested_combinations.offset == 24
     assert G.GrandDecoderResult.time_sum_ms.offset == 40
