@@ -53,6 +53,23 @@ int grand_session_decode_device(GrandSession *s, const int *d_input, int input_i
                                 int num_shots, GrandDecoderResult *d_results, unsigned *d_decoded,
                                 unsigned *d_decoded_logicals, unsigned long long *d_stats, void *stream);
 
+/* Learned search order.  As grand_session_decode_device, but every shot first
+ * runs a learned pass over its first active list: positions sorted by
+ * descending score, patterns of weight 1..max_weight tried so that a pattern
+ * comes as soon as its worst-scored position does, each checked exactly
+ * against the syndrome.  The first match is the result; after `budget` patterns
+ * without one, the plain search runs as usual.  total_tested_combinations
+ * counts the patterns of both.  d_scores: device float [num_shots][stride],
+ * indexed by position in the first active list (scores_per_column = 0, stride
+ * >= the list length, longer lists skip the pass) or by column
+ * (scores_per_column = 1, stride >= m).  Not for GRAND_KIND_ML_CLN or the
+ * plain engine.  budget = 0 gives the plain results.  Returns 0 or -1. */
+int grand_session_decode_learned(GrandSession *s, const int *d_input, int input_is_errors, const int *d_logicals,
+                                 int num_shots, const float *d_scores, int stride, int scores_per_column,
+                                 int max_weight, unsigned long long budget, GrandDecoderResult *d_results,
+                                 unsigned *d_decoded, unsigned *d_decoded_logicals, unsigned long long *d_stats,
+                                 void *stream);
+
 /* Error bits raised by the kernels of the last decode on this session
  * (1 = an input value other than 0/1, 2 = an active set larger than the
  * supported 1024 positions); synchronizes `stream`. */

@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-SOURCES = ["grand_kernels.hip", "grand_host.cpp", "grand_server.cpp"]
+SOURCES = ["grand_kernels.hip", "grand_mlp.hip", "grand_host.cpp", "grand_server.cpp"]
 
 
 def _run(cmd, cwd=None):

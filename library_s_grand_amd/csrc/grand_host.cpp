@@ -904,6 +904,28 @@ int grand_session_decode_device(GrandSession *s, const int *d_input, int input_i
                          d_stats, st);
 }
 
+int grand_session_decode_learned(GrandSession *s, const int *d_input, int input_is_errors, const int *d_logicals,
+                                 int num_shots, const float *d_scores, int stride, int scores_per_column,
+                                 int max_weight, unsigned long long budget, GrandDecoderResult *d_results,
+                                 unsigned *d_decoded, unsigned *d_decoded_logicals, unsigned long long *d_stats,
+                                 void *stream) {
+    if (!s || s->search == kML || s->ex.engine == GRAND_ENGINE_PLAIN) return -1;
+    if (num_shots > 0 && (!d_input || !d_results || !d_scores)) return -1;
+    if (s->family == kCLN && s->code->NL > 0 && num_shots > 0 && !d_logicals) return -1;
+    if (max_weight < 1 || max_weight > kMaxW || stride < 1 || (scores_per_column && stride < s->code->dev.m)) return -1;
+    hipStream_t st = stream ? (hipStream_t)stream : rt().stream;
+    DevCall &c = s->call;
+    c.lscores = d_scores;
+    c.lstride = stride;
+    c.lper_column = scores_per_column ? 1 : 0;
+    c.lmaxw = max_weight;
+    c.lbudget = budget;
+    const int rc = decode_device(s, d_input, input_is_errors, d_logicals, num_shots, d_results, d_decoded,
+                                 d_decoded_logicals, d_stats, st);
+    c.lscores = nullptr;   // the descriptor was copied into the launch; later decodes are plain
+    return rc;
+}
+
 int grand_session_error_bits(GrandSession *s, void *stream) {
     if (!s || !s->call.counters) return 0;
     hipStream_t st = stream ? (hipStream_t)stream : rt().stream;

@@ -145,6 +145,7 @@ struct WaveLds {
     unsigned long long LDS *ht;   // (signature << 32) | position, empty = kHtEmpty
     uint32_t LDS *ew;  // packed error words of the shot (triage), kMaxM / 32
     uint32_t LDS *sw;  // syndrome words of the shot being searched, kMaxWords
+    uint32_t LDS *sw2; // syndrome minus one column (learned pass), kMaxWords
 };
 
 extern __shared__ uint32_t lds_raw[];
@@ -161,7 +162,8 @@ __device__ __forceinline__ WaveLds carve_lds(int cap, bool ml) {
     WaveLds L;
     L.ew = base;
     L.sw = base + kMaxM / 32;
-    uint32_t LDS *p = L.sw + kMaxWords;
+    L.sw2 = L.sw + kMaxWords;
+    uint32_t LDS *p = L.sw2 + kMaxWords;
     L.ht = (unsigned long long LDS *)p; p += 2 * ht_capacity(cap);
     L.A = (int LDS *)p; p += cap;
     L.sigA = p; p += cap;
@@ -219,10 +221,17 @@ __device__ int build_active(const DevCode &code, const uint32_t LDS *s, int thr,
 
 // Stable sort of A[0..tam) by llr (ties keep active-list order), the order of
 // the reference's insertion sort (grand_decoder.cu:598-610).
+__device__ void sort_by_key(const WaveLds &L, int tam);
 __device__ void sort_by_llr(const DevCode &code, const WaveLds &L, int tam) {
     const int lane = lane_id();
     for (int i = lane; i < tam; i += 64) L.llrA[i] = code.llr[L.A[i]];
     __syncthreads();
+    sort_by_key(L, tam);
+}
+
+// Stable ascending sort of A[0..tam) by the keys in llrA[0..tam) into P / sigP / llrP.
+__device__ void sort_by_key(const WaveLds &L, int tam) {
+    const int lane = lane_id();
     for (int i0 = 0; i0 < tam; i0 += 64) {
         const int i = i0 + lane;
         const float li = (i < tam) ? L.llrA[i] : 0.0f;
@@ -311,7 +320,7 @@ __device__ __forceinline__ PrefHit scan_prefix(const DevCode &code, const uint32
             const unsigned long long e = ht[pos];
             if (e == kHtEmpty) break;
             const int x = (int)(uint32_t)e;
-            if ((uint32_t)(e >> 32) == t && x > last) {
+            if ((uint32_t)(e >> 32) == t && x > last && x < tam) {
                 int cols[kMaxW];
 #pragma unroll
                 for (int l = 0; l < kMaxW; ++l) cols[l] = (l < WP) ? act[c[l]] : ((l == WP) ? act[x] : 0);
@@ -949,6 +958,116 @@ __device__ void finalize_shot(const DevCode &code, const DevCall &call, int shot
     }
 }
 
+// ------------------------------------------------------------ learned pass
+// Learned search order (no counterpart in the reference).  The positions of the
+// first active list are sorted by descending score (ties: list order) into P,
+// and patterns of weight 1..lmaxw over P are tried stage by stage: stage k
+// holds the patterns whose last position in P is k, by weight, each weight
+// lexicographic over P[0..k).  So a pattern comes as soon as its worst-scored
+// position does.  Every candidate is checked exactly against the syndrome; the
+// pass stops at the first match or after lbudget patterns.  Returns the match
+// (count = patterns tried up to and including it) or false with count = the
+// patterns tried.
+
+// First match among the w'-subsets S of P[0..k) with H(S) = s2 (signature t2),
+// rank < lim in lexicographic order; returns its rank or ~0.
+__device__ uint64_t learned_class(const DevCode &code, const DevCall &call, const uint32_t LDS *s2, uint32_t t2,
+                                  int wp, int k, uint64_t lim, const WaveLds &L, unsigned hmask) {
+    const int lane = lane_id();
+    if (wp == 1) {
+        for (int i0 = 0; i0 < k && (uint64_t)i0 < lim; i0 += 64) {
+            const int i = i0 + lane;
+            bool hit = false;
+            if (i < k && (uint64_t)i < lim && L.sigP[i] == t2) {
+                int cols[kMaxW] = {L.P[i], 0, 0, 0, 0, 0};
+                hit = exact_match(code, s2, cols, 1);
+            }
+            const uint64_t b = ballot(hit);
+            if (b) return (uint64_t)(i0 + __ffsll((long long)b) - 1);
+        }
+        return ~0ull;
+    }
+    const uint64_t NP = ncr(code.nCr, k - 1, wp - 1);
+    uint64_t pos = 0;
+    while (pos < NP) {
+        const uint64_t rem = NP - pos;
+        const uint64_t r64 = (rem + 63u) / 64u;
+        const int R = (int)(r64 < (uint64_t)call.rmax ? r64 : (uint64_t)call.rmax);
+        const uint64_t q0 = pos + (uint64_t)lane * (uint64_t)R;
+        const int len = q0 >= NP ? 0 : (int)((NP - q0) < (uint64_t)R ? (NP - q0) : (uint64_t)R);
+        const PrefHit h = scan_prefix_w<0>(wp, code, s2, t2, q0, len, k, L.P, L.sigP, nullptr, L.ht, hmask);
+        const uint64_t b = ballot(h.j >= 0);
+        if (b) {
+            const uint64_t r = (uint64_t)__shfl((long long)h.rank, __ffsll((long long)b) - 1, 64);
+            return r < lim ? r : ~0ull;   // later prefixes only hold higher ranks
+        }
+        pos += 64ull * (uint64_t)R;
+    }
+    return ~0ull;
+}
+
+__device__ bool learned_pass(const DevCode &code, const DevCall &call, int shot, const uint32_t LDS *s, uint32_t t0,
+                             const WaveLds &L, ShotOutcome &o) {
+    const int lane = lane_id();
+    const int thr = code.max_soma;
+    int tam = build_active(code, s, thr, L, call.cap);
+    if (call.family == kCLN && tam > call.max_size) tam = 0;      // the first iteration has no list
+    if (tam <= 0 || tam > call.cap || (!call.lper_column && tam > call.lstride) || branch_of(call, tam) == 0)
+        return false;
+    const float *sc = call.lscores + (size_t)shot * call.lstride;
+    for (int i = lane; i < tam; i += 64) {
+        const float v = sc[call.lper_column ? L.A[i] : i];
+        L.llrA[i] = (v == v) ? -v : __builtin_inff();             // descending score; NaN last
+    }
+    __syncthreads();
+    sort_by_key(L, tam);
+    const unsigned hmask = build_ht(L.ht, L.sigP, tam);
+    const uint64_t budget = call.lbudget;
+    const int maxw = call.lmaxw < kMaxW ? call.lmaxw : kMaxW;
+    uint64_t G = 0;
+    bool stop = false;
+    for (int k = 0; k < tam && !stop; ++k) {
+        const int ck = L.P[k];
+        if (lane < kMaxWords) L.sw2[lane] = (lane < code.D) ? (s[lane] ^ code.Hcm[(size_t)ck * code.D + lane]) : 0u;
+        __syncthreads();
+        const uint32_t t2 = t0 ^ L.sigP[k];
+        for (int wp = 0; wp < maxw; ++wp) {
+            const uint64_t N = (wp == 0) ? 1ull : ncr(code.nCr, k, wp);
+            if (N == 0) break;                      // larger weights do not fit in P[0..k) either
+            if (G >= budget) { stop = true; break; }
+            const uint64_t lim = (budget - G) < N ? (budget - G) : N;
+            uint64_t r = ~0ull;
+            if (wp == 0) {
+                bool z = true;
+                if (t2 == 0u)
+                    for (int d = 0; d < code.D; ++d) z = z && (L.sw2[d] == 0u);
+                r = (t2 == 0u && z) ? 0ull : ~0ull;
+            } else {
+                r = learned_class(code, call, L.sw2, t2, wp, k, lim, L, hmask);
+            }
+            if (r != ~0ull) {
+                int c[kMaxW] = {0, 0, 0, 0, 0, 0};
+                if (wp > 0) unrank_dyn(r, k, wp, code.nCr, c);
+                o.fw = wp + 1;
+#pragma unroll
+                for (int l = 0; l < kMaxW; ++l) o.win[l] = (l < wp) ? L.P[c[l]] : ((l == wp) ? ck : -1);
+                o.count += G + r + 1;
+                o.wops += (G + r + 1) * (uint64_t)(wp + 2);
+                o.found = 1;
+                o.iteration = 1;
+                o.tam = tam;
+                o.hibrido = (branch_of(call, tam) > 0) ? 0 : -1;
+                return true;
+            }
+            G += lim;
+            o.wops += lim * (uint64_t)(wp + 2);
+            if (lim < N) { stop = true; break; }
+        }
+    }
+    o.count += G;
+    return false;
+}
+
 // The whole per-shot pipeline in one wave (grand_decoder.cu:1284-1343 and the
 // equivalent loops of the other four drivers).
 __device__ void decode_shot_wave(const DevCode &code, const DevCall &call, int shot, const WaveLds &L) {
@@ -964,7 +1083,8 @@ __device__ void decode_shot_wave(const DevCode &code, const DevCall &call, int s
 #pragma unroll
     for (int l = 0; l < kMaxW; ++l) o.win[l] = -1;
     int it = 0, tam = 0, tam_old = 0;
-    while (it < code.max_soma && it < 3) {
+    const bool learned_hit = call.lscores && call.kind != kML && learned_pass(code, call, shot, s, t0, L, o);
+    while (!learned_hit && it < code.max_soma && it < 3) {
         const int thr = code.max_soma - it;
         ++it;
         tam = build_active(code, s, thr, L, call.cap);
@@ -987,9 +1107,11 @@ __device__ void decode_shot_wave(const DevCode &code, const DevCall &call, int s
         }
         if (hit) { o.found = 1; break; }
     }
-    o.iteration = it;
-    o.tam = tam;
-    // the decoded columns are reported sorted (the ML winner is in LLR order)
+    if (!learned_hit) {
+        o.iteration = it;
+        o.tam = tam;
+    }
+    // the decoded columns are reported sorted (the ML and learned winners are in score order)
     if (o.found) {
 #pragma unroll
         for (int a = 0; a < kMaxW; ++a)
@@ -1149,7 +1271,7 @@ __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ 
 __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ codep, const DevCall *__restrict__ callp) {
     const DevCode &code = *codep;
     const DevCall &call = *callp;
-    const WaveLds L = carve_lds(call.cap, call.kind == kML);
+    const WaveLds L = carve_lds(call.cap, call.kind == kML || call.lscores != nullptr);
     const unsigned total = call.counters[0];
     // shots per claim: the tuned batch when the worklist is long, fewer when
     // there are not several claims per wave (small batches finish sooner)
@@ -1430,7 +1552,7 @@ using grand::DevCall;
 using grand::DevCode;
 
 size_t grand_fused_lds_bytes(int cap, int ml) {
-    return sizeof(uint32_t) * ((size_t)grand::kMaxM / 32 + grand::kMaxWords + 2 * (size_t)grand::ht_capacity(cap) +
+    return sizeof(uint32_t) * ((size_t)grand::kMaxM / 32 + 2 * grand::kMaxWords + 2 * (size_t)grand::ht_capacity(cap) +
                                (size_t)cap * (ml ? 6 : 2));
 }
 
@@ -1450,7 +1572,7 @@ hipError_t grand_launch_triage(const DevCode *code, const DevCall *call, int gri
 }
 
 hipError_t grand_launch_fused(const DevCode *code, const DevCall *call, const DevCall *hcall, int grid, hipStream_t st) {
-    const size_t lds = grand_fused_lds_bytes(hcall->cap, hcall->kind == grand::kML);
+    const size_t lds = grand_fused_lds_bytes(hcall->cap, hcall->kind == grand::kML || hcall->lscores != nullptr);
     hipLaunchKernelGGL(grand::fused_kernel, dim3(grid), dim3(64), lds, st, code, call);
     return hipGetLastError();
 }

@@ -91,6 +91,13 @@ struct DevCall {
     unsigned long long split_threshold;  // classes larger than this are shared (0 = never)
     unsigned *dbg;                    // optional chunk trace (debug builds of a run), [0] = records
     unsigned dbg_cap;
+    // learned search order, tried before the plain search (lscores == null: off)
+    const float *lscores;             // per shot: [lstride] scores, by slot of the first active list or by column
+    int lstride;
+    int lper_column;                  // 1: lscores indexed by column, 0: by position in the first active list
+    int lmaxw;                        // pattern weights 1..lmaxw are tried (<= kMaxW)
+    int lpad;
+    unsigned long long lbudget;       // patterns the learned pass may test before plain search takes over
 };
 
 // Search outcome of one shot, handed from a search engine to the finalizer.

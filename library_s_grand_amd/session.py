@@ -21,6 +21,10 @@ def bind_session(lib):
     lib.grand_session_create.restype = _VP
     lib.grand_session_decode_device.argtypes = [_VP, _VP, ctypes.c_int, _VP, ctypes.c_int, _VP, _VP, _VP, _VP, _VP]
     lib.grand_session_decode_device.restype = ctypes.c_int
+    lib.grand_session_decode_learned.argtypes = [_VP, _VP, ctypes.c_int, _VP, ctypes.c_int, _VP, ctypes.c_int,
+                                                 ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, _VP, _VP, _VP, _VP,
+                                                 _VP]
+    lib.grand_session_decode_learned.restype = ctypes.c_int
     lib.grand_session_error_bits.argtypes = [_VP, _VP]
     lib.grand_session_enable_timing.argtypes = [_VP, ctypes.c_int]
     lib.grand_session_last_timing.argtypes = [_VP, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
@@ -80,6 +84,19 @@ class Session:
                                                   _ptr(d_decoded_logicals), _ptr(d_stats), st)
         if rc != 0:
             raise RuntimeError(f"grand_session_decode_device returned {rc}")
+
+    def decode_learned(self, d_input, input_is_errors, d_results, d_scores, per_column, max_weight=6,
+                       budget=100_000, d_logicals=None, d_decoded=None, d_decoded_logicals=None, d_stats=None,
+                       stream=None, num_shots=None):
+        """Learned pass (scores: float32 CUDA tensor [shots][stride]) before the plain search."""
+        shots = num_shots if num_shots is not None else d_input.shape[0]
+        st = None if stream is None else ctypes.c_void_p(stream)
+        rc = self.lib.grand_session_decode_learned(self.h, _ptr(d_input), int(bool(input_is_errors)), _ptr(d_logicals),
+                                                   shots, _ptr(d_scores), d_scores.shape[1], int(bool(per_column)),
+                                                   max_weight, budget, _ptr(d_results), _ptr(d_decoded),
+                                                   _ptr(d_decoded_logicals), _ptr(d_stats), st)
+        if rc != 0:
+            raise RuntimeError(f"grand_session_decode_learned returned {rc}")
 
     def error_bits(self, stream=None):
         return self.lib.grand_session_error_bits(self.h, None if stream is None else ctypes.c_void_p(stream))
