@@ -15,7 +15,10 @@ Reported beside the fused engine:
   * the CPU oracle (a restatement of the reference CPU search) timed on 1 core,
   * a bit-exactness check of the fused results against the oracle,
   * the serving loop on a mixed BB72/BB144/BB288 Poisson request stream
-    (request latency p50 / p99, served shots/s).
+    (request latency p50 / p99, served shots/s),
+  * the learned search order on the 216 x 3000 DEM (grand_decode_prob_cln):
+    trained on 2M plain-decoder pairs on the GPU, mean patterns per shot and
+    logical error rate against plain decoding on 1M held-out shots.
 """
 from __future__ import annotations
 
@@ -100,6 +103,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-engine leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-order leg")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB twice (a kernel of known traffic "
                          "for scaling hardware byte counters)")
@@ -228,6 +232,9 @@ def main():
     if not a.no_serving:
         from library_s_grand_amd import serving
         out["serving"] = serving.measure(rate_hz=2000.0, n_requests=4000)
+    if not a.no_learned:
+        from library_s_grand_amd import learned
+        out["learned"] = learned.experiment(log=lambda *x: None)
     print(json.dumps(out))
 
 

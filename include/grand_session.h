@@ -47,7 +47,8 @@ GrandSession *grand_session_create(int kind, const int *H, const int *H2, int n,
  * started, tick when it completed (0, 0 for zero-syndrome shots), and the
  * weighted pattern count  sum over tested patterns of (weight + 1).
  * All pointers are device memory; work is queued on `stream` (a hipStream_t,
- * NULL = the session's own stream) and the call returns without waiting.
+ * NULL = the default stream, so work the caller queued there comes first) and
+ * the call returns without waiting.
  * Returns 0, or -1 for invalid arguments. */
 int grand_session_decode_device(GrandSession *s, const int *d_input, int input_is_errors, const int *d_logicals,
                                 int num_shots, GrandDecoderResult *d_results, unsigned *d_decoded,

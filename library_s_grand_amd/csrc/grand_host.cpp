@@ -533,7 +533,7 @@ int run_plain(GrandSession *s, hipStream_t st, int num_shots) {
         const int shot = work[wi];
         ShotOutcome o{};
         o.hibrido = -2;
-        for (int l = 0; l < kMaxW; ++l) o.win[l] = -1;
+        for (int l = 0; l < kMaxWin; ++l) o.win[l] = -1;
         unsigned long long hit_rank = 0;
         int it = 0, tam = 0, tam_old = 0;
         bool found = false;
@@ -899,7 +899,7 @@ int grand_session_decode_device(GrandSession *s, const int *d_input, int input_i
                                 unsigned *d_decoded_logicals, unsigned long long *d_stats, void *stream) {
     if (!s || (num_shots > 0 && (!d_input || !d_results))) return -1;
     if (s->family == kCLN && s->code->NL > 0 && num_shots > 0 && !d_logicals) return -1;
-    hipStream_t st = stream ? (hipStream_t)stream : rt().stream;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the default stream, ordered with other default-stream work
     return decode_device(s, d_input, input_is_errors, d_logicals, num_shots, d_results, d_decoded, d_decoded_logicals,
                          d_stats, st);
 }
@@ -912,8 +912,8 @@ int grand_session_decode_learned(GrandSession *s, const int *d_input, int input_
     if (!s || s->search == kML || s->ex.engine == GRAND_ENGINE_PLAIN) return -1;
     if (num_shots > 0 && (!d_input || !d_results || !d_scores)) return -1;
     if (s->family == kCLN && s->code->NL > 0 && num_shots > 0 && !d_logicals) return -1;
-    if (max_weight < 1 || max_weight > kMaxW || stride < 1 || (scores_per_column && stride < s->code->dev.m)) return -1;
-    hipStream_t st = stream ? (hipStream_t)stream : rt().stream;
+    if (max_weight < 1 || max_weight > kMaxWin || stride < 1 || (scores_per_column && stride < s->code->dev.m)) return -1;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the default stream, ordered with other default-stream work
     DevCall &c = s->call;
     c.lscores = d_scores;
     c.lstride = stride;
@@ -928,7 +928,7 @@ int grand_session_decode_learned(GrandSession *s, const int *d_input, int input_
 
 int grand_session_error_bits(GrandSession *s, void *stream) {
     if (!s || !s->call.counters) return 0;
-    hipStream_t st = stream ? (hipStream_t)stream : rt().stream;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the default stream, ordered with other default-stream work
     unsigned bits = 0;
     if (hipMemcpyAsync(&bits, s->call.counters + 2, sizeof(unsigned), hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
     if (hipStreamSynchronize(st) != hipSuccess) return -1;
@@ -937,7 +937,7 @@ int grand_session_error_bits(GrandSession *s, void *stream) {
 
 int grand_session_error_bits_async(GrandSession *s, unsigned *host_dst, void *stream) {
     if (!s || !host_dst || !s->call.counters) return -1;
-    hipStream_t st = stream ? (hipStream_t)stream : rt().stream;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the default stream, ordered with other default-stream work
     return hipMemcpyAsync(host_dst, s->call.counters + 2, sizeof(unsigned), hipMemcpyDeviceToHost, st) == hipSuccess
                ? 0 : -1;
 }

@@ -109,11 +109,12 @@ __device__ __forceinline__ void unrank_t(uint64_t r, int tam, const unsigned lon
     }
 }
 
-// Same with the weight known only at run time (per lane).
+// Same with the weight known only at run time (per lane), at most MAXW.
+template <int MAXW = kMaxW>
 __device__ __forceinline__ void unrank_dyn(uint64_t r, int tam, int w, const unsigned long long *nCr, int *c) {
     int prev = -1;
 #pragma unroll
-    for (int i = 0; i < kMaxW; ++i) {
+    for (int i = 0; i < MAXW; ++i) {
         if (i < w) {
             const int k = w - i;
             const uint64_t tot = nCr[(tam - 1 - prev) * kNcrStride + k];
@@ -179,13 +180,14 @@ __device__ __forceinline__ WaveLds carve_lds(int cap, bool ml) {
 }
 
 // Exact check of a candidate: s ^ XOR_l column(cols[l]) == 0 over all D words.
+template <int MAXW = kMaxW>
 __device__ __forceinline__ bool exact_match(const DevCode &code, const uint32_t LDS *s, const int *cols, int w) {
     bool ok = true;
 #pragma unroll 1
     for (int d = 0; d < code.D; ++d) {
         uint32_t x = s[d];
 #pragma unroll
-        for (int l = 0; l < kMaxW; ++l)
+        for (int l = 0; l < MAXW; ++l)
             if (l < w) x ^= code.Hcm[(size_t)cols[l] * code.D + d];
         ok = ok && (x == 0u);
     }
@@ -321,10 +323,10 @@ __device__ __forceinline__ PrefHit scan_prefix(const DevCode &code, const uint32
             if (e == kHtEmpty) break;
             const int x = (int)(uint32_t)e;
             if ((uint32_t)(e >> 32) == t && x > last && x < tam) {
-                int cols[kMaxW];
+                int cols[WP + 1];
 #pragma unroll
-                for (int l = 0; l < kMaxW; ++l) cols[l] = (l < WP) ? act[c[l]] : ((l == WP) ? act[x] : 0);
-                if (exact_match(code, s, cols, WP + 1)) {
+                for (int l = 0; l <= WP; ++l) cols[l] = (l < WP) ? act[c[l]] : act[x];
+                if (exact_match<WP + 1>(code, s, cols, WP + 1)) {
                     const uint64_t r = rank_with_last<WP>(c, x, tam, code.nCr);
                     if (MODE == 0) {
                         if (r < h.rank) { h.rank = r; h.j = j; }
@@ -377,6 +379,23 @@ __device__ __attribute__((noinline)) PrefHit scan_prefix_w(int w, const DevCode 
     }
 }
 
+// First-hit scan for the learned pass, whose sub-classes go up to weight 7
+// (patterns up to kMaxWin columns); kept apart so that the plain search's
+// dispatcher does not carry the registers of the larger cases.
+__device__ __attribute__((noinline)) PrefHit scan_prefix_learned(int w, const DevCode &code, const uint32_t LDS *s,
+                                                                 uint32_t t0, uint64_t q0, int len, int tam,
+                                                                 const int LDS *act, const uint32_t LDS *sg,
+                                                                 const unsigned long long LDS *ht, unsigned hmask) {
+    switch (w) {
+        case 2: return scan_prefix<1, 0>(code, s, t0, q0, len, tam, act, sg, nullptr, ht, hmask);
+        case 3: return scan_prefix<2, 0>(code, s, t0, q0, len, tam, act, sg, nullptr, ht, hmask);
+        case 4: return scan_prefix<3, 0>(code, s, t0, q0, len, tam, act, sg, nullptr, ht, hmask);
+        case 5: return scan_prefix<4, 0>(code, s, t0, q0, len, tam, act, sg, nullptr, ht, hmask);
+        case 6: return scan_prefix<5, 0>(code, s, t0, q0, len, tam, act, sg, nullptr, ht, hmask);
+        default: return scan_prefix<6, 0>(code, s, t0, q0, len, tam, act, sg, nullptr, ht, hmask);
+    }
+}
+
 // Weight 1: every position with sig == t0, scanned by the wave 64 at a time.
 // MODE 0 returns the lowest matching position; MODE 1 the cheapest (ties: lowest).
 template <int MODE>
@@ -388,8 +407,8 @@ __device__ PrefHit weight_one(const DevCode &code, const uint32_t LDS *s, uint32
         const int i = i0 + lane;
         bool hit = false;
         if (i < tam && sg[i] == t0) {
-            int cols[kMaxW] = {act[i], 0, 0, 0, 0, 0};
-            hit = exact_match(code, s, cols, 1);
+            int cols[1] = {act[i]};
+            hit = exact_match<1>(code, s, cols, 1);
         }
         const float cost = (hit && MODE == 1) ? __fadd_rn(0.0f, llr[i]) : 0.0f;
         const uint32_t key = hit ? order_key(cost) : 0xFFFFFFFFu;
@@ -894,7 +913,7 @@ __device__ void finalize_shot(const DevCode &code, const DevCall &call, int shot
         uint32_t v = 0;
         if (o.found) {
 #pragma unroll
-            for (int l = 0; l < kMaxW; ++l)
+            for (int l = 0; l < kMaxWin; ++l)
                 if (l < o.fw && (o.win[l] >> 5) == wd) v |= 1u << (o.win[l] & 31);
         }
         dw[q] = v;
@@ -940,7 +959,7 @@ __device__ void finalize_shot(const DevCode &code, const DevCall &call, int shot
             uint32_t le = 0;
             if (o.found) {
 #pragma unroll
-                for (int l = 0; l < kMaxW; ++l)
+                for (int l = 0; l < kMaxWin; ++l)
                     if (l < o.fw) le ^= code.Lp[(size_t)o.win[l] * NL32 + wd];
             }
             call.declog[(size_t)shot * NL32 + wd] = le;
@@ -979,8 +998,8 @@ __device__ uint64_t learned_class(const DevCode &code, const DevCall &call, cons
             const int i = i0 + lane;
             bool hit = false;
             if (i < k && (uint64_t)i < lim && L.sigP[i] == t2) {
-                int cols[kMaxW] = {L.P[i], 0, 0, 0, 0, 0};
-                hit = exact_match(code, s2, cols, 1);
+                int cols[1] = {L.P[i]};
+                hit = exact_match<1>(code, s2, cols, 1);
             }
             const uint64_t b = ballot(hit);
             if (b) return (uint64_t)(i0 + __ffsll((long long)b) - 1);
@@ -995,7 +1014,7 @@ __device__ uint64_t learned_class(const DevCode &code, const DevCall &call, cons
         const int R = (int)(r64 < (uint64_t)call.rmax ? r64 : (uint64_t)call.rmax);
         const uint64_t q0 = pos + (uint64_t)lane * (uint64_t)R;
         const int len = q0 >= NP ? 0 : (int)((NP - q0) < (uint64_t)R ? (NP - q0) : (uint64_t)R);
-        const PrefHit h = scan_prefix_w<0>(wp, code, s2, t2, q0, len, k, L.P, L.sigP, nullptr, L.ht, hmask);
+        const PrefHit h = scan_prefix_learned(wp, code, s2, t2, q0, len, k, L.P, L.sigP, L.ht, hmask);
         const uint64_t b = ballot(h.j >= 0);
         if (b) {
             const uint64_t r = (uint64_t)__shfl((long long)h.rank, __ffsll((long long)b) - 1, 64);
@@ -1023,7 +1042,7 @@ __device__ bool learned_pass(const DevCode &code, const DevCall &call, int shot,
     sort_by_key(L, tam);
     const unsigned hmask = build_ht(L.ht, L.sigP, tam);
     const uint64_t budget = call.lbudget;
-    const int maxw = call.lmaxw < kMaxW ? call.lmaxw : kMaxW;
+    const int maxw = call.lmaxw < kMaxWin ? call.lmaxw : kMaxWin;
     uint64_t G = 0;
     bool stop = false;
     for (int k = 0; k < tam && !stop; ++k) {
@@ -1046,11 +1065,11 @@ __device__ bool learned_pass(const DevCode &code, const DevCall &call, int shot,
                 r = learned_class(code, call, L.sw2, t2, wp, k, lim, L, hmask);
             }
             if (r != ~0ull) {
-                int c[kMaxW] = {0, 0, 0, 0, 0, 0};
-                if (wp > 0) unrank_dyn(r, k, wp, code.nCr, c);
+                int c[kMaxWin] = {0, 0, 0, 0, 0, 0, 0, 0};
+                if (wp > 0) unrank_dyn<kMaxWin>(r, k, wp, code.nCr, c);
                 o.fw = wp + 1;
 #pragma unroll
-                for (int l = 0; l < kMaxW; ++l) o.win[l] = (l < wp) ? L.P[c[l]] : ((l == wp) ? ck : -1);
+                for (int l = 0; l < kMaxWin; ++l) o.win[l] = (l < wp) ? L.P[c[l]] : ((l == wp) ? ck : -1);
                 o.count += G + r + 1;
                 o.wops += (G + r + 1) * (uint64_t)(wp + 2);
                 o.found = 1;
@@ -1081,7 +1100,7 @@ __device__ void decode_shot_wave(const DevCode &code, const DevCall &call, int s
 
     o.found = 0; o.fw = 0; o.count = 0; o.wops = 0; o.hibrido = -2;
 #pragma unroll
-    for (int l = 0; l < kMaxW; ++l) o.win[l] = -1;
+    for (int l = 0; l < kMaxWin; ++l) o.win[l] = -1;
     int it = 0, tam = 0, tam_old = 0;
     const bool learned_hit = call.lscores && call.kind != kML && learned_pass(code, call, shot, s, t0, L, o);
     while (!learned_hit && it < code.max_soma && it < 3) {
@@ -1114,9 +1133,9 @@ __device__ void decode_shot_wave(const DevCode &code, const DevCall &call, int s
     // the decoded columns are reported sorted (the ML and learned winners are in score order)
     if (o.found) {
 #pragma unroll
-        for (int a = 0; a < kMaxW; ++a)
+        for (int a = 0; a < kMaxWin; ++a)
 #pragma unroll
-            for (int b = 0; b < kMaxW - 1 - a; ++b)
+            for (int b = 0; b < kMaxWin - 1 - a; ++b)
                 if (b + 1 < o.fw && o.win[b] > o.win[b + 1]) { const int t = o.win[b]; o.win[b] = o.win[b + 1]; o.win[b + 1] = t; }
     }
     o.t1 = now_ticks();
@@ -1509,9 +1528,9 @@ __global__ __launch_bounds__(64) void plain_finalize_kernel(const DevCode *__res
     }
     if (o.found) {
 #pragma unroll
-        for (int a = 0; a < kMaxW; ++a)
+        for (int a = 0; a < kMaxWin; ++a)
 #pragma unroll
-            for (int b = 0; b < kMaxW - 1 - a; ++b)
+            for (int b = 0; b < kMaxWin - 1 - a; ++b)
                 if (b + 1 < o.fw && o.win[b] > o.win[b + 1]) { const int t = o.win[b]; o.win[b] = o.win[b + 1]; o.win[b + 1] = t; }
     }
     finalize_shot(code, call, shot, o);

@@ -6,7 +6,8 @@
 
 namespace grand {
 
-constexpr int kMaxW = GRAND_MAX_W;   // largest enumerated weight
+constexpr int kMaxW = GRAND_MAX_W;   // largest enumerated weight of the plain search
+constexpr int kMaxWin = 8;            // most columns of a decoded pattern (the learned pass goes up to 8)
 constexpr int kMaxTam = 1024;        // largest active set (reference: 300, grand_decoder.cu:13)
 constexpr int kMaxWords = 16;        // 32-bit words per packed column -> n <= 512 checks
 constexpr int kMaxM = 4096;          // columns -> <= 128 packed words per pattern
@@ -95,7 +96,7 @@ struct DevCall {
     const float *lscores;             // per shot: [lstride] scores, by slot of the first active list or by column
     int lstride;
     int lper_column;                  // 1: lscores indexed by column, 0: by position in the first active list
-    int lmaxw;                        // pattern weights 1..lmaxw are tried (<= kMaxW)
+    int lmaxw;                        // pattern weights 1..lmaxw are tried (<= kMaxWin)
     int lpad;
     unsigned long long lbudget;       // patterns the learned pass may test before plain search takes over
 };
@@ -103,7 +104,7 @@ struct DevCall {
 // Search outcome of one shot, handed from a search engine to the finalizer.
 struct ShotOutcome {
     int found, fw, iteration, tam, hibrido;
-    int win[kMaxW];                   // decoded columns (ascending within the pattern order)
+    int win[kMaxWin];                 // decoded columns (ascending within the pattern order)
     unsigned long long count;
     unsigned long long wops;          // sum over tested patterns of (weight + 1)
     unsigned long long t0, t1;        // device timestamps

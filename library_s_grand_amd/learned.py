@@ -15,7 +15,7 @@ plain search's decoded pattern contains it.
 
 Decoding with the scores (Session.decode_learned) runs, per shot, a learned pass
 over the first active list: positions in descending score order, patterns of
-weight 1..6 tried so that a pattern comes as soon as its worst-scored position
+weight 1..8 tried so that a pattern comes as soon as its worst-scored position
 does, every candidate checked exactly against the syndrome, at most ``budget``
 patterns; then plain GRAND as usual.  Counts include both.
 
@@ -244,7 +244,7 @@ class DemTask:
         from .session import Session
         return Session(KIND_PROB_CLN, self.H, self.L, self.cfg, n_logicos=self.NL, file=self.phase_file)
 
-    def decode(self, sess, det, lg, scores=None, per_column=False, budget=100_000, max_weight=6, chunk=200_000):
+    def decode(self, sess, det, lg, scores=None, per_column=False, budget=100_000, max_weight=8, chunk=200_000):
         """Plain (scores None) or learned decode on the device; returns the
         result records (numpy), decoded pattern bits (torch bool [B][m]) and
         decoded logicals (torch int32 [B][NL])."""
@@ -266,8 +266,9 @@ class DemTask:
             else:
                 sess.decode_learned(det[i:], False, res[i * 72:], scores[i:], per_column, max_weight=max_weight,
                                     budget=budget, **args)
-        if sess.error_bits(stream) != 0:
-            raise RuntimeError("decoder error bits")
+            bits_raised = sess.error_bits(stream)
+            if bits_raised != 0:
+                raise RuntimeError(f"decoder error bits {bits_raised} in shots [{i}, {i + k})")
         r = results_from_bytes(res.cpu().numpy())
         bits = ((dec.view(torch.int32).unsqueeze(2) >> torch.arange(32, device="cuda", dtype=torch.int32)) & 1)
         bits = bits.reshape(B, mw * 32)[:, :self.m].bool()
@@ -275,11 +276,20 @@ class DemTask:
         lbits = lbits.reshape(B, lw * 32)[:, :self.NL].to(torch.int32)
         return r, bits, lbits
 
-    def labels(self, det, dec_bits, found, target):
+    def labels(self, det, dec_bits, found, target, chunk=200_000):
         """Targets and masks for the network from the plain decoder's patterns.
         slot: [B][max_size], position j of the first active list; column: [B][m].
         Shots the plain decoder did not solve, or whose first list is empty or
         longer than max_size, are masked out."""
+        torch = self.torch
+        ys, ms = [], []
+        for i in range(0, det.shape[0], chunk):
+            y, m = self._labels(det[i:i + chunk], dec_bits[i:i + chunk], found[i:i + chunk], target)
+            ys.append(y)
+            ms.append(m)
+        return torch.cat(ys), torch.cat(ms)
+
+    def _labels(self, det, dec_bits, found, target):
         torch = self.torch
         act, tam = self.first_active(det)
         usable = (torch.from_numpy(found.astype(np.bool_)).to(det.device)) & (tam > 0) & (tam <= self.max_size)
@@ -337,7 +347,7 @@ def train(task, model, det, y, mask, epochs=4, batch=4096, lr=2e-3, seed=0, log=
 
 
 def experiment(train_shots=2_000_000, eval_shots=1_000_000, target="column", epochs=4, budget=100_000,
-               max_weight=6, seed=2024, log=print):
+               max_weight=8, seed=2024, log=print):
     """Trains on plain-decoder pairs and compares plain vs learned decoding on
     held-out shots: mean patterns per shot and logical error rate."""
     import torch
@@ -395,7 +405,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--target", choices=("slot", "column"), default="column")
     ap.add_argument("--budget", type=int, default=100_000)
-    ap.add_argument("--max-weight", type=int, default=6)
+    ap.add_argument("--max-weight", type=int, default=8)
     a = ap.parse_args()
     print(json.dumps(experiment(a.train, a.eval, a.target, a.epochs, a.budget, a.max_weight)), flush=True)
 

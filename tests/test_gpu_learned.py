@@ -120,19 +120,19 @@ def test_learned_pass_matches_cpu_statement(oracle):
     det_np, lg_np = det.cpu().numpy(), lg.cpu().numpy()
     rc, o = T.run_cln(oracle, G.KIND_PROB_CLN, task.H, task.L, det_np, lg_np, cfg=cfg, file=task.phase_file)[:2]
     assert rc == 0
-    for per_column, budget in ((True, 50), (True, 20_000), (False, 2000)):
+    for per_column, budget, maxw in ((True, 50, 4), (True, 20_000, 4), (False, 2000, 4), (True, 3000, 8)):
         width = task.m if per_column else task.max_size
         scores = rng.standard_normal((det.shape[0], width)).astype(np.float32)
         scores[0, :] = 0.0                                          # ties keep list order
         r, bits, _ = task.decode(sess, det, lg, scores=torch.from_numpy(scores).cuda(), per_column=per_column,
-                                 budget=budget, max_weight=4)
+                                 budget=budget, max_weight=maxw)
         bits = bits.cpu().numpy()
         for i in range(det.shape[0]):
             s = det_np[i]
             if not s.any():
                 assert r["found"][i] == 0 and r["total_tested_combinations"][i] == 0
                 continue
-            hit, tried, tam = learned_ref.learned_pass(task.H, s, scores[i], per_column, 4, budget, task.max_size)
+            hit, tried, tam = learned_ref.learned_pass(task.H, s, scores[i], per_column, maxw, budget, task.max_size)
             if hit is not None:
                 assert r["found"][i] == 1 and r["iteration"][i] == 1 and r["tam"][i] == tam, i
                 assert r["total_tested_combinations"][i] == tried, i
@@ -178,4 +178,20 @@ def test_training_gain_over_plain():
     print(f"mean patterns plain {mean_pl:.1f} learned {mean_le:.1f}; LER plain {ler_pl:.4f} learned {ler_le:.4f}")
     assert mean_le < mean_pl
     assert ler_le <= ler_pl + 3 * np.sqrt(ler_pl * (1 - ler_pl) / det_e.shape[0])
+    sess.close()
+
+
+def test_session_decode_is_ordered_after_default_stream_work():
+    """Inputs written by torch kernels on the default stream and decoded at once
+    (no synchronization in between) give the results of decoding them after a
+    full synchronization: a NULL stream means the default stream."""
+    task = _small_dem_task()
+    sess = task.session()
+    det, lg = task.sample(400_000, seed=31)          # queued on the default stream, not waited for
+    r1, b1, l1 = task.decode(sess, det, lg)
+    torch.cuda.synchronize()
+    r2, b2, l2 = task.decode(sess, det, lg)
+    for f in G.PARITY_FIELDS:
+        assert (r1[f] == r2[f]).all(), f
+    assert torch.equal(b1, b2) and torch.equal(l1, l2)
     sess.close()
