@@ -29,6 +29,7 @@ import argparse
 import ctypes
 import json
 import os
+import sys
 import tempfile
 import time
 
@@ -407,7 +408,8 @@ def main():
     ap.add_argument("--budget", type=int, default=100_000)
     ap.add_argument("--max-weight", type=int, default=8)
     a = ap.parse_args()
-    print(json.dumps(experiment(a.train, a.eval, a.target, a.epochs, a.budget, a.max_weight)), flush=True)
+    log = lambda msg: print(msg, file=sys.stderr, flush=True)   # noqa: E731
+    print(json.dumps(experiment(a.train, a.eval, a.target, a.epochs, a.budget, a.max_weight, log=log)), flush=True)
 
 
 if __name__ == "__main__":
