@@ -104,6 +104,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-engine leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-order leg")
+    ap.add_argument("--no-tune", action="store_true", help="keep the built-in tuning defaults")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB twice (a kernel of known traffic "
                          "for scaling hardware byte counters)")
@@ -144,6 +145,16 @@ def main():
               f"copy", file=sys.stderr)
         del src, dst
 
+    tuning = None
+    if not a.no_tune:
+        from library_s_grand_amd import autotune
+
+        def tune_run():
+            step()
+            return sess.last_timing()
+
+        best, best_ms, table = autotune.tune(lib, tune_run, reps=3, max_rounds=2)
+        tuning = {"chosen": best, "search_ms": best_ms, "configs_measured": len(table)}
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
@@ -161,6 +172,20 @@ def main():
     ms_per_step = 1e3 * wall / a.steps
     res = results_from_bytes(d_res.cpu().numpy())
     stats = d_stats.cpu().numpy().view(np.uint64).reshape(shots, 3)
+
+    # the same step replayed from a HIP graph (one graph launch per step)
+    gstream = torch.cuda.Stream()
+    graph = sess.capture(d_err, True, d_res, gstream.cuda_stream, d_decoded=d_dec, d_stats=d_stats)
+    graph.launch(gstream.cuda_stream)
+    gstream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        graph.launch(gstream.cuda_stream)
+    gstream.synchronize()
+    graph_ms = 1e3 * (time.perf_counter() - t0) / a.steps
+    graph.close()
+    res_g = results_from_bytes(d_res.cpu().numpy())
+    graph_same = all(bool((res_g[f] == res[f]).all()) for f in PARITY_FIELDS)
 
     searched = stats[:, 1] > 0
     lat_us = (stats[searched, 1] - stats[searched, 0]).astype(np.float64) * sess.ms_per_tick * 1e3
@@ -193,6 +218,9 @@ def main():
         "latency_basis": "device service time of each searched shot (claim to completion); zero-syndrome "
                          "shots finish in the triage kernel",
         "searched_shots": int(searched.sum()),
+        "graph_replay": {"ms_per_step": graph_ms, "shots_per_s": shots / (graph_ms * 1e-3),
+                         "bit_identical_to_eager": graph_same},
+        "tuning": tuning,
         "patterns_per_s": patterns / (ms_per_step * 1e-3),
         "kernel_ms": {"triage": tri, "search": srch},
         "roofline": {"bound": "valu", "achieved": achieved_int, "peak": peak_int, "unit": "TOP/s",

@@ -71,6 +71,21 @@ int grand_session_decode_learned(GrandSession *s, const int *d_input, int input_
                                  unsigned *d_decoded, unsigned *d_decoded_logicals, unsigned long long *d_stats,
                                  void *stream);
 
+/* HIP graph of one decode, for callers that repeat the same call (same
+ * buffers and shot count, new contents).  grand_session_capture records the
+ * kernels grand_session_decode_device would queue (descriptor install,
+ * triage, search; no memory set or copy nodes) on `stream`, which must be a
+ * non-default stream, and instantiates them; every grand_graph_launch decodes
+ * the current contents of the buffers.  Allocation happens before capture.
+ * Launches of one graph, and decodes on its session, must not overlap.  Not
+ * for the plain engine.  Returns NULL on failure. */
+typedef struct GrandGraph GrandGraph;
+GrandGraph *grand_session_capture(GrandSession *s, const int *d_input, int input_is_errors, const int *d_logicals,
+                                  int num_shots, GrandDecoderResult *d_results, unsigned *d_decoded,
+                                  unsigned *d_decoded_logicals, unsigned long long *d_stats, void *stream);
+int grand_graph_launch(GrandGraph *g, void *stream);
+void grand_graph_destroy(GrandGraph *g);
+
 /* Error bits raised by the kernels of the last decode on this session
  * (1 = an input value other than 0/1, 2 = an active set larger than the
  * supported 1024 positions); synchronizes `stream`. */

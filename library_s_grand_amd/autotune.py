@@ -1,0 +1,91 @@
+"""Autotuner of the persistent search kernel.
+
+Knobs (grand_set_tuning, process-wide defaults used by every session that does
+not override them in its GrandDecoderExConfig):
+
+  waves_per_cu      persistent grid = CUs x waves_per_cu (residency is capped by
+                    the kernel's registers and LDS: 2 waves per SIMD today)
+  claim_batch       worklist shots taken per atomic claim (the kernel lowers it
+                    for short worklists)
+  rmax              prefixes per lane per chunk (chunk = 64 x rmax prefixes)
+  split_threshold   weight classes with more prefixes than this are shared
+                    between waves through the task table
+
+The tuner measures the search kernel with HIP events on the caller's workload
+(device-resident inputs, a session with timing enabled) and walks the knobs one
+at a time from the current defaults, keeping a change only when it is faster by
+more than the noise margin; it repeats the sweep until nothing improves.  The
+result can be saved as JSON and applied again (apply()).
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+
+import numpy as np
+
+from .session import bind_session
+
+SPACE = {
+    "waves_per_cu": (2, 4, 8, 16),
+    "claim_batch": (1, 2, 4, 8),
+    "rmax": (8, 16, 32, 64),
+    "split_threshold": (256, 1024, 2048, 8192, 32768),
+}
+KEYS = ("waves_per_cu", "claim_batch", "rmax", "split_threshold")
+
+
+def get(lib):
+    bind_session(lib)
+    v = [ctypes.c_int() for _ in KEYS]
+    lib.grand_get_tuning(*[ctypes.byref(x) for x in v])
+    return {k: x.value for k, x in zip(KEYS, v)}
+
+
+def apply(lib, cfg):
+    bind_session(lib)
+    lib.grand_set_tuning(*[int(cfg[k]) for k in KEYS])
+
+
+def measure(run, reps=5):
+    """Median search-kernel milliseconds of `run()` (which decodes once and
+    returns (triage_ms, search_ms))."""
+    run()   # warm the configuration
+    return float(np.median([run()[1] for _ in range(reps)]))
+
+
+def tune(lib, run, reps=5, margin=0.02, max_rounds=3, space=SPACE, log=None):
+    """Coordinate search from the current defaults; returns (best config,
+    its search ms, every measurement)."""
+    best = get(lib)
+    apply(lib, best)
+    best_ms = measure(run, reps)
+    table = [dict(best, search_ms=best_ms)]
+    for _ in range(max_rounds):
+        improved = False
+        for key in KEYS:
+            for val in space[key]:
+                if val == best[key]:
+                    continue
+                cand = dict(best, **{key: val})
+                apply(lib, cand)
+                ms = measure(run, reps)
+                table.append(dict(cand, search_ms=ms))
+                if log:
+                    log(f"{cand} -> {ms:.3f} ms")
+                if ms < best_ms * (1.0 - margin):
+                    best, best_ms, improved = cand, ms, True
+        if not improved:
+            break
+    apply(lib, best)
+    return best, best_ms, table
+
+
+def save(path, cfg):
+    with open(path, "w", encoding="utf-8") as f:
+        json.dump({k: int(cfg[k]) for k in KEYS}, f)
+
+
+def load(path):
+    with open(path, encoding="utf-8") as f:
+        return json.load(f)

@@ -926,6 +926,52 @@ int grand_session_decode_learned(GrandSession *s, const int *d_input, int input_
     return rc;
 }
 
+struct GrandGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~GrandGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+GrandGraph *grand_session_capture(GrandSession *s, const int *d_input, int input_is_errors, const int *d_logicals,
+                                  int num_shots, GrandDecoderResult *d_results, unsigned *d_decoded,
+                                  unsigned *d_decoded_logicals, unsigned long long *d_stats, void *stream) {
+    if (!s || !stream || num_shots <= 0 || !d_input || !d_results || s->ex.engine == GRAND_ENGINE_PLAIN) return nullptr;
+    if (s->family == kCLN && s->code->NL > 0 && !d_logicals) return nullptr;
+    hipStream_t st = (hipStream_t)stream;
+    // everything that allocates or clears memory happens outside the capture
+    if (grand_session_reserve(s, num_shots) != 0) return nullptr;
+    if (s->ws.tasks_fresh) {
+        if (!hip_check(grand_launch_fill(reinterpret_cast<unsigned *>(s->ws.tasks.p), 0u,
+                                         (int)(sizeof(TaskSlot) * kTaskSlots / 4), st), "fill"))
+            return nullptr;
+        s->ws.tasks_fresh = false;
+    }
+    auto g = std::make_unique<GrandGraph>();
+    const bool timing = s->timing;
+    s->timing = false;
+    if (!hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture")) {
+        s->timing = timing;
+        return nullptr;
+    }
+    const int rc = decode_device(s, d_input, input_is_errors, d_logicals, num_shots, d_results, d_decoded,
+                                 d_decoded_logicals, d_stats, st);
+    const hipError_t ec = hipStreamEndCapture(st, &g->graph);
+    s->timing = timing;
+    if (rc != 0 || !hip_check(ec, "hipStreamEndCapture")) return nullptr;
+    if (!hip_check(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "hipGraphInstantiate")) return nullptr;
+    return g.release();
+}
+
+int grand_graph_launch(GrandGraph *g, void *stream) {
+    if (!g || !g->exec) return -1;
+    return hip_check(hipGraphLaunch(g->exec, (hipStream_t)stream), "hipGraphLaunch") ? 0 : -1;
+}
+
+void grand_graph_destroy(GrandGraph *g) { delete g; }
+
 int grand_session_error_bits(GrandSession *s, void *stream) {
     if (!s || !s->call.counters) return 0;
     hipStream_t st = (hipStream_t)stream;   // NULL: the default stream, ordered with other default-stream work

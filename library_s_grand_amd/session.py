@@ -25,6 +25,12 @@ def bind_session(lib):
                                                  ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, _VP, _VP, _VP, _VP,
                                                  _VP]
     lib.grand_session_decode_learned.restype = ctypes.c_int
+    lib.grand_session_capture.argtypes = [_VP, _VP, ctypes.c_int, _VP, ctypes.c_int, _VP, _VP, _VP, _VP, _VP]
+    lib.grand_session_capture.restype = _VP
+    lib.grand_graph_launch.argtypes = [_VP, _VP]
+    lib.grand_graph_launch.restype = ctypes.c_int
+    lib.grand_graph_destroy.argtypes = [_VP]
+    lib.grand_graph_destroy.restype = None
     lib.grand_session_error_bits.argtypes = [_VP, _VP]
     lib.grand_session_enable_timing.argtypes = [_VP, ctypes.c_int]
     lib.grand_session_last_timing.argtypes = [_VP, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
@@ -98,6 +104,18 @@ class Session:
         if rc != 0:
             raise RuntimeError(f"grand_session_decode_learned returned {rc}")
 
+    def capture(self, d_input, input_is_errors, d_results, stream, d_logicals=None, d_decoded=None,
+                d_decoded_logicals=None, d_stats=None, num_shots=None):
+        """HIP graph of one decode on `stream` (a non-default stream handle);
+        launch() decodes the buffers' current contents."""
+        shots = num_shots if num_shots is not None else d_input.shape[0]
+        h = self.lib.grand_session_capture(self.h, _ptr(d_input), int(bool(input_is_errors)), _ptr(d_logicals), shots,
+                                           _ptr(d_results), _ptr(d_decoded), _ptr(d_decoded_logicals), _ptr(d_stats),
+                                           ctypes.c_void_p(stream))
+        if not h:
+            raise RuntimeError("grand_session_capture failed")
+        return Graph(self.lib, h)
+
     def error_bits(self, stream=None):
         return self.lib.grand_session_error_bits(self.h, None if stream is None else ctypes.c_void_p(stream))
 
@@ -116,6 +134,28 @@ class Session:
     def close(self):
         if self.h:
             self.lib.grand_session_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """A captured decode (grand_session_capture)."""
+
+    def __init__(self, lib, h):
+        self.lib, self.h = lib, h
+
+    def launch(self, stream):
+        if self.lib.grand_graph_launch(self.h, ctypes.c_void_p(stream)) != 0:
+            raise RuntimeError("grand_graph_launch failed")
+
+    def close(self):
+        if self.h:
+            self.lib.grand_graph_destroy(self.h)
             self.h = None
 
     def __del__(self):
