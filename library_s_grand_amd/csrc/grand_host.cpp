@@ -280,6 +280,8 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
         rank = rr;
         R.assign(rows.begin(), rows.begin() + (size_t)rank * M32);
     }
+    std::vector<int> pivrow(m, -1);
+    for (int r = 0; r < rank; ++r) pivrow[piv[r]] = r;
     std::vector<uint32_t> Lp;
     if (family == kCLN) {
         Lp.assign((size_t)m * NL32, 0);
@@ -300,7 +302,7 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
     const size_t iHcw = add(Hcw.data(), Hcw.size() * 4), iHcm = add(Hcm.data(), Hcm.size() * 4),
                  iHrow = add(Hrow.data(), Hrow.size() * 4), iSig = add(sig.data(), sig.size() * 4),
                  iRs = add(rowsig.data(), rowsig.size() * 4), iOff = add(off.data(), off.size() * 4),
-                 iR = add(R.data(), R.size() * 4), iPiv = add(piv.data(), piv.size() * 4),
+                 iR = add(R.data(), R.size() * 4), iPiv = add(pivrow.data(), pivrow.size() * 4),
                  iL = add(Lp.data(), Lp.size() * 4),
                  iLlr = add(llr ? llr->data() : nullptr, llr ? llr->size() * 4 : 0);
     if (!ce->mem.ensure(total)) return nullptr;
@@ -320,7 +322,7 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
     dc.rowsig = (const uint32_t *)(base + segs[iRs].off);
     dc.off = (const int *)(base + segs[iOff].off);
     dc.R = (const uint32_t *)(base + segs[iR].off);
-    dc.piv = (const int *)(base + segs[iPiv].off);
+    dc.pivrow = (const int *)(base + segs[iPiv].off);
     dc.Lp = (const uint32_t *)(base + segs[iL].off);
     dc.llr = (const float *)(base + segs[iLlr].off);
     dc.nCr = rt().nCr.as<unsigned long long>();

@@ -590,14 +590,15 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
 __device__ bool help_task(const DevCode &code, const DevCall &call, const WaveLds &L, int &ctask, unsigned &hmask) {
     const int lane = lane_id();
     if (__builtin_amdgcn_readfirstlane(ld_relaxed(&call.task_counters[0])) == 0u) return false;
-    int slot = -1;
+    static_assert(kTaskSlots == 128, "the open-slot scan reads two 64-slot halves");
     const int start = (int)((blockIdx.x * 37u) & (kTaskSlots - 1));
-    for (int b0 = 0; b0 < kTaskSlots; b0 += 64) {
-        const int k = (start + b0 + lane) & (kTaskSlots - 1);
-        const unsigned long long c = ld_relaxed(&call.tasks[k].ctl);
-        const uint64_t b = ballot(ctl_next(c) < ctl_nch(c));
-        if (b) { slot = (int)uni((unsigned)(start + b0 + __ffsll((long long)b) - 1) & (kTaskSlots - 1)); break; }
-    }
+    const unsigned long long c0 = ld_relaxed(&call.tasks[(start + lane) & (kTaskSlots - 1)].ctl);
+    const unsigned long long c1 = ld_relaxed(&call.tasks[(start + 64 + lane) & (kTaskSlots - 1)].ctl);
+    const uint64_t b0 = ballot(ctl_next(c0) < ctl_nch(c0));
+    const uint64_t b1 = ballot(ctl_next(c1) < ctl_nch(c1));
+    int slot = -1;
+    if (b0) slot = (int)uni((unsigned)(start + __ffsll((long long)b0) - 1) & (kTaskSlots - 1));
+    else if (b1) slot = (int)uni((unsigned)(start + 64 + __ffsll((long long)b1) - 1) & (kTaskSlots - 1));
     if (slot < 0) return false;
     TaskSlot *T = &call.tasks[slot];
     unsigned g;
@@ -936,20 +937,34 @@ __device__ void finalize_shot(const DevCode &code, const DevCall &call, int shot
         ow = pc;
         if (o.found) {
             eq = ballot(!same) == 0ull;
-            uint32_t dl[2] = {ew[0] ^ dw[0], ew[1] ^ dw[1]};
-            for (int r = 0; r < code.rank; ++r) {
-                const int p = code.piv[r];
-                const int wd = p >> 5;
-                const uint32_t word = bcast_u((wd >> 6) ? dl[1] : dl[0], wd & 63);
-                if ((word >> (p & 31)) & 1u) {
+            // e ^ e_hat is in the row space of the other matrix iff it equals the
+            // XOR of the RREF rows whose pivot bit it has set: in reduced echelon
+            // form a pivot column is zero outside its own row, so adding rows never
+            // changes which other pivots are set.  The residue is sparse; its set
+            // bits are looked up in pivrow and the selected rows XORed (independent
+            // loads, no chain through the rank).
+            const uint32_t dl[2] = {ew[0] ^ dw[0], ew[1] ^ dw[1]};
+            uint32_t acc[2] = {0u, 0u};
+            uint32_t pend[2] = {dl[0], dl[1]};
+            for (;;) {
+                int c = -1;
+                if (pend[0]) { c = lane * 32 + __ffs(pend[0]) - 1; pend[0] &= pend[0] - 1u; }
+                else if (pend[1]) { c = (lane + 64) * 32 + __ffs(pend[1]) - 1; pend[1] &= pend[1] - 1u; }
+                if (ballot(c >= 0) == 0ull) break;
+                const int r = (c >= 0) ? code.pivrow[c] : -1;
+                uint64_t sel = ballot(r >= 0);
+                while (sel) {
+                    const int src = __ffsll((long long)sel) - 1;
+                    sel &= sel - 1;
+                    const int rr = bcast_i(r, src);
 #pragma unroll
                     for (int q = 0; q < 2; ++q) {
                         const int w2 = lane + 64 * q;
-                        if (w2 < M32) dl[q] ^= code.R[(size_t)r * M32 + w2];
+                        if (w2 < M32) acc[q] ^= code.R[(size_t)rr * M32 + w2];
                     }
                 }
             }
-            deg = ballot((dl[0] | dl[1]) != 0u) == 0ull;
+            deg = ballot(((dl[0] ^ acc[0]) | (dl[1] ^ acc[1])) != 0u) == 0ull;
         }
     } else if (call.family == kCLN) {
         const int NL32 = code.NL32;

@@ -35,7 +35,7 @@ struct DevCode {
     const uint32_t *rowsig;           // [n] signature of each unit row vector
     const int *off;                   // [m] score offset (irregular codes), kNegScore for empty columns
     const uint32_t *R;                // [rank][M32] RREF of the other CSS matrix (cc)
-    const int *piv;                   // [rank]
+    const int *pivrow;                // [m] RREF row whose pivot is this column, -1 for other columns
     const uint32_t *Lp;               // [m][NL32] logical flips per mechanism (cln)
     const float *llr;                 // [m] (ML)
     const unsigned long long *nCr;    // [(kMaxTam + 1) * kNcrStride]
@@ -44,7 +44,7 @@ struct DevCode {
 // A weight class too large for one wave, shared through the task table.  The
 // owner wave publishes it; any wave claims chunks of 64 * R prefixes with one
 // fetch-and-add on ctl = (generation << 52) | (nchunks << 32) | next_chunk.
-constexpr int kTaskSlots = 1024;
+constexpr int kTaskSlots = 128;
 struct TaskSlot {
     unsigned long long ctl;           // generation | nchunks | next chunk to claim
     unsigned long long best;          // first-hit: lowest matching rank; ML: (cost key << 32) | rank

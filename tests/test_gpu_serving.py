@@ -51,13 +51,15 @@ def test_mixed_requests_equal_alone_and_oracle(oracle):
 def test_hbm_budget_admission():
     hx, hz = codes.bb(144)
     cfg = G.make_config(max_error_weight=3)
-    with serving.Server(max_batch_shots=1024, lanes_per_code=1, hbm_budget_bytes=1 << 20) as srv:
-        with pytest.raises(MemoryError):
-            srv.add_code(G.KIND_CC, hx, hz, cfg)
     with serving.Server(max_batch_shots=1024, lanes_per_code=1, hbm_budget_bytes=1 << 34) as srv:
         cid = srv.add_code(G.KIND_CC, hx, hz, cfg)
         one = srv.footprint(cid)
         assert one > 0 and srv.footprint() == one
+    with serving.Server(max_batch_shots=1024, lanes_per_code=1, hbm_budget_bytes=one - 1) as srv:
+        with pytest.raises(MemoryError):
+            srv.add_code(G.KIND_CC, hx, hz, cfg)
+    with serving.Server(max_batch_shots=1024, lanes_per_code=1, hbm_budget_bytes=one) as srv:
+        srv.add_code(G.KIND_CC, hx, hz, cfg)
     with serving.Server(max_batch_shots=1024, lanes_per_code=1, hbm_budget_bytes=one + one // 2) as srv:
         srv.add_code(G.KIND_CC, hx, hz, cfg)
         with pytest.raises(MemoryError):
