@@ -409,7 +409,7 @@ int kind_search(int kind) {
 }
 
 // Autotuned defaults (see tune_* in the Python package); overridable per call.
-struct Tuning { int waves_per_cu = 8; int claim_batch = 4; int rmax = 32; int split_threshold = 2048; };
+struct Tuning { int waves_per_cu = 4; int claim_batch = 2; int rmax = 32; int split_threshold = 4096; };
 Tuning g_tuning;
 
 // Validates exactly as the reference driver for `kind` would, then applies the

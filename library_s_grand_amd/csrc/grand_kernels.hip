@@ -131,6 +131,29 @@ __device__ __forceinline__ void unrank_dyn(uint64_t r, int tam, int w, const uns
     }
 }
 
+// Unranking in a class small enough that every binomial it reads is below
+// 2^32 (tam <= 128), from the wave's LDS copy of the table.
+__device__ __forceinline__ void unrank_lds(uint32_t r, int tam, int w, const uint32_t LDS *t,
+                                           int *c) {
+    int prev = -1;
+#pragma unroll
+    for (int i = 0; i < kMaxW; ++i) {
+        if (i < w) {
+            const int k = w - i;
+            const uint32_t tot = t[(tam - 1 - prev) * kNcrStride + k];
+            const uint32_t T = tot - r;
+            int lo = prev + 1, hi = tam - k;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (t[(tam - mid) * kNcrStride + k] >= T) lo = mid; else hi = mid - 1;
+            }
+            r -= tot - t[(tam - lo) * kNcrStride + k];
+            c[i] = lo;
+            prev = lo;
+        }
+    }
+}
+
 // ------------------------------------------------------------- wave shot state
 // Per-wave LDS layout (cap entries each): A (active columns) and their
 // signatures; for ML also the llr of A and the LLR-sorted copies P / sigP /
@@ -147,10 +170,13 @@ struct WaveLds {
     uint32_t LDS *ew;  // packed error words of the shot (triage), kMaxM / 32
     uint32_t LDS *sw;  // syndrome words of the shot being searched, kMaxWords
     uint32_t LDS *sw2; // syndrome minus one column (learned pass), kMaxWords
+    uint32_t LDS *ncr; // C(n, k) for n <= 128, k < 8 (saturated to 32 bits): unranking in small classes
 };
 
 extern __shared__ uint32_t lds_raw[];
 __device__ __forceinline__ uint32_t LDS *lds_base() { return (uint32_t LDS *)lds_raw; }
+
+constexpr int kNcrLdsRows = 129;   // n = 0..128
 
 __host__ __device__ __forceinline__ int ht_capacity(int cap) {
     int h = 64;
@@ -164,7 +190,8 @@ __device__ __forceinline__ WaveLds carve_lds(int cap, bool ml) {
     L.ew = base;
     L.sw = base + kMaxM / 32;
     L.sw2 = L.sw + kMaxWords;
-    uint32_t LDS *p = L.sw2 + kMaxWords;
+    L.ncr = L.sw2 + kMaxWords;
+    uint32_t LDS *p = L.ncr + kNcrLdsRows * kNcrStride;
     L.ht = (unsigned long long LDS *)p; p += 2 * ht_capacity(cap);
     L.A = (int LDS *)p; p += cap;
     L.sigA = p; p += cap;
@@ -200,16 +227,21 @@ __device__ __forceinline__ bool exact_match(const DevCode &code, const uint32_t 
 __device__ int build_active(const DevCode &code, const uint32_t LDS *s, int thr, const WaveLds &L, int cap) {
     const int lane = lane_id();
     int tam = 0;
-    for (int c0 = 0; c0 < code.m; c0 += 64) {
+    const int D = code.D, m = code.m;
+    for (int c0 = 0; c0 < m; c0 += 64) {
         const int c = c0 + lane;
-        int score = kNegScore;
-        if (c < code.m) {
-            int sc = 0;
-#pragma unroll 1
-            for (int d = 0; d < code.D; ++d) sc += __popc(code.Hcw[(size_t)d * code.m + c] & s[d]);
-            score = sc + code.off[c];
+        const int cc = c < m ? c : m - 1;            // in-bounds column for the loads
+        int sc = code.off[cc];
+        // column words four at a time (indices clamped, so the loads issue together)
+        for (int d0 = 0; d0 < D; d0 += 4) {
+            uint32_t h[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) h[j] = code.Hcw[(size_t)((d0 + j < D) ? d0 + j : D - 1) * m + cc];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (d0 + j < D) sc += __popc(h[j] & s[d0 + j]);
         }
-        const bool on = score >= thr;
+        const bool on = c < m && sc >= thr;
         const uint64_t b = ballot(on);
         if (on) {
             const int p = tam + lanes_below(b);
@@ -718,7 +750,7 @@ __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, int sho
                     int k = kc;
                     while (idx >= cs + N[k]) { cs += N[k]; ++k; }
                     w = wl[k];
-                    unrank_dyn(idx - cs, tam, w, code.nCr, c);
+                    unrank_lds((uint32_t)(idx - cs), tam, w, L.ncr, c);   // flat classes have tam <= 128
                     uint32_t x = t0;
 #pragma unroll
                     for (int l = 0; l < kMaxW; ++l) if (l < w) x ^= L.sigA[c[l]];
@@ -1313,28 +1345,50 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
     B = B < 1u ? 1u : (B > (unsigned)call.claim_batch ? (unsigned)call.claim_batch : B);
     int ctask = -1;        // task whose arrays and hash table this wave's LDS holds
     unsigned hmask = 0;
+    for (int i = lane_id(); i < kNcrLdsRows * kNcrStride; i += 64) {
+        const unsigned long long v = code.nCr[i];
+        L.ncr[i] = v > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)v;
+    }
+    __syncthreads();
+    // Each iteration issues the claim and the open-task poll together (one round
+    // trip).  The claim word holds (waves started << 32) | claim cursor: a wave
+    // counts itself started with one RMW on entry and claims with another, so
+    // every wave holding claimed shots is counted in the word before the cursor
+    // passes the end (RMWs on one location are totally ordered).  A wave counts
+    // itself finished once it finds the worklist drained.  Only a wave still
+    // decoding can publish a shared class, so helpers leave when finished ==
+    // started and no class is open; waves not yet resident never hold work.
+    unsigned long long *claim_word = reinterpret_cast<unsigned long long *>(call.counters + 8);
+    if (lane_id() == 0) atomicAdd(claim_word, 1ull << 32);
+    bool drained = false;
     for (;;) {
-        // shared classes of deep shots first: they bound the batch's completion time
-        if (call.split_threshold && help_task(code, call, L, ctask, hmask)) continue;
-        unsigned first = uni(ld_relaxed(&call.counters[1]));
-        if (first < total) {
-            unsigned got = 0;
-            if (lane_id() == 0) got = atomicAdd(&call.counters[1], B);
-            first = uni(got);
+        unsigned got = 0xFFFFFFFFu, open = 0u;
+        if (lane_id() == 0) {
+            if (call.split_threshold) open = ld_relaxed(&call.task_counters[0]);
+            if (!drained) got = (unsigned)atomicAdd(claim_word, (unsigned long long)B);
         }
-        if (first < total) {
-            if (lane_id() == 0) atomicAdd(&call.task_counters[1], 1u);
-            const unsigned last = (first + B < total) ? first + B : total;
-            for (unsigned i = first; i < last; ++i) decode_shot_wave(code, call, call.worklist[i], L);
+        open = uni(open);
+        got = uni(got);
+        // shared classes of deep shots first: they bound the batch's completion time
+        bool helped = false;
+        if (open != 0u)
+            while (help_task(code, call, L, ctask, hmask)) helped = true;
+        if (!drained && got < total) {
+            const unsigned last = (got + B < total) ? got + B : total;
+            for (unsigned i = got; i < last; ++i) decode_shot_wave(code, call, call.worklist[i], L);
             ctask = -1;
-            if (lane_id() == 0) atomicSub(&call.task_counters[1], 1u);
             continue;
         }
+        if (!drained) {
+            drained = true;
+            if (lane_id() == 0) atomicAdd(&call.task_counters[1], 1u);
+        }
         if (!call.split_threshold) break;
-        // queue drained: stay while shots are in flight, to help their shared classes
-        const unsigned inflight = __builtin_amdgcn_readfirstlane(ld_relaxed(&call.task_counters[1]));
-        const unsigned open = __builtin_amdgcn_readfirstlane(ld_relaxed(&call.task_counters[0]));
-        if (inflight == 0u && open == 0u) break;
+        if (helped) continue;
+        const unsigned fin = uni(ld_relaxed(&call.task_counters[1]));     // finished first, then started
+        const unsigned started = (unsigned)(uni64(ld_relaxed(claim_word)) >> 32);
+        const unsigned op = uni(ld_relaxed(&call.task_counters[0]));
+        if (fin == started && op == 0u) break;
         __builtin_amdgcn_s_sleep(2);
     }
 }
@@ -1568,7 +1622,7 @@ __global__ void prep_kernel(const DevCall c, DevCall *dst, unsigned *counters) {
     const uint32_t *src = reinterpret_cast<const uint32_t *>(&c);
     uint32_t *d = reinterpret_cast<uint32_t *>(dst);
     for (int i = threadIdx.x; i < (int)(sizeof(DevCall) / 4); i += blockDim.x) d[i] = src[i];
-    if (threadIdx.x < 8) counters[threadIdx.x] = 0u;
+    if (threadIdx.x < 16) counters[threadIdx.x] = 0u;
 }
 
 __global__ void fill_u32_kernel(unsigned *p, unsigned v, int n) {
@@ -1586,7 +1640,8 @@ using grand::DevCall;
 using grand::DevCode;
 
 size_t grand_fused_lds_bytes(int cap, int ml) {
-    return sizeof(uint32_t) * ((size_t)grand::kMaxM / 32 + 2 * grand::kMaxWords + 2 * (size_t)grand::ht_capacity(cap) +
+    return sizeof(uint32_t) * ((size_t)grand::kMaxM / 32 + 2 * grand::kMaxWords +
+                               (size_t)grand::kNcrLdsRows * grand::kNcrStride + 2 * (size_t)grand::ht_capacity(cap) +
                                (size_t)cap * (ml ? 6 : 2));
 }
 

@@ -81,14 +81,14 @@ struct DevCall {
     uint32_t *lg_packed;              // [shots][NL32] (cln)
     uint32_t *s_sig;                  // [shots]
     int *worklist;                    // nonzero-syndrome shots
-    unsigned *counters;               // [0] worklist length, [1] claim cursor, [2] error bits
+    unsigned *counters;               // [0] worklist length, [2] error bits, [8..9] claim word (u64); 16 words
     GrandDecoderResult *results;      // [shots]
     uint32_t *dec;                    // [shots][M32] decoded pattern
     uint32_t *declog;                 // [shots][NL32] decoded logicals (cln)
     unsigned long long *stats;        // [shots][3] claim tick, completion tick, weighted pattern count; or null
     TaskSlot *tasks;                  // [kTaskSlots]
     uint32_t *task_arrays;            // [kTaskSlots][3][cap]: active list, signatures, sorted LLRs
-    unsigned *task_counters;          // [0] open tasks, [1] shots in flight
+    unsigned *task_counters;          // [0] open tasks, [1] waves finished with their own shots, [2] task ids
     unsigned long long split_threshold;  // classes larger than this are shared (0 = never)
     unsigned *dbg;                    // optional chunk trace (debug builds of a run), [0] = records
     unsigned dbg_cap;
