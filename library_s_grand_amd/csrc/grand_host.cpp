@@ -362,7 +362,7 @@ struct Workspace {
         const size_t S = (size_t)std::max(shots_, 1);
         if (tasks.bytes == 0) tasks_fresh = true;
         if (!s_packed.ensure(S * c.D * 4) || !e_packed.ensure(S * c.M32 * 4) || !lg_packed.ensure(S * c.NL32 * 4) ||
-            !s_sig.ensure(S * 4) || !worklist.ensure(S * 4) || !counters.ensure(64) || !dec.ensure(S * c.M32 * 4) ||
+            !s_sig.ensure(S * 4) || !worklist.ensure(S * 4) || !counters.ensure(kCounterBytes) || !dec.ensure(S * c.M32 * 4) ||
             !declog.ensure(S * c.NL32 * 4) || !tasks.ensure(sizeof(TaskSlot) * kTaskSlots) ||
             !task_arrays.ensure((size_t)kTaskSlots * 3 * std::max(cap, 1) * 4))
             return false;
@@ -371,7 +371,7 @@ struct Workspace {
     }
     size_t bytes_for(const DevCode &c, int shots_, int cap) const {
         const size_t S = (size_t)std::max(shots_, 1);
-        return S * 4 * ((size_t)c.D + 2 * c.M32 + 2 * c.NL32 + 2) + 64 + sizeof(TaskSlot) * kTaskSlots +
+        return S * 4 * ((size_t)c.D + 2 * c.M32 + 2 * c.NL32 + 2) + kCounterBytes + sizeof(TaskSlot) * kTaskSlots +
                (size_t)kTaskSlots * 3 * std::max(cap, 1) * 4;
     }
 };

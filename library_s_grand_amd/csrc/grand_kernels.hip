@@ -1350,22 +1350,27 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
         L.ncr[i] = v > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)v;
     }
     __syncthreads();
-    // Each iteration issues the claim and the open-task poll together (one round
-    // trip).  The claim word holds (waves started << 32) | claim cursor: a wave
-    // counts itself started with one RMW on entry and claims with another, so
-    // every wave holding claimed shots is counted in the word before the cursor
-    // passes the end (RMWs on one location are totally ordered).  A wave counts
-    // itself finished once it finds the worklist drained.  Only a wave still
-    // decoding can publish a shared class, so helpers leave when finished ==
-    // started and no class is open; waves not yet resident never hold work.
-    unsigned long long *claim_word = reinterpret_cast<unsigned long long *>(call.counters + 8);
-    if (lane_id() == 0) atomicAdd(claim_word, 1ull << 32);
+    // Claims.  The worklist positions are dealt over kClaimWords claim words
+    // (word x hands out positions x, x + 8, x + 16, ...), each in its own cache
+    // line; a wave starts at the word of its XCD slot (workgroups are dispatched
+    // to the 8 XCDs round-robin) and moves to the next word when one runs dry, so
+    // the hot atomics are spread 8 ways.  A claim word is (registrations << 32) |
+    // cursor: every claim RMW adds (1 << 32) + B, so a wave that got shots is
+    // registered in that word before its cursor passed the end (RMWs on one
+    // location are totally ordered), and each wave adds its number of claim RMWs
+    // to the finished count once it has found every word dry.  Only a wave still
+    // decoding can publish a shared class, so helpers leave when finished equals
+    // the registrations of all words and no class is open; waves that are not
+    // resident yet hold no work.  The open-class poll rides along with the claim.
+    unsigned long long *cw = reinterpret_cast<unsigned long long *>(call.counters) + kClaimBase;
+    int cur = (int)(blockIdx.x & (kClaimWords - 1)), dry = 0;
+    unsigned claims = 0;
     bool drained = false;
     for (;;) {
         unsigned got = 0xFFFFFFFFu, open = 0u;
         if (lane_id() == 0) {
             if (call.split_threshold) open = ld_relaxed(&call.task_counters[0]);
-            if (!drained) got = (unsigned)atomicAdd(claim_word, (unsigned long long)B);
+            if (!drained) got = (unsigned)atomicAdd(cw + (size_t)cur * kClaimStride, (1ull << 32) + B);
         }
         open = uni(open);
         got = uni(got);
@@ -1373,22 +1378,32 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
         bool helped = false;
         if (open != 0u)
             while (help_task(code, call, L, ctask, hmask)) helped = true;
-        if (!drained && got < total) {
-            const unsigned last = (got + B < total) ? got + B : total;
-            for (unsigned i = got; i < last; ++i) decode_shot_wave(code, call, call.worklist[i], L);
-            ctask = -1;
-            continue;
-        }
         if (!drained) {
+            ++claims;
+            const unsigned p0 = (unsigned)cur + kClaimWords * got;
+            if (got < total && p0 < total) {
+                for (unsigned j = 0; j < B; ++j) {
+                    const unsigned p = p0 + kClaimWords * j;
+                    if (p >= total) break;
+                    decode_shot_wave(code, call, call.worklist[p], L);
+                }
+                ctask = -1;
+                continue;
+            }
+            if (++dry < kClaimWords) { cur = (cur + 1) & (kClaimWords - 1); continue; }
             drained = true;
-            if (lane_id() == 0) atomicAdd(&call.task_counters[1], 1u);
+            if (lane_id() == 0) atomicAdd(&call.task_counters[1], claims);
         }
         if (!call.split_threshold) break;
         if (helped) continue;
-        const unsigned fin = uni(ld_relaxed(&call.task_counters[1]));     // finished first, then started
-        const unsigned started = (unsigned)(uni64(ld_relaxed(claim_word)) >> 32);
+        const unsigned fin = uni(ld_relaxed(&call.task_counters[1]));     // finished first, then registrations
+        unsigned reg = (lane_id() < kClaimWords)
+                           ? (unsigned)(ld_relaxed(cw + (size_t)lane_id() * kClaimStride) >> 32) : 0u;
+#pragma unroll
+        for (int o = 4; o >= 1; o >>= 1) reg += (unsigned)__shfl_xor((int)reg, o, 64);
+        reg = uni(reg);
         const unsigned op = uni(ld_relaxed(&call.task_counters[0]));
-        if (fin == started && op == 0u) break;
+        if (fin == reg && op == 0u) break;
         __builtin_amdgcn_s_sleep(2);
     }
 }
@@ -1623,6 +1638,8 @@ __global__ void prep_kernel(const DevCall c, DevCall *dst, unsigned *counters) {
     uint32_t *d = reinterpret_cast<uint32_t *>(dst);
     for (int i = threadIdx.x; i < (int)(sizeof(DevCall) / 4); i += blockDim.x) d[i] = src[i];
     if (threadIdx.x < 16) counters[threadIdx.x] = 0u;
+    unsigned long long *claim_words = reinterpret_cast<unsigned long long *>(counters) + kClaimBase;
+    if (threadIdx.x < kClaimWords) claim_words[threadIdx.x * kClaimStride] = 0ull;
 }
 
 __global__ void fill_u32_kernel(unsigned *p, unsigned v, int n) {

@@ -45,6 +45,13 @@ struct DevCode {
 // owner wave publishes it; any wave claims chunks of 64 * R prefixes with one
 // fetch-and-add on ctl = (generation << 52) | (nchunks << 32) | next_chunk.
 constexpr int kTaskSlots = 128;
+
+// Claim words of the persistent search kernel: kClaimWords 64-bit words, one
+// per 128-byte line, starting at 32-bit counter index 2 * kClaimBase.
+constexpr int kClaimWords = 8;
+constexpr int kClaimBase = 16;        // in 64-bit words: byte offset 128
+constexpr int kClaimStride = 16;      // in 64-bit words: 128 bytes
+constexpr int kCounterBytes = 8 * (kClaimBase + kClaimWords * kClaimStride);
 struct TaskSlot {
     unsigned long long ctl;           // generation | nchunks | next chunk to claim
     unsigned long long best;          // first-hit: lowest matching rank; ML: (cost key << 32) | rank
@@ -81,7 +88,7 @@ struct DevCall {
     uint32_t *lg_packed;              // [shots][NL32] (cln)
     uint32_t *s_sig;                  // [shots]
     int *worklist;                    // nonzero-syndrome shots
-    unsigned *counters;               // [0] worklist length, [2] error bits, [8..9] claim word (u64); 16 words
+    unsigned *counters;               // [0] worklist length, [2] error bits, [4..6] task counters, claim words
     GrandDecoderResult *results;      // [shots]
     uint32_t *dec;                    // [shots][M32] decoded pattern
     uint32_t *declog;                 // [shots][NL32] decoded logicals (cln)
