@@ -473,7 +473,13 @@ __device__ __forceinline__ unsigned long long make_ctl(unsigned gen, unsigned nc
 template <class T>
 __device__ __forceinline__ T ld_relaxed(T *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 template <class T>
-__device__ __forceinline__ T ld_acquire(T *p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void st_relaxed(T *p, T v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+// Shared-task data is written with device-coherent (write-through) stores and
+// read with device-coherent loads, so no cache-wide acquire/release is needed:
+// a writer only waits for its own stores and atomics to complete before the
+// store or atomic that publishes them.  (An agent-scope acquire or release
+// fence would invalidate or write back the whole L2 of the XCD each time.)
+__device__ __forceinline__ void wait_mem() { __builtin_amdgcn_s_waitcnt(0); }
 
 __device__ __forceinline__ float key_to_float(uint32_t key) {
     const uint32_t b = (key & 0x80000000u) ? (key & 0x7FFFFFFFu) : ~key;
@@ -487,7 +493,7 @@ __device__ __forceinline__ unsigned claim_chunk(TaskSlot *T, unsigned *open_coun
     const unsigned long long snap = uni64(ld_relaxed(&T->ctl));
     if (ctl_next(snap) >= ctl_nch(snap)) return ~0u;
     unsigned long long old = 0;
-    if (lane_id() == 0) old = __hip_atomic_fetch_add(&T->ctl, 1ull, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane_id() == 0) old = __hip_atomic_fetch_add(&T->ctl, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     old = uni64(old);
     if (ctl_next(old) >= ctl_nch(old)) return ~0u;
     const unsigned ch = ctl_next(old);
@@ -583,23 +589,24 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
     TaskSlot *T = &call.tasks[slot];
     uint32_t *arr = call.task_arrays + (size_t)slot * 3 * call.cap;
     for (int i = lane; i < tam; i += 64) {
-        arr[i] = (uint32_t)act[i];
-        arr[call.cap + i] = sg[i];
-        if (ml) arr[2 * call.cap + i] = __float_as_uint(llr[i]);
+        st_relaxed(&arr[i], (uint32_t)act[i]);
+        st_relaxed(&arr[call.cap + i], (uint32_t)sg[i]);
+        if (ml) st_relaxed(&arr[2 * call.cap + i], __float_as_uint(llr[i]));
     }
     const unsigned gen = (ctl_gen(ld_relaxed(&T->ctl)) + 1u) & 0xFFFu;
     unsigned uid = 0;
     if (lane == 0) uid = atomicAdd(&call.task_counters[2], 1u);
     uid = uni(uid);
     if (lane == 0) {
-        T->gen = uid; T->nchunks = nch; T->shot = shot; T->w = w; T->tam = tam; T->R = R; T->ml = ml ? 1 : 0;
-        T->t0 = t0; T->nw = Nw; T->np = NP; T->done = 0u; T->best = ~0ull; T->best_q = ~0ull;
-#pragma unroll
-        for (int d = 0; d < kMaxWords; ++d) T->s[d] = s[d];
+        st_relaxed(&T->gen, uid); st_relaxed(&T->nchunks, nch); st_relaxed(&T->shot, shot); st_relaxed(&T->w, w);
+        st_relaxed(&T->tam, tam); st_relaxed(&T->R, R); st_relaxed(&T->ml, ml ? 1 : 0); st_relaxed(&T->t0, t0);
+        st_relaxed(&T->nw, (unsigned long long)Nw); st_relaxed(&T->np, (unsigned long long)NP);
+        st_relaxed(&T->done, 0u); st_relaxed(&T->best, ~0ull); st_relaxed(&T->best_q, ~0ull);
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane < kMaxWords) st_relaxed(&T->s[lane], (uint32_t)s[lane]);
+    wait_mem();                                   // the task is complete in memory before it is published
     if (lane == 0) {
-        __hip_atomic_exchange(&T->ctl, make_ctl(gen, nch), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_exchange(&T->ctl, make_ctl(gen, nch), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         atomicAdd(&call.task_counters[0], 1u);
     }
     for (;;) {
@@ -608,12 +615,13 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
         if (ch == ~0u) break;
         const unsigned long long res = task_chunk(code, T, s, t0, w, tam, R, NP, ml, ch, act, sg, llr, ht, hmask);
         trace_chunk(call, 0, slot, uid, ch, w, tam, R, NP, t0, s, act, sg, res);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        if (lane == 0) __hip_atomic_fetch_add(&T->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        wait_mem();                               // the chunk's result atomics complete before it counts as done
+        if (lane == 0) atomicAdd(&T->done, 1u);
     }
-    while (uni(ld_acquire(&T->done)) < nch) __builtin_amdgcn_s_sleep(1);
-    best = uni64(ld_acquire(&T->best));
-    if (lane == 0) __hip_atomic_store(&T->busy, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    while (uni(ld_relaxed(&T->done)) < nch) __builtin_amdgcn_s_sleep(1);
+    best = uni64(ld_relaxed(&T->best));
+    wait_mem();
+    if (lane == 0) st_relaxed(&T->busy, 0u);
     return true;
 }
 
@@ -636,7 +644,8 @@ __device__ bool help_task(const DevCode &code, const DevCall &call, const WaveLd
     unsigned g;
     const unsigned ch = claim_chunk(T, &call.task_counters[0], g);
     if (ch == ~0u) return true;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // the task was complete in memory before its ctl was published; device-coherent
+    // loads read it after the claim
     const int w = ld_relaxed(&T->w), tam = ld_relaxed(&T->tam), R = ld_relaxed(&T->R);
     const unsigned uid = uni(ld_relaxed(&T->gen));
     const bool ml = ld_relaxed(&T->ml) != 0;
@@ -648,11 +657,11 @@ __device__ bool help_task(const DevCode &code, const DevCall &call, const WaveLd
     int LDS *act = ml ? L.P : L.A;
     uint32_t LDS *sg = ml ? L.sigP : L.sigA;
     if (ctask != (int)uid) {
-        const uint32_t *arr = call.task_arrays + (size_t)slot * 3 * call.cap;
+        uint32_t *arr = call.task_arrays + (size_t)slot * 3 * call.cap;
         for (int i = lane; i < tam; i += 64) {
-            act[i] = (int)arr[i];
-            sg[i] = arr[call.cap + i];
-            if (ml) L.llrP[i] = __uint_as_float(arr[2 * call.cap + i]);
+            act[i] = (int)ld_relaxed(&arr[i]);
+            sg[i] = ld_relaxed(&arr[call.cap + i]);
+            if (ml) L.llrP[i] = __uint_as_float(ld_relaxed(&arr[2 * call.cap + i]));
         }
         __syncthreads();
         hmask = build_ht(L.ht, sg, tam);
@@ -661,8 +670,8 @@ __device__ bool help_task(const DevCode &code, const DevCall &call, const WaveLd
     const unsigned long long res =
         task_chunk(code, T, s, t0, w, tam, R, NP, ml, ch, act, sg, ml ? L.llrP : nullptr, L.ht, hmask);
     trace_chunk(call, 1, slot, uid, ch, w, tam, R, NP, t0, s, act, sg, res);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (lane == 0) __hip_atomic_fetch_add(&T->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    wait_mem();                                   // the chunk's result atomics complete before it counts as done
+    if (lane == 0) atomicAdd(&T->done, 1u);
     return true;
 }
 
