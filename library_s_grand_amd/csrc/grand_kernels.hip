@@ -178,6 +178,15 @@ __device__ __forceinline__ uint32_t LDS *lds_base() { return (uint32_t LDS *)lds
 
 constexpr int kNcrLdsRows = 129;   // n = 0..128
 
+// Signature bitmap kept after the hash table: at least 32 bits per entry, so
+// a prefix whose target signature is absent is rejected by one LDS read in all
+// but a fraction <= 1/32 of cases, without walking a probe chain.
+__host__ __device__ __forceinline__ int bitmap_words(int cap) {
+    int bits = 1024;
+    while (bits < 32 * cap) bits <<= 1;
+    return bits / 32;
+}
+
 __host__ __device__ __forceinline__ int ht_capacity(int cap) {
     int h = 64;
     while (h < 2 * cap) h <<= 1;
@@ -192,7 +201,7 @@ __device__ __forceinline__ WaveLds carve_lds(int cap, bool ml) {
     L.sw2 = L.sw + kMaxWords;
     L.ncr = L.sw2 + kMaxWords;
     uint32_t LDS *p = L.ncr + kNcrLdsRows * kNcrStride;
-    L.ht = (unsigned long long LDS *)p; p += 2 * ht_capacity(cap);
+    L.ht = (unsigned long long LDS *)p; p += 2 * ht_capacity(cap) + bitmap_words(cap);
     L.A = (int LDS *)p; p += cap;
     L.sigA = p; p += cap;
     if (ml) {
@@ -283,14 +292,22 @@ __device__ void sort_by_key(const WaveLds &L, int tam) {
 // signature & mask, linear probing, at most half full.  Equal signatures
 // (equal columns) occupy successive slots of one probe chain.
 constexpr unsigned long long kHtEmpty = 0xFFFFFFFFull;
+// Returns the table mask in the low 16 bits and the bitmap's bit mask in the
+// high 16 bits; the bitmap follows the table's hsz entries.
 __device__ unsigned build_ht(unsigned long long LDS *ht, const uint32_t LDS *sg, int tam) {
     const int lane = lane_id();
     const int hsz = ht_capacity(tam);
     const unsigned mask = (unsigned)hsz - 1u;
+    uint32_t LDS *bm = (uint32_t LDS *)(ht + hsz);
+    const int bw = bitmap_words(tam);
+    const unsigned bmask = (unsigned)bw * 32u - 1u;
     for (int i = lane; i < hsz; i += 64) ht[i] = kHtEmpty;
+    for (int i = lane; i < bw; i += 64) bm[i] = 0u;
     __syncthreads();
     for (int i = lane; i < tam; i += 64) {
         const uint32_t key = sg[i];
+        __hip_atomic_fetch_or(&bm[(key & bmask) >> 5], 1u << (key & 31u), __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_WORKGROUP);
         const unsigned long long ent = ((unsigned long long)key << 32) | (unsigned)i;
         unsigned h = key & mask;
         for (;;) {
@@ -302,7 +319,7 @@ __device__ unsigned build_ht(unsigned long long LDS *ht, const uint32_t LDS *sg,
         }
     }
     __syncthreads();
-    return mask;
+    return mask | (bmask << 16);
 }
 
 // ------------------------------------------------------------ prefix scanners
@@ -346,11 +363,14 @@ __device__ __forceinline__ PrefHit scan_prefix(const DevCode &code, const uint32
     uint32_t a = t0;
 #pragma unroll
     for (int l = 0; l < WP; ++l) { a ^= sg[c[l]]; acc[l] = a; }
+    const unsigned hm = hmask & 0xFFFFu, bmsk = hmask >> 16;
+    const uint32_t LDS *bm = (const uint32_t LDS *)(ht + hm + 1u);
     for (int j = 0; j < len; ++j) {
         const uint32_t t = acc[WP - 1];
         const int last = c[WP - 1];
-        unsigned pos = t & hmask;
-        for (;;) {
+        const unsigned bi = t & bmsk;
+        unsigned pos = ((bm[bi >> 5] >> (bi & 31u)) & 1u) ? (t & hm) : 0xFFFFFFFFu;
+        for (; pos != 0xFFFFFFFFu;) {
             const unsigned long long e = ht[pos];
             if (e == kHtEmpty) break;
             const int x = (int)(uint32_t)e;
@@ -371,7 +391,7 @@ __device__ __forceinline__ PrefHit scan_prefix(const DevCode &code, const uint32
                     }
                 }
             }
-            pos = (pos + 1u) & hmask;
+            pos = (pos + 1u) & hm;
         }
         if (MODE == 0 && h.j >= 0) break;
         if (j + 1 == len) break;
@@ -1668,6 +1688,7 @@ using grand::DevCode;
 size_t grand_fused_lds_bytes(int cap, int ml) {
     return sizeof(uint32_t) * ((size_t)grand::kMaxM / 32 + 2 * grand::kMaxWords +
                                (size_t)grand::kNcrLdsRows * grand::kNcrStride + 2 * (size_t)grand::ht_capacity(cap) +
+                               (size_t)grand::bitmap_words(cap) +
                                (size_t)cap * (ml ? 6 : 2));
 }
 
