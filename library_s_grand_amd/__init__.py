@@ -302,3 +302,78 @@ def save_results_cln_csv(output_file, results, num_shots, using_errors=True):
             w.writerow([i, r.found, r.found_weight, r.original_weight, r.degenerate, r.is_equal, r.tam,
                         r.total_tested_combinations, r.hibrido, r.iteration, r.time_sum_ms, r.total_time_search_ms,
                         r.transfer_time_ms, r.total_to_find_ms])
+
+
+# ------------------------------------------------------ file and simulation helpers
+# Same names and behaviour as auxilary_functions.py:37-73, :109-131, :168-170.
+def save_syndromes_to_file(syndromes, output_file):
+    """One row per line, values separated by single spaces (auxilary_functions.py:37-41)."""
+    import csv
+    with open(output_file, "w", newline="", encoding="utf-8") as f:
+        w = csv.writer(f, delimiter=" ")
+        for row in syndromes:
+            w.writerow([int(v) for v in row])
+
+
+def read_errors_or_syndromes_from_file(file_path, n_cols, max_shots=None):
+    """Rows of n_cols ints or one n_cols-character bitstring (auxilary_functions.py:44-73)."""
+    from .codes import read_rows
+    return read_rows(file_path, n_cols, max_shots)
+
+
+def read_matrix_from_file(file_path):
+    """Whitespace-separated rows; int8 like auxilary_functions.py:109-121."""
+    from .codes import read_matrix
+    return read_matrix(file_path).astype(np.int8)
+
+
+def compute_syndrome(H, error):
+    """(H @ e) mod 2 (auxilary_functions.py:124-126)."""
+    return (np.asarray(H) @ np.asarray(error)) % 2
+
+
+def simulate_random_error(n, p_error):
+    """Nonzero i.i.d. Bernoulli(p_error) vector, redrawn until nonzero (auxilary_functions.py:128-132)."""
+    while True:
+        e = (np.random.rand(n) < p_error).astype(int)
+        if np.any(e):
+            return e
+
+
+def error_positions_random(n, error_weight):
+    """error_weight distinct positions in [0, n) (auxilary_functions.py:168-170)."""
+    return np.random.choice(n, size=error_weight, replace=False)
+
+
+def build_quantum_circuit_H(H, error_positions, measure):
+    """The reference builds a Qiskit syndrome-extraction circuit (auxilary_functions.py:134-155):
+    X on the error qubits, a CNOT from data qubit c to ancilla r for every H[r][c] = 1, and
+    optional measurement of the ancillas.  Needs qiskit, which this package does not require."""
+    try:
+        from qiskit import QuantumCircuit
+    except ImportError as exc:
+        raise ImportError("build_quantum_circuit_H needs qiskit, which is not installed") from exc
+    m, n = np.asarray(H).shape
+    qc = QuantumCircuit(n + m, m)
+    if error_positions is not None:
+        for q in error_positions:
+            qc.x(int(q))
+    for row in range(m):
+        for col in range(n):
+            if H[row, col] == 1:
+                qc.cx(col, n + row)
+    if measure:
+        for row in range(m):
+            qc.measure(n + row, row)
+    return qc
+
+
+def extract_syndrome(qc, n_shots):
+    """Runs the circuit on Qiskit Aer and returns the first measured syndrome
+    (auxilary_functions.py:157-166); needs qiskit-aer."""
+    try:
+        from qiskit_aer import AerSimulator
+    except ImportError as exc:
+        raise ImportError("extract_syndrome needs qiskit-aer, which is not installed") from exc
+    counts = AerSimulator().run(qc, shots=n_shots).result().get_counts()
+    return [int(b) for b in next(iter(counts))[::-1]]

@@ -185,3 +185,32 @@ def test_csv_writers_column_order(tmp_path):
     lines = f.read_text().splitlines()
     assert lines[0].startswith("shot,found,found_weight,size_active_positions,total_tested_combinations")
     assert lines[2].startswith("1,1,0,5,12")
+
+
+def test_cln_csv_one_row_per_shot(tmp_path):
+    res = (G.GrandDecoderResult * 3)()
+    res[2].found, res[2].degenerate = 1, 1
+    f = tmp_path / "cln.csv"
+    G.save_results_cln_csv(str(f), res, 3, True)
+    lines = f.read_text().splitlines()
+    assert len(lines) == 4 and lines[3].startswith("2,1,0,0,1,0")
+
+
+def test_reference_file_and_simulation_helpers(tmp_path):
+    hz = G.read_matrix_from_file(os.path.join(T.GOLDEN, "hz_ibm_72.txt"))
+    assert hz.dtype == np.int8 and hz.shape == (36, 72)
+    pats = G.read_errors_or_syndromes_from_file(os.path.join(T.GOLDEN, "patterns_72_first30.txt"), 72, max_shots=7)
+    assert pats.shape == (7, 72)
+    syn = np.array([G.compute_syndrome(hz.astype(np.int64), e) for e in pats])
+    ref = G.read_errors_or_syndromes_from_file(os.path.join(T.GOLDEN, "syndromes_72.txt"), 36)[:7]
+    assert (syn == ref).all()
+    out = tmp_path / "syn.txt"
+    G.save_syndromes_to_file(syn, str(out))
+    assert (G.read_errors_or_syndromes_from_file(str(out), 36) == syn).all()
+    np.random.seed(3)
+    e = G.simulate_random_error(72, 0.01)
+    assert e.any() and e.shape == (72,)
+    pos = G.error_positions_random(72, 5)
+    assert len(set(pos.tolist())) == 5 and pos.max() < 72
+    with pytest.raises(ImportError):
+        G.build_quantum_circuit_H(hz, [1], True)
