@@ -90,8 +90,8 @@ __device__ __forceinline__ uint64_t ncr(const unsigned long long *t, int n, int 
 // Lexicographic unranking (rank 0 = {0..w-1}); the reference scans linearly
 // (grand_decoder.cu:83-100), here each position is a binary search on the
 // hockey-stick identity  sum_{y=p+1}^{x-1} C(t-1-y, k-1) = C(t-1-p, k) - C(t-x, k).
-template <int W>
-__device__ __forceinline__ void unrank_t(uint64_t r, int tam, const unsigned long long *nCr, int *c) {
+template <int W, class P>
+__device__ __forceinline__ void unrank_t(uint64_t r, int tam, P nCr, int *c) {
     int prev = -1;
 #pragma unroll
     for (int i = 0; i < W; ++i) {
@@ -110,8 +110,8 @@ __device__ __forceinline__ void unrank_t(uint64_t r, int tam, const unsigned lon
 }
 
 // Same with the weight known only at run time (per lane), at most MAXW.
-template <int MAXW = kMaxW>
-__device__ __forceinline__ void unrank_dyn(uint64_t r, int tam, int w, const unsigned long long *nCr, int *c) {
+template <int MAXW = kMaxW, class P = const unsigned long long *>
+__device__ __forceinline__ void unrank_dyn(uint64_t r, int tam, int w, P nCr, int *c) {
     int prev = -1;
 #pragma unroll
     for (int i = 0; i < MAXW; ++i) {
@@ -133,21 +133,21 @@ __device__ __forceinline__ void unrank_dyn(uint64_t r, int tam, int w, const uns
 
 // Unranking in a class small enough that every binomial it reads is below
 // 2^32 (tam <= 128), from the wave's LDS copy of the table.
-__device__ __forceinline__ void unrank_lds(uint32_t r, int tam, int w, const uint32_t LDS *t,
+__device__ __forceinline__ void unrank_lds(uint32_t r, int tam, int w, const unsigned long long LDS *t,
                                            int *c) {
     int prev = -1;
 #pragma unroll
     for (int i = 0; i < kMaxW; ++i) {
         if (i < w) {
             const int k = w - i;
-            const uint32_t tot = t[(tam - 1 - prev) * kNcrStride + k];
+            const uint32_t tot = (uint32_t)t[(tam - 1 - prev) * kNcrStride + k];
             const uint32_t T = tot - r;
             int lo = prev + 1, hi = tam - k;
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
-                if (t[(tam - mid) * kNcrStride + k] >= T) lo = mid; else hi = mid - 1;
+                if ((uint32_t)t[(tam - mid) * kNcrStride + k] >= T) lo = mid; else hi = mid - 1;
             }
-            r -= tot - t[(tam - lo) * kNcrStride + k];
+            r -= tot - (uint32_t)t[(tam - lo) * kNcrStride + k];
             c[i] = lo;
             prev = lo;
         }
@@ -170,7 +170,7 @@ struct WaveLds {
     uint32_t LDS *ew;  // packed error words of the shot (triage), kMaxM / 32
     uint32_t LDS *sw;  // syndrome words of the shot being searched, kMaxWords
     uint32_t LDS *sw2; // syndrome minus one column (learned pass), kMaxWords
-    uint32_t LDS *ncr; // C(n, k) for n <= 128, k < 8 (saturated to 32 bits): unranking in small classes
+    unsigned long long LDS *ncr; // C(n, k) for n < kNcrLdsRows, k < 8 (search kernel only)
 };
 
 extern __shared__ uint32_t lds_raw[];
@@ -199,8 +199,8 @@ __device__ __forceinline__ WaveLds carve_lds(int cap, bool ml) {
     L.ew = base;
     L.sw = base + kMaxM / 32;
     L.sw2 = L.sw + kMaxWords;
-    L.ncr = L.sw2 + kMaxWords;
-    uint32_t LDS *p = L.ncr + kNcrLdsRows * kNcrStride;
+    L.ncr = (unsigned long long LDS *)(L.sw2 + kMaxWords);
+    uint32_t LDS *p = L.sw2 + kMaxWords + 2 * kNcrLdsRows * kNcrStride;
     L.ht = (unsigned long long LDS *)p; p += 2 * ht_capacity(cap) + bitmap_words(cap);
     L.A = (int LDS *)p; p += cap;
     L.sigA = p; p += cap;
@@ -213,6 +213,18 @@ __device__ __forceinline__ WaveLds carve_lds(int cap, bool ml) {
         L.llrA = nullptr; L.P = nullptr; L.sigP = nullptr; L.llrP = nullptr;
     }
     return L;
+}
+
+// The search kernel keeps C(n, k), n < kNcrLdsRows, in every wave's LDS at the
+// same offset (after the shot's packed errors and syndrome words), filled when
+// the kernel starts; lookups with small n read it instead of the global table.
+__device__ __forceinline__ const unsigned long long LDS *ncr_lds() {
+    return (const unsigned long long LDS *)(lds_base() + kMaxM / 32 + 2 * kMaxWords);
+}
+
+__device__ __forceinline__ uint64_t ncr_w(const DevCode &code, int n, int k) {
+    if (n < 0 || k < 0 || k > n) return 0ull;
+    return (n < kNcrLdsRows) ? ncr_lds()[n * kNcrStride + k] : code.nCr[n * kNcrStride + k];
 }
 
 // Exact check of a candidate: s ^ XOR_l column(cols[l]) == 0 over all D words.
@@ -331,8 +343,8 @@ __device__ unsigned build_ht(unsigned long long LDS *ht, const uint32_t LDS *sg,
 // pattern-by-pattern walk.
 
 // Rank of the w-subset (c[0..WP-1], x) of [0, tam), w = WP + 1.
-template <int WP>
-__device__ __forceinline__ uint64_t rank_with_last(const int *c, int x, int tam, const unsigned long long *nCr) {
+template <int WP, class P>
+__device__ __forceinline__ uint64_t rank_with_last(const int *c, int x, int tam, P nCr) {
     uint64_t r = 0;
     int prev = -1;
 #pragma unroll
@@ -358,7 +370,8 @@ __device__ __forceinline__ PrefHit scan_prefix(const DevCode &code, const uint32
     if (len <= 0) return h;
     const int M = tam - 1;
     int c[WP];
-    unrank_t<WP>(q0, M, code.nCr, c);
+    const bool small = tam < kNcrLdsRows;
+    if (small) unrank_t<WP>(q0, M, ncr_lds(), c); else unrank_t<WP>(q0, M, code.nCr, c);
     uint32_t acc[WP];   // acc[l] = t0 ^ sg[c0] ^ ... ^ sg[c_l]
     uint32_t a = t0;
 #pragma unroll
@@ -379,7 +392,8 @@ __device__ __forceinline__ PrefHit scan_prefix(const DevCode &code, const uint32
 #pragma unroll
                 for (int l = 0; l <= WP; ++l) cols[l] = (l < WP) ? act[c[l]] : act[x];
                 if (exact_match<WP + 1>(code, s, cols, WP + 1)) {
-                    const uint64_t r = rank_with_last<WP>(c, x, tam, code.nCr);
+                    const uint64_t r = small ? rank_with_last<WP>(c, x, tam, ncr_lds())
+                                             : rank_with_last<WP>(c, x, tam, code.nCr);
                     if (MODE == 0) {
                         if (r < h.rank) { h.rank = r; h.j = j; }
                     } else {
@@ -704,7 +718,7 @@ __device__ uint64_t class_first_hit(const DevCode &code, const DevCall &call, in
                                     const uint32_t LDS *sg, const unsigned long long LDS *ht, unsigned hmask) {
     const int lane = lane_id();
     if (w == 1) return weight_one<0>(code, s, t0, tam, act, sg, nullptr).rank;
-    const uint64_t NP = ncr(code.nCr, tam - 1, w - 1);
+    const uint64_t NP = ncr_w(code, tam - 1, w - 1);
     unsigned long long shared_best;
     if (call.split_threshold && NP > call.split_threshold &&
         run_shared(code, call, shot, s, t0, w, tam, Nw, NP, false, act, sg, nullptr, ht, hmask, shared_best))
@@ -752,7 +766,7 @@ __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, int sho
     uint64_t total = 0;
 #pragma unroll
     for (int k = 0; k < kMaxW; ++k) {
-        N[k] = (k < nw) ? ncr(code.nCr, tam, wl[k]) : 0ull;
+        N[k] = (k < nw) ? ncr_w(code, tam, wl[k]) : 0ull;
         total += N[k];
     }
     uint64_t class_start = 0;
@@ -808,7 +822,8 @@ __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, int sho
                 hit_lane = 0;
                 hit_w = w;
                 hit_idx = class_start + r;
-                unrank_dyn(r, tam, w, code.nCr, c);   // same values in every lane
+                if (tam < kNcrLdsRows) unrank_dyn(r, tam, w, ncr_lds(), c);   // same values in every lane
+                else unrank_dyn(r, tam, w, code.nCr, c);
             } else {
                 class_start += N[kc];
                 ++kc;
@@ -850,7 +865,7 @@ __device__ bool ml_wave(const DevCode &code, const DevCall &call, int shot, cons
     float best = 0.0f;
     for (int w = 1; w <= max_w; ++w) {
         if (have && prefix[w] >= best) break;
-        const uint64_t Nw = ncr(code.nCr, tam, w);
+        const uint64_t Nw = ncr_w(code, tam, w);
         count += Nw;
         wops += Nw * (uint64_t)(w + 1);
         bool chit = false;
@@ -860,7 +875,7 @@ __device__ bool ml_wave(const DevCode &code, const DevCall &call, int shot, cons
             const PrefHit h = weight_one<1>(code, s, t0, tam, L.P, L.sigP, L.llrP);
             if (h.j >= 0) { chit = true; cbest = h.cost; crank = h.rank; }
         } else {
-            const uint64_t NP = ncr(code.nCr, tam - 1, w - 1);
+            const uint64_t NP = ncr_w(code, tam - 1, w - 1);
             unsigned long long shared_best;
             if (call.split_threshold && NP > call.split_threshold && Nw < (1ull << 32) &&
                 run_shared(code, call, shot, s, t0, w, tam, Nw, NP, true, L.P, L.sigP, L.llrP, L.ht, hmask,
@@ -897,7 +912,8 @@ __device__ bool ml_wave(const DevCode &code, const DevCall &call, int shot, cons
         }
         if (chit && (!have || cbest < best)) {
             int c[kMaxW] = {0, 0, 0, 0, 0, 0};
-            unrank_dyn(crank, tam, w, code.nCr, c);
+            if (tam < kNcrLdsRows) unrank_dyn(crank, tam, w, ncr_lds(), c);
+            else unrank_dyn(crank, tam, w, code.nCr, c);
             have = true;
             best = cbest;
             fw = w;
@@ -1082,7 +1098,7 @@ __device__ uint64_t learned_class(const DevCode &code, const DevCall &call, cons
         }
         return ~0ull;
     }
-    const uint64_t NP = ncr(code.nCr, k - 1, wp - 1);
+    const uint64_t NP = ncr_w(code, k - 1, wp - 1);
     uint64_t pos = 0;
     while (pos < NP) {
         const uint64_t rem = NP - pos;
@@ -1127,7 +1143,7 @@ __device__ bool learned_pass(const DevCode &code, const DevCall &call, int shot,
         __syncthreads();
         const uint32_t t2 = t0 ^ L.sigP[k];
         for (int wp = 0; wp < maxw; ++wp) {
-            const uint64_t N = (wp == 0) ? 1ull : ncr(code.nCr, k, wp);
+            const uint64_t N = (wp == 0) ? 1ull : ncr_w(code, k, wp);
             if (N == 0) break;                      // larger weights do not fit in P[0..k) either
             if (G >= budget) { stop = true; break; }
             const uint64_t lim = (budget - G) < N ? (budget - G) : N;
@@ -1142,7 +1158,10 @@ __device__ bool learned_pass(const DevCode &code, const DevCall &call, int shot,
             }
             if (r != ~0ull) {
                 int c[kMaxWin] = {0, 0, 0, 0, 0, 0, 0, 0};
-                if (wp > 0) unrank_dyn<kMaxWin>(r, k, wp, code.nCr, c);
+                if (wp > 0) {
+                    if (k < kNcrLdsRows) unrank_dyn<kMaxWin>(r, k, wp, ncr_lds(), c);
+                    else unrank_dyn<kMaxWin>(r, k, wp, code.nCr, c);
+                }
                 o.fw = wp + 1;
 #pragma unroll
                 for (int l = 0; l < kMaxWin; ++l) o.win[l] = (l < wp) ? L.P[c[l]] : ((l == wp) ? ck : -1);
@@ -1374,10 +1393,7 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
     B = B < 1u ? 1u : (B > (unsigned)call.claim_batch ? (unsigned)call.claim_batch : B);
     int ctask = -1;        // task whose arrays and hash table this wave's LDS holds
     unsigned hmask = 0;
-    for (int i = lane_id(); i < kNcrLdsRows * kNcrStride; i += 64) {
-        const unsigned long long v = code.nCr[i];
-        L.ncr[i] = v > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)v;
-    }
+    for (int i = lane_id(); i < kNcrLdsRows * kNcrStride; i += 64) L.ncr[i] = code.nCr[i];
     __syncthreads();
     // Claims.  The worklist positions are dealt over kClaimWords claim words
     // (word x hands out positions x, x + 8, x + 16, ...), each in its own cache
@@ -1687,7 +1703,7 @@ using grand::DevCode;
 
 size_t grand_fused_lds_bytes(int cap, int ml) {
     return sizeof(uint32_t) * ((size_t)grand::kMaxM / 32 + 2 * grand::kMaxWords +
-                               (size_t)grand::kNcrLdsRows * grand::kNcrStride + 2 * (size_t)grand::ht_capacity(cap) +
+                               2 * (size_t)grand::kNcrLdsRows * grand::kNcrStride + 2 * (size_t)grand::ht_capacity(cap) +
                                (size_t)grand::bitmap_words(cap) +
                                (size_t)cap * (ml ? 6 : 2));
 }
