@@ -701,6 +701,11 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
         if (s->timing) HIPX(hipEventRecord(s->ev[2], st));
         return rc;
     }
+    // small codes are staged in each search wave's LDS (column words, offsets, signatures)
+    {
+        const long long words = (long long)(code.D + 2) * code.m;
+        c.code_lds_words = (words <= kCodeLdsMaxWords) ? (int)words : 0;
+    }
     const int wpc = (s->ex.waves_per_cu > 0) ? s->ex.waves_per_cu : g_tuning.waves_per_cu;
     const int grid = std::max(1, std::min(R.cu_count * wpc, num_shots));
     HIPX(grand_launch_fused(dcode, dcall, &c, grid, st));

@@ -167,6 +167,9 @@ struct WaveLds {
     uint32_t LDS *sigP;
     float LDS *llrP;
     unsigned long long LDS *ht;   // (signature << 32) | position, empty = kHtEmpty
+    const uint32_t LDS *chw;   // code staged in LDS by the search kernel (small codes), or null:
+    const int LDS *coff;       //   column words [D][m], score offsets [m], signatures [m]
+    const uint32_t LDS *csig;
     uint32_t LDS *ew;  // packed error words of the shot (triage), kMaxM / 32
     uint32_t LDS *sw;  // syndrome words of the shot being searched, kMaxWords
     uint32_t LDS *sw2; // syndrome minus one column (learned pass), kMaxWords
@@ -193,7 +196,7 @@ __host__ __device__ __forceinline__ int ht_capacity(int cap) {
     return h;
 }
 
-__device__ __forceinline__ WaveLds carve_lds(int cap, bool ml) {
+__device__ __forceinline__ WaveLds carve_lds(int cap, bool ml, int code_words = 0, int m = 0, int D = 0) {
     uint32_t LDS *base = lds_base();
     WaveLds L;
     L.ew = base;
@@ -211,6 +214,13 @@ __device__ __forceinline__ WaveLds carve_lds(int cap, bool ml) {
         L.llrP = (float LDS *)p; p += cap;
     } else {
         L.llrA = nullptr; L.P = nullptr; L.sigP = nullptr; L.llrP = nullptr;
+    }
+    if (code_words > 0) {
+        L.chw = p; p += (size_t)D * m;
+        L.coff = (const int LDS *)p; p += m;
+        L.csig = p;
+    } else {
+        L.chw = nullptr; L.coff = nullptr; L.csig = nullptr;
     }
     return L;
 }
@@ -245,19 +255,20 @@ __device__ __forceinline__ bool exact_match(const DevCode &code, const uint32_t 
 // Active set for threshold thr: ascending columns with score >= thr, where
 // score[c] = |supp(h_c) & supp(s)| + off[c]  (grand_decoder.cu:352-416, :495-504).
 // Writes the first min(tam, cap) entries; returns the full tam.
-__device__ int build_active(const DevCode &code, const uint32_t LDS *s, int thr, const WaveLds &L, int cap) {
+template <class PW, class PO, class PS>
+__device__ __forceinline__ int build_active_t(PW Hcw, PO off, PS sig, int D, int m, const uint32_t LDS *s, int thr,
+                                              const WaveLds &L, int cap) {
     const int lane = lane_id();
     int tam = 0;
-    const int D = code.D, m = code.m;
     for (int c0 = 0; c0 < m; c0 += 64) {
         const int c = c0 + lane;
         const int cc = c < m ? c : m - 1;            // in-bounds column for the loads
-        int sc = code.off[cc];
+        int sc = off[cc];
         // column words four at a time (indices clamped, so the loads issue together)
         for (int d0 = 0; d0 < D; d0 += 4) {
             uint32_t h[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) h[j] = code.Hcw[(size_t)((d0 + j < D) ? d0 + j : D - 1) * m + cc];
+            for (int j = 0; j < 4; ++j) h[j] = Hcw[(size_t)((d0 + j < D) ? d0 + j : D - 1) * m + cc];
 #pragma unroll
             for (int j = 0; j < 4; ++j)
                 if (d0 + j < D) sc += __popc(h[j] & s[d0 + j]);
@@ -266,12 +277,17 @@ __device__ int build_active(const DevCode &code, const uint32_t LDS *s, int thr,
         const uint64_t b = ballot(on);
         if (on) {
             const int p = tam + lanes_below(b);
-            if (p < cap) { L.A[p] = c; L.sigA[p] = code.sig[c]; }
+            if (p < cap) { L.A[p] = c; L.sigA[p] = sig[c]; }
         }
         tam += __popcll(b);
     }
     __syncthreads();
     return tam;
+}
+
+__device__ int build_active(const DevCode &code, const uint32_t LDS *s, int thr, const WaveLds &L, int cap) {
+    if (L.chw) return build_active_t(L.chw, L.coff, L.csig, code.D, code.m, s, thr, L, cap);
+    return build_active_t(code.Hcw, code.off, code.sig, code.D, code.m, s, thr, L, cap);
 }
 
 // Stable sort of A[0..tam) by llr (ties keep active-list order), the order of
@@ -1255,11 +1271,25 @@ __device__ __forceinline__ void write_zero_result(GrandDecoderResult *dst, int o
 __device__ __forceinline__ void pack_tile(const int *rows, int stride, int base, int cnt, int c0, int cols,
                                           uint32_t LDS *tile, uint32_t &w0, uint32_t &w1, unsigned &bad) {
     const int lane = lane_id();
+    const bool vec = c0 + 64 <= cols && (stride & 3) == 0 && (((uintptr_t)rows) & 15) == 0;
+    if (vec) {
+        // 16 lanes per row, 16 bytes per lane: 16 independent loads cover the tile
+        const int rr = lane >> 4, cc = (lane & 15) * 4;
+#pragma unroll
+        for (int it = 0; it < 16; ++it) {
+            const int j = it * 4 + rr;
+            int4 v = make_int4(0, 0, 0, 0);
+            if (j < cnt) v = *reinterpret_cast<const int4 *>(rows + (size_t)(base + j) * stride + c0 + cc);
+            uint32_t LDS *d = tile + j * 65 + cc;
+            d[0] = (uint32_t)v.x; d[1] = (uint32_t)v.y; d[2] = (uint32_t)v.z; d[3] = (uint32_t)v.w;
+        }
+    } else {
 #pragma unroll 8
-    for (int j = 0; j < 64; ++j) {
-        int v = 0;
-        if (j < cnt && c0 + lane < cols) v = rows[(size_t)(base + j) * stride + c0 + lane];
-        tile[j * 65 + lane] = (uint32_t)v;
+        for (int j = 0; j < 64; ++j) {
+            int v = 0;
+            if (j < cnt && c0 + lane < cols) v = rows[(size_t)(base + j) * stride + c0 + lane];
+            tile[j * 65 + lane] = (uint32_t)v;
+        }
     }
     __syncthreads();
     w0 = 0u; w1 = 0u;
@@ -1385,7 +1415,19 @@ __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ 
 __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ codep, const DevCall *__restrict__ callp) {
     const DevCode &code = *codep;
     const DevCall &call = *callp;
-    const WaveLds L = carve_lds(call.cap, call.kind == kML || call.lscores != nullptr);
+    const WaveLds L = carve_lds(call.cap, call.kind == kML || call.lscores != nullptr, call.code_lds_words, code.m,
+                                code.D);
+    if (call.code_lds_words > 0) {
+        uint32_t LDS *dst = const_cast<uint32_t LDS *>(L.chw);
+        const int nw = code.D * code.m;
+#pragma unroll 8
+        for (int i = lane_id(); i < nw; i += 64) dst[i] = code.Hcw[i];
+#pragma unroll 4
+        for (int i = lane_id(); i < code.m; i += 64) {
+            dst[nw + i] = (uint32_t)code.off[i];
+            dst[nw + code.m + i] = code.sig[i];
+        }
+    }
     const unsigned total = call.counters[0];
     // shots per claim: the tuned batch when the worklist is long, fewer when
     // there are not several claims per wave (small batches finish sooner)
@@ -1393,7 +1435,21 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
     B = B < 1u ? 1u : (B > (unsigned)call.claim_batch ? (unsigned)call.claim_batch : B);
     int ctask = -1;        // task whose arrays and hash table this wave's LDS holds
     unsigned hmask = 0;
-    for (int i = lane_id(); i < kNcrLdsRows * kNcrStride; i += 64) L.ncr[i] = code.nCr[i];
+    {
+        // 17 loads per lane, issued together
+        constexpr int kN = kNcrLdsRows * kNcrStride;
+        unsigned long long v[(kN + 63) / 64];
+#pragma unroll
+        for (int q = 0; q < (kN + 63) / 64; ++q) {
+            const int i = lane_id() + 64 * q;
+            v[q] = (i < kN) ? code.nCr[i] : 0ull;
+        }
+#pragma unroll
+        for (int q = 0; q < (kN + 63) / 64; ++q) {
+            const int i = lane_id() + 64 * q;
+            if (i < kN) L.ncr[i] = v[q];
+        }
+    }
     __syncthreads();
     // Claims.  The worklist positions are dealt over kClaimWords claim words
     // (word x hands out positions x, x + 8, x + 16, ...), each in its own cache
@@ -1724,7 +1780,8 @@ hipError_t grand_launch_triage(const DevCode *code, const DevCall *call, int gri
 }
 
 hipError_t grand_launch_fused(const DevCode *code, const DevCall *call, const DevCall *hcall, int grid, hipStream_t st) {
-    const size_t lds = grand_fused_lds_bytes(hcall->cap, hcall->kind == grand::kML || hcall->lscores != nullptr);
+    const size_t lds = grand_fused_lds_bytes(hcall->cap, hcall->kind == grand::kML || hcall->lscores != nullptr) +
+                       sizeof(uint32_t) * (size_t)hcall->code_lds_words;
     hipLaunchKernelGGL(grand::fused_kernel, dim3(grid), dim3(64), lds, st, code, call);
     return hipGetLastError();
 }

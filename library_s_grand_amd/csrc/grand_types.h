@@ -45,6 +45,7 @@ struct DevCode {
 // owner wave publishes it; any wave claims chunks of 64 * R prefixes with one
 // fetch-and-add on ctl = (generation << 52) | (nchunks << 32) | next_chunk.
 constexpr int kTaskSlots = 128;
+constexpr int kCodeLdsMaxWords = 4096;   // codes up to 16 KB of (D + 2) * m words are staged in LDS
 
 // Claim words of the persistent search kernel: kClaimWords 64-bit words, one
 // per 128-byte line, starting at 32-bit counter index 2 * kClaimBase.
@@ -104,7 +105,7 @@ struct DevCall {
     int lstride;
     int lper_column;                  // 1: lscores indexed by column, 0: by position in the first active list
     int lmaxw;                        // pattern weights 1..lmaxw are tried (<= kMaxWin)
-    int lpad;
+    int code_lds_words;               // search kernel: (D + 2) * m words of the code staged in LDS, 0 = read global
     unsigned long long lbudget;       // patterns the learned pass may test before plain search takes over
 };
 
