@@ -16,6 +16,9 @@ Reported beside the fused engine:
   * a bit-exactness check of the fused results against the oracle,
   * the serving loop on a mixed BB72/BB144/BB288 Poisson request stream
     (request latency p50 / p99, served shots/s),
+  * BB72 grand_decode_cc (10,000 shots, W=3) and the 216 x 3000 DEM with
+    grand_decode_ML_cln (1,000,000 shots, W=4, max_size=64), each checked
+    bit-exact against the oracle (all BB72 shots, the first 2,000 DEM shots),
   * the learned search order on the 216 x 3000 DEM (grand_decode_prob_cln):
     trained on 2M plain-decoder pairs on the GPU, mean patterns per shot and
     logical error rate against plain decoding on 1M held-out shots.
@@ -93,6 +96,77 @@ def cpu_baseline(hx, hz, errors, prob_file, fused_res):
     return single, multi, bad == 0, o
 
 
+def leg_bb72_cc(lib, oracle_path):
+    """BB72 grand_decode_cc, 10,000 shots of X errors (p = 0.005), max_error_weight = 3: device-resident
+    throughput and a bit-exact check of every shot against the oracle."""
+    import torch
+    hx, hz = codes.bb(72)
+    errors = codes.sample_errors(np.random.default_rng(SEED + 72), 10_000, 0.005, hx.shape[1])
+    cfg = G.make_config(max_error_weight=3)
+    s = Session(G.KIND_CC, hx, hz, cfg, type_of_error="x", lib=lib)
+    s.enable_timing(True)
+    d_e = torch.from_numpy(errors).cuda()
+    d_r = torch.empty(errors.shape[0] * 72, dtype=torch.uint8, device="cuda")
+    s.decode(d_e, True, d_r)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(10):
+        s.decode(d_e, True, d_r)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) / 10 * 1e3
+    res = results_from_bytes(d_r.cpu().numpy())
+    s.close()
+    out = {"workload": "BB72 [[72,12,6]] grand_decode_cc, 10000 shots, X errors p=0.005, max_error_weight=3",
+           "shots_per_s": errors.shape[0] / (ms * 1e-3), "ms_per_step": ms}
+    if oracle_path:
+        olib = G.bind(ctypes.CDLL(oracle_path))
+        rc, o, _ = G.run_grand_shots_cc_ex(olib, hx, hz, errors, None, cfg, "x", want_decoded=False)
+        o = G.results_to_numpy(o)
+        out["bit_exact_vs_oracle"] = bool(rc == 0 and all((o[f] == res[f]).all() for f in PARITY_FIELDS))
+    return out
+
+
+def leg_dem_ml(lib, oracle_path, shots=1_000_000, sample=2000):
+    """Synthetic DEM 216 x 3000 grand_decode_ML_cln, max_error_weight = 4, max_size = 64: throughput on
+    `shots` shots sampled on the device, and a bit-exact check against the oracle on the first `sample`."""
+    import torch
+    from library_s_grand_amd import learned
+    task = learned.DemTask(W=4, max_size=64)
+    llr_path = os.path.join(task.tmp, "llr.txt")
+    codes.write_llr_file(llr_path, codes.llr_from_p(task.p))
+    det, lg = task.sample(shots, SEED + 3000)
+    s = Session(G.KIND_ML_CLN, task.H, task.L, task.cfg, n_logicos=task.NL, file=llr_path, lib=lib)
+    s.enable_timing(True)
+    d_r = torch.empty(shots * 72, dtype=torch.uint8, device="cuda")
+    s.decode(det, False, d_r, d_logicals=lg, num_shots=sample)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    s.decode(det, False, d_r, d_logicals=lg)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t
+    res = results_from_bytes(d_r.cpu().numpy())
+    if s.error_bits() != 0:
+        raise RuntimeError("decoder error bits")
+    s.close()
+    cnt = res["total_tested_combinations"].astype(np.float64)
+    out = {"workload": "synthetic DEM 216x3000 (column weight 1..4, 6 windows), 12 logicals, grand_decode_ML_cln, "
+                       f"{shots} shots, max_error_weight=4, max_size_active_positions=64",
+           "shots_per_s": shots / sec, "seconds": sec, "patterns_per_s": cnt.sum() / sec,
+           "mean_patterns": float(cnt.mean())}
+    if oracle_path:
+        olib = G.bind(ctypes.CDLL(oracle_path))
+        olib.oracle_set_threads.argtypes = [ctypes.c_int]
+        olib.oracle_set_threads(len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 1)
+        det_np, lg_np = det[:sample].cpu().numpy(), lg[:sample].cpu().numpy()
+        rc, o, _, _ = G.run_grand_ML_shots_cln_ex(olib, task.H, task.L, det_np, lg_np, task.NL, task.cfg, llr_path,
+                                                  want_decoded=False)
+        olib.oracle_set_threads(1)
+        o = G.results_to_numpy(o)
+        out["bit_exact_vs_oracle"] = bool(rc == 0 and all((o[f] == res[f][:sample]).all() for f in PARITY_FIELDS))
+        out["oracle_sample"] = sample
+    return out
+
+
 def measured_traffic():
     """HBM bytes per launch of each kernel, from the committed rocprofv3 PMC summary
     (FETCH_SIZE / WRITE_SIZE runs scaled by a copy kernel of known size)."""
@@ -117,6 +191,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-order leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in tuning defaults")
+    ap.add_argument("--no-legs", action="store_true", help="skip the BB72 and DEM ML legs")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB twice (a kernel of known traffic "
                          "for scaling hardware byte counters)")
@@ -246,6 +321,17 @@ def main():
                             "kernel": "grand::triage_kernel",
                             "bytes_per_launch": tri_bytes},
     }
+    # the reference's own entry point on host buffers (PCIe copies included; never `value`)
+    abi_lib = G.load_library()
+    G.run_grand_prob_shots_cc(abi_lib, hx, hz, errors, None, cfg, prob_file, "x")     # warm
+    t = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        rc_abi, _ = G.run_grand_prob_shots_cc(abi_lib, hx, hz, errors, None, cfg, prob_file, "x")
+    abi_s = (time.perf_counter() - t) / reps
+    out["abi_host_buffers"] = {"shots_per_s": shots / abi_s, "ms_per_call": abi_s * 1e3, "rc": rc_abi,
+                               "basis": "grand_decode_prob_cc on host int32 error rows: H2D of the rows, decode, "
+                                        "D2H of the 72-byte results, per call"}
     if not a.no_plain:
         k = min(PLAIN_SAMPLE, shots)
         ex = G.make_ex(engine=G.GRAND_ENGINE_PLAIN)
@@ -270,6 +356,10 @@ def main():
         out["cpu_baseline_all_cores"] = cbm
         out["bit_exact_vs_oracle"] = ok
     sess.close()
+    if not a.no_legs:
+        oracle_path = None if a.no_cpu else os.path.join(ROOT, "oracle", "libgrand_oracle.so")
+        out["bb72_cc"] = leg_bb72_cc(lib, oracle_path)
+        out["dem_ml_cln"] = leg_dem_ml(lib, oracle_path)
     if not a.no_serving:
         from library_s_grand_amd import serving
         out["serving"] = serving.measure(rate_hz=2000.0, n_requests=4000)
