@@ -30,12 +30,11 @@
 #include "../../include/grand_session.h"
 
 extern "C" {
-size_t grand_fused_lds_bytes(int cap, int ml);
 hipError_t grand_launch_fill(unsigned *p, unsigned v, int n, hipStream_t st);
 hipError_t grand_launch_prep(const grand::DevCall *hcall, grand::DevCall *dcall, unsigned *counters, hipStream_t st);
 hipError_t grand_launch_triage(const grand::DevCode *code, const grand::DevCall *call, int grid, hipStream_t st);
 hipError_t grand_launch_fused(const grand::DevCode *code, const grand::DevCall *call, const grand::DevCall *hcall,
-                              int grid, hipStream_t st);
+                              int code_lds_words, int grid, hipStream_t st);
 hipError_t grand_launch_expand(const uint32_t *packed, int words, int cols, int rows, int *out, hipStream_t st);
 size_t grand_plain_state_bytes();
 size_t grand_plain_record_bytes();
@@ -201,9 +200,11 @@ struct CodeEntry {
     uint64_t key = 0;
     DevCode dev{};
     DevMem mem;
-    DevMem dcode;                     // device copy of `dev`
+    DevMem dcode;                     // device copy of `dev` (lds_words = 0)
+    DevMem dcode_search;              // the search kernel's copy (lds_words > 0 when the code is staged in LDS)
     int family = kCC;
     int n = 0, m = 0, NL = 0;
+    int search_lds_words = 0;         // dcode_search's lds_words
     size_t bytes = 0;
 };
 
@@ -326,9 +327,27 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
     dc.Lp = (const uint32_t *)(base + segs[iL].off);
     dc.llr = (const float *)(base + segs[iLlr].off);
     dc.nCr = rt().nCr.as<unsigned long long>();
+    dc.lds_words = 0;
+    dc.lds_post = 0;
     if (!ce->dcode.ensure(sizeof(DevCode)) ||
         !hip_check(hipMemcpy(ce->dcode.p, &dc, sizeof(DevCode), hipMemcpyHostToDevice), "hipMemcpy DevCode"))
         return nullptr;
+    // the search kernel stages the column words, offsets and signatures of a
+    // small code in LDS, and the degeneracy data (pivot rows, RREF rows) too
+    // when they also fit
+    DevCode ds = dc;
+    const long long cw = (long long)(D + 2) * m;
+    const long long post = (family == kCC) ? (long long)m + (long long)rank * M32 : 0;
+    if (cw + post <= kCodeLdsMaxWords) {
+        ds.lds_words = (int)(cw + post);
+        ds.lds_post = post > 0 ? 1 : 0;
+    } else if (cw <= kCodeLdsMaxWords) {
+        ds.lds_words = (int)cw;
+    }
+    if (!ce->dcode_search.ensure(sizeof(DevCode)) ||
+        !hip_check(hipMemcpy(ce->dcode_search.p, &ds, sizeof(DevCode), hipMemcpyHostToDevice), "hipMemcpy DevCode"))
+        return nullptr;
+    ce->search_lds_words = ds.lds_words;
     return ce;
 }
 
@@ -499,8 +518,6 @@ GrandSession *create_session(int kind, const int *H, const int *H2, int n, int m
     int cap = std::min(m, kMaxTam);
     if (family == kCLN) cap = std::min(cap, std::max(0, cfg->max_size_active_positions));
     c.cap = cap;
-    c.rmax = g_tuning.rmax;
-    c.claim_batch = (s->ex.claim_batch > 0) ? s->ex.claim_batch : g_tuning.claim_batch;
     c.ms_per_tick = R.ms_per_tick;
     return s;
 }
@@ -677,6 +694,12 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
                           : st_ex > 0 ? (unsigned long long)st_ex
                           : (unsigned long long)g_tuning.split_threshold;
     }
+    // tuning knobs are read at every decode, so the autotuner can vary them on a live session
+    c.rmax = g_tuning.rmax;
+    c.claim_batch = std::max(1, std::min((s->ex.claim_batch > 0) ? s->ex.claim_batch : g_tuning.claim_batch, kMaxBatch));
+    // one claimed shot in the search kernel's LDS: id, signature, syndrome words,
+    // then the packed errors (cc with errors) or the observed logicals (cln)
+    c.stage_words = 2 + code.D + (c.input_errors ? code.M32 : 0) + (s->family == kCLN ? code.NL32 : 0);
 
     // The task table is cleared once when allocated: every launch leaves all
     // slots idle (each owner waits for its chunks and frees its slot).  The
@@ -701,14 +724,9 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
         if (s->timing) HIPX(hipEventRecord(s->ev[2], st));
         return rc;
     }
-    // small codes are staged in each search wave's LDS (column words, offsets, signatures)
-    {
-        const long long words = (long long)(code.D + 2) * code.m;
-        c.code_lds_words = (words <= kCodeLdsMaxWords) ? (int)words : 0;
-    }
     const int wpc = (s->ex.waves_per_cu > 0) ? s->ex.waves_per_cu : g_tuning.waves_per_cu;
     const int grid = std::max(1, std::min(R.cu_count * wpc, num_shots));
-    HIPX(grand_launch_fused(dcode, dcall, &c, grid, st));
+    HIPX(grand_launch_fused(s->code->dcode_search.as<DevCode>(), dcall, &c, s->code->search_lds_words, grid, st));
     if (s->timing) HIPX(hipEventRecord(s->ev[2], st));
     return 0;
 }
@@ -1052,7 +1070,7 @@ int grand_device_info(int *cus, int *clock_khz, int *wall_clock_khz) {
 // Tuning knobs (written by the autotuner).
 void grand_set_tuning(int waves_per_cu, int claim_batch, int rmax, int split_threshold) {
     if (waves_per_cu > 0) g_tuning.waves_per_cu = waves_per_cu;
-    if (claim_batch > 0) g_tuning.claim_batch = claim_batch;
+    if (claim_batch > 0) g_tuning.claim_batch = std::min(claim_batch, kMaxBatch);
     if (rmax > 0) g_tuning.rmax = rmax;
     if (split_threshold > 0) g_tuning.split_threshold = split_threshold;
 }

@@ -167,10 +167,7 @@ struct WaveLds {
     uint32_t LDS *sigP;
     float LDS *llrP;
     unsigned long long LDS *ht;   // (signature << 32) | position, empty = kHtEmpty
-    const uint32_t LDS *chw;   // code staged in LDS by the search kernel (small codes), or null:
-    const int LDS *coff;       //   column words [D][m], score offsets [m], signatures [m]
-    const uint32_t LDS *csig;
-    uint32_t LDS *ew;  // packed error words of the shot (triage), kMaxM / 32
+    uint32_t LDS *stage;       // search kernel: the claimed shots, DevCall::stage_words each
     uint32_t LDS *sw;  // syndrome words of the shot being searched, kMaxWords
     uint32_t LDS *sw2; // syndrome minus one column (learned pass), kMaxWords
     unsigned long long LDS *ncr; // C(n, k) for n < kNcrLdsRows, k < 8 (search kernel only)
@@ -196,14 +193,20 @@ __host__ __device__ __forceinline__ int ht_capacity(int cap) {
     return h;
 }
 
-__device__ __forceinline__ WaveLds carve_lds(int cap, bool ml, int code_words = 0, int m = 0, int D = 0) {
+// Every wave's LDS, in order: syndrome words (kMaxWords), the learned pass's
+// reduced syndrome (kMaxWords), C(n, k) for n < kNcrLdsRows (search kernel),
+// the staged code (DevCode::lds_words, search kernel), then the per-class
+// arrays sized by the active-list capacity and the claimed shots.
+constexpr int kLdsNcrOffset = 2 * kMaxWords;
+constexpr int kLdsCodeOffset = kLdsNcrOffset + 2 * kNcrLdsRows * kNcrStride;
+
+__device__ __forceinline__ WaveLds carve_lds(int cap, bool ml, int code_words = 0, int batch_words = 0) {
     uint32_t LDS *base = lds_base();
     WaveLds L;
-    L.ew = base;
-    L.sw = base + kMaxM / 32;
+    L.sw = base;
     L.sw2 = L.sw + kMaxWords;
-    L.ncr = (unsigned long long LDS *)(L.sw2 + kMaxWords);
-    uint32_t LDS *p = L.sw2 + kMaxWords + 2 * kNcrLdsRows * kNcrStride;
+    L.ncr = (unsigned long long LDS *)(base + kLdsNcrOffset);
+    uint32_t LDS *p = base + kLdsCodeOffset + code_words;
     L.ht = (unsigned long long LDS *)p; p += 2 * ht_capacity(cap) + bitmap_words(cap);
     L.A = (int LDS *)p; p += cap;
     L.sigA = p; p += cap;
@@ -215,21 +218,35 @@ __device__ __forceinline__ WaveLds carve_lds(int cap, bool ml, int code_words = 
     } else {
         L.llrA = nullptr; L.P = nullptr; L.sigP = nullptr; L.llrP = nullptr;
     }
-    if (code_words > 0) {
-        L.chw = p; p += (size_t)D * m;
-        L.coff = (const int LDS *)p; p += m;
-        L.csig = p;
-    } else {
-        L.chw = nullptr; L.coff = nullptr; L.csig = nullptr;
-    }
+    L.stage = batch_words > 0 ? p : nullptr;
     return L;
 }
 
-// The search kernel keeps C(n, k), n < kNcrLdsRows, in every wave's LDS at the
-// same offset (after the shot's packed errors and syndrome words), filled when
-// the kernel starts; lookups with small n read it instead of the global table.
+// The search kernel keeps C(n, k), n < kNcrLdsRows, in every wave's LDS at a
+// fixed offset, filled when the kernel starts; lookups with small n read it
+// instead of the global table.
 __device__ __forceinline__ const unsigned long long LDS *ncr_lds() {
-    return (const unsigned long long LDS *)(lds_base() + kMaxM / 32 + 2 * kMaxWords);
+    return (const unsigned long long LDS *)(lds_base() + kLdsNcrOffset);
+}
+
+// The staged code (search kernel, code.lds_words > 0): column words [D][m],
+// offsets [m], signatures [m], pivot rows [m] and RREF rows [rank][M32].
+__device__ __forceinline__ const uint32_t LDS *code_lds() { return lds_base() + kLdsCodeOffset; }
+__device__ __forceinline__ const uint32_t LDS *code_lds_off(const DevCode &code) {
+    return code_lds() + (size_t)code.D * code.m;
+}
+__device__ __forceinline__ const uint32_t LDS *code_lds_sig(const DevCode &code) {
+    return code_lds() + (size_t)(code.D + 1) * code.m;
+}
+__device__ __forceinline__ const int LDS *code_lds_pivrow(const DevCode &code) {
+    return (const int LDS *)(code_lds() + (size_t)(code.D + 2) * code.m);
+}
+__device__ __forceinline__ const uint32_t LDS *code_lds_R(const DevCode &code) {
+    return code_lds() + (size_t)(code.D + 3) * code.m;
+}
+// word d of column c
+__device__ __forceinline__ uint32_t col_word(const DevCode &code, int c, int d) {
+    return code.lds_words ? code_lds()[(size_t)d * code.m + c] : code.Hcm[(size_t)c * code.D + d];
 }
 
 __device__ __forceinline__ uint64_t ncr_w(const DevCode &code, int n, int k) {
@@ -246,7 +263,7 @@ __device__ __forceinline__ bool exact_match(const DevCode &code, const uint32_t 
         uint32_t x = s[d];
 #pragma unroll
         for (int l = 0; l < MAXW; ++l)
-            if (l < w) x ^= code.Hcm[(size_t)cols[l] * code.D + d];
+            if (l < w) x ^= col_word(code, cols[l], d);
         ok = ok && (x == 0u);
     }
     return ok;
@@ -286,7 +303,9 @@ __device__ __forceinline__ int build_active_t(PW Hcw, PO off, PS sig, int D, int
 }
 
 __device__ int build_active(const DevCode &code, const uint32_t LDS *s, int thr, const WaveLds &L, int cap) {
-    if (L.chw) return build_active_t(L.chw, L.coff, L.csig, code.D, code.m, s, thr, L, cap);
+    if (code.lds_words)
+        return build_active_t(code_lds(), (const int LDS *)code_lds_off(code), code_lds_sig(code), code.D, code.m, s,
+                              thr, L, cap);
     return build_active_t(code.Hcw, code.off, code.sig, code.D, code.m, s, thr, L, cap);
 }
 
@@ -996,8 +1015,11 @@ __device__ void write_result(GrandDecoderResult *dst, int found, int fw, int ow,
 }
 
 // Post-processing and result record of a searched shot (grand_decoder.cu:1359-1370
-// for the CSS family, :1623-1648 for the DEM family).
-__device__ void finalize_shot(const DevCode &code, const DevCall &call, int shot, const ShotOutcome &o) {
+// for the CSS family, :1623-1648 for the DEM family).  The shot's packed errors
+// (cc) or observed logicals (cln) come from its staged copy in LDS (`slot`, laid
+// out as DevCall::stage_words describes) or, with slot null, from global memory.
+__device__ void finalize_shot(const DevCode &code, const DevCall &call, int shot, const ShotOutcome &o,
+                              const uint32_t LDS *slot) {
     const int lane = lane_id();
     const int M32 = code.M32;
     uint32_t dw[2] = {0u, 0u};   // decoded pattern words lane + 64 q, q < 2 (m <= 4096)
@@ -1021,7 +1043,7 @@ __device__ void finalize_shot(const DevCode &code, const DevCall &call, int shot
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const int wd = lane + 64 * q;
-            if (wd < M32) ew[q] = call.e_packed[(size_t)shot * M32 + wd];
+            if (wd < M32) ew[q] = slot ? slot[2 + code.D + wd] : call.e_packed[(size_t)shot * M32 + wd];
             pc += __popc(ew[q]);
             same = same && (ew[q] == dw[q]);
         }
@@ -1044,7 +1066,7 @@ __device__ void finalize_shot(const DevCode &code, const DevCall &call, int shot
                 if (pend[0]) { c = lane * 32 + __ffs(pend[0]) - 1; pend[0] &= pend[0] - 1u; }
                 else if (pend[1]) { c = (lane + 64) * 32 + __ffs(pend[1]) - 1; pend[1] &= pend[1] - 1u; }
                 if (ballot(c >= 0) == 0ull) break;
-                const int r = (c >= 0) ? code.pivrow[c] : -1;
+                const int r = (c >= 0) ? (code.lds_post ? code_lds_pivrow(code)[c] : code.pivrow[c]) : -1;
                 uint64_t sel = ballot(r >= 0);
                 while (sel) {
                     const int src = __ffsll((long long)sel) - 1;
@@ -1053,7 +1075,9 @@ __device__ void finalize_shot(const DevCode &code, const DevCall &call, int shot
 #pragma unroll
                     for (int q = 0; q < 2; ++q) {
                         const int w2 = lane + 64 * q;
-                        if (w2 < M32) acc[q] ^= code.R[(size_t)rr * M32 + w2];
+                        if (w2 < M32)
+                            acc[q] ^= code.lds_post ? code_lds_R(code)[(size_t)rr * M32 + w2]
+                                                    : code.R[(size_t)rr * M32 + w2];
                     }
                 }
             }
@@ -1071,7 +1095,8 @@ __device__ void finalize_shot(const DevCode &code, const DevCall &call, int shot
                     if (l < o.fw) le ^= code.Lp[(size_t)o.win[l] * NL32 + wd];
             }
             call.declog[(size_t)shot * NL32 + wd] = le;
-            same = same && (le == call.lg_packed[(size_t)shot * NL32 + wd]);
+            const uint32_t lo = slot ? slot[2 + code.D + wd] : call.lg_packed[(size_t)shot * NL32 + wd];
+            same = same && (le == lo);
         }
         if (o.found) deg = ballot(!same) == 0ull;
     }
@@ -1155,7 +1180,7 @@ __device__ bool learned_pass(const DevCode &code, const DevCall &call, int shot,
     bool stop = false;
     for (int k = 0; k < tam && !stop; ++k) {
         const int ck = L.P[k];
-        if (lane < kMaxWords) L.sw2[lane] = (lane < code.D) ? (s[lane] ^ code.Hcm[(size_t)ck * code.D + lane]) : 0u;
+        if (lane < kMaxWords) L.sw2[lane] = (lane < code.D) ? (s[lane] ^ col_word(code, ck, lane)) : 0u;
         __syncthreads();
         const uint32_t t2 = t0 ^ L.sigP[k];
         for (int wp = 0; wp < maxw; ++wp) {
@@ -1200,14 +1225,16 @@ __device__ bool learned_pass(const DevCode &code, const DevCall &call, int shot,
 
 // The whole per-shot pipeline in one wave (grand_decoder.cu:1284-1343 and the
 // equivalent loops of the other four drivers).
-__device__ void decode_shot_wave(const DevCode &code, const DevCall &call, int shot, const WaveLds &L) {
+// One claimed shot, staged in LDS by stage_claim: slot = [shot, signature,
+// syndrome words, error words or observed logicals].
+__device__ void decode_shot_wave(const DevCode &code, const DevCall &call, const uint32_t LDS *slot, const WaveLds &L) {
     ShotOutcome o;
     o.t0 = now_ticks();
-    if (lane_id() < kMaxWords)
-        L.sw[lane_id()] = (lane_id() < code.D) ? call.s_packed[(size_t)shot * code.D + lane_id()] : 0u;
+    const int shot = (int)uni(slot[0]);
+    if (lane_id() < kMaxWords) L.sw[lane_id()] = (lane_id() < code.D) ? slot[2 + lane_id()] : 0u;
     __syncthreads();
     const uint32_t LDS *s = L.sw;
-    const uint32_t t0 = call.s_sig[shot];
+    const uint32_t t0 = uni(slot[1]);
 
     o.found = 0; o.fw = 0; o.count = 0; o.wops = 0; o.hibrido = -2;
 #pragma unroll
@@ -1250,7 +1277,44 @@ __device__ void decode_shot_wave(const DevCode &code, const DevCall &call, int s
                 if (b + 1 < o.fw && o.win[b] > o.win[b + 1]) { const int t = o.win[b]; o.win[b] = o.win[b + 1]; o.win[b + 1] = t; }
     }
     o.t1 = now_ticks();
-    finalize_shot(code, call, shot, o);
+    finalize_shot(code, call, shot, o, slot);
+}
+
+// Stages the shots of one claim (worklist positions p0, p0 + 8, ...; nb of them)
+// in LDS: the worklist entries first, then every shot's words in one round of
+// loads, so the shots decode without waiting on global memory between them.
+__device__ void stage_claim(const DevCode &code, const DevCall &call, unsigned p0, int nb, const WaveLds &L) {
+    const int lane = lane_id();
+    const int SW = call.stage_words;
+    const int per = SW - 1;                        // loaded words per shot: signature, syndrome, errors or logicals
+    const int myshot = (lane < nb) ? call.worklist[p0 + (unsigned)kClaimWords * (unsigned)lane] : 0;
+    if (lane < nb) L.stage[lane * SW] = (uint32_t)myshot;
+    const int E = call.input_errors ? code.M32 : 0;
+    const int tot = nb * per;
+    for (int k0 = 0; k0 < tot; k0 += 256) {
+        uint32_t v[4];
+        int dst[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = k0 + lane + 64 * q;
+            const int j = (k < tot) ? k / per : 0;
+            const int f = k - j * per;
+            const int shot = __shfl(myshot, j, 64);
+            uint32_t x = 0u;
+            if (k < tot) {
+                if (f == 0) x = call.s_sig[shot];
+                else if (f <= code.D) x = call.s_packed[(size_t)shot * code.D + (f - 1)];
+                else if (f <= code.D + E) x = call.e_packed[(size_t)shot * code.M32 + (f - 1 - code.D)];
+                else x = call.lg_packed[(size_t)shot * code.NL32 + (f - 1 - code.D - E)];
+            }
+            v[q] = x;
+            dst[q] = (k < tot) ? j * SW + 1 + f : -1;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (dst[q] >= 0) L.stage[dst[q]] = v[q];
+    }
+    __syncthreads();
 }
 
 // ================================================================== kernels
@@ -1415,10 +1479,10 @@ __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ 
 __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ codep, const DevCall *__restrict__ callp) {
     const DevCode &code = *codep;
     const DevCall &call = *callp;
-    const WaveLds L = carve_lds(call.cap, call.kind == kML || call.lscores != nullptr, call.code_lds_words, code.m,
-                                code.D);
-    if (call.code_lds_words > 0) {
-        uint32_t LDS *dst = const_cast<uint32_t LDS *>(L.chw);
+    const WaveLds L = carve_lds(call.cap, call.kind == kML || call.lscores != nullptr, code.lds_words,
+                                call.claim_batch * call.stage_words);
+    if (code.lds_words > 0) {
+        uint32_t LDS *dst = lds_base() + kLdsCodeOffset;
         const int nw = code.D * code.m;
 #pragma unroll 8
         for (int i = lane_id(); i < nw; i += 64) dst[i] = code.Hcw[i];
@@ -1426,6 +1490,14 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
         for (int i = lane_id(); i < code.m; i += 64) {
             dst[nw + i] = (uint32_t)code.off[i];
             dst[nw + code.m + i] = code.sig[i];
+        }
+        if (code.lds_post) {
+            uint32_t LDS *post = dst + nw + 2 * code.m;
+            const int nr = code.rank * code.M32;
+#pragma unroll 4
+            for (int i = lane_id(); i < code.m; i += 64) post[i] = (uint32_t)code.pivrow[i];
+#pragma unroll 4
+            for (int i = lane_id(); i < nr; i += 64) post[code.m + i] = code.R[i];
         }
     }
     const unsigned total = call.counters[0];
@@ -1483,11 +1555,10 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
             ++claims;
             const unsigned p0 = (unsigned)cur + kClaimWords * got;
             if (got < total && p0 < total) {
-                for (unsigned j = 0; j < B; ++j) {
-                    const unsigned p = p0 + kClaimWords * j;
-                    if (p >= total) break;
-                    decode_shot_wave(code, call, call.worklist[p], L);
-                }
+                const unsigned left = (total - 1u - p0) / kClaimWords + 1u;   // positions p0, p0 + 8, ... below total
+                const int nb = (int)(left < B ? left : B);
+                stage_claim(code, call, p0, nb, L);
+                for (int j = 0; j < nb; ++j) decode_shot_wave(code, call, L.stage + j * call.stage_words, L);
                 ctask = -1;
                 continue;
             }
@@ -1718,7 +1789,7 @@ __global__ __launch_bounds__(64) void plain_finalize_kernel(const DevCode *__res
             for (int b = 0; b < kMaxWin - 1 - a; ++b)
                 if (b + 1 < o.fw && o.win[b] > o.win[b + 1]) { const int t = o.win[b]; o.win[b] = o.win[b + 1]; o.win[b + 1] = t; }
     }
-    finalize_shot(code, call, shot, o);
+    finalize_shot(code, call, shot, o, nullptr);
 }
 
 // Expands packed words to the ABI's int32 0/1 rows.
@@ -1757,11 +1828,10 @@ extern "C" {
 using grand::DevCall;
 using grand::DevCode;
 
+// LDS of one wave without the staged code and claimed shots (carve_lds).
 size_t grand_fused_lds_bytes(int cap, int ml) {
-    return sizeof(uint32_t) * ((size_t)grand::kMaxM / 32 + 2 * grand::kMaxWords +
-                               2 * (size_t)grand::kNcrLdsRows * grand::kNcrStride + 2 * (size_t)grand::ht_capacity(cap) +
-                               (size_t)grand::bitmap_words(cap) +
-                               (size_t)cap * (ml ? 6 : 2));
+    return sizeof(uint32_t) * ((size_t)grand::kLdsCodeOffset + 2 * (size_t)grand::ht_capacity(cap) +
+                               (size_t)grand::bitmap_words(cap) + (size_t)cap * (ml ? 6 : 2));
 }
 
 hipError_t grand_launch_prep(const DevCall *hcall, DevCall *dcall, unsigned *counters, hipStream_t st) {
@@ -1779,9 +1849,10 @@ hipError_t grand_launch_triage(const DevCode *code, const DevCall *call, int gri
     return hipGetLastError();
 }
 
-hipError_t grand_launch_fused(const DevCode *code, const DevCall *call, const DevCall *hcall, int grid, hipStream_t st) {
+hipError_t grand_launch_fused(const DevCode *code, const DevCall *call, const DevCall *hcall, int code_lds_words,
+                              int grid, hipStream_t st) {
     const size_t lds = grand_fused_lds_bytes(hcall->cap, hcall->kind == grand::kML || hcall->lscores != nullptr) +
-                       sizeof(uint32_t) * (size_t)hcall->code_lds_words;
+                       sizeof(uint32_t) * ((size_t)code_lds_words + (size_t)hcall->claim_batch * hcall->stage_words);
     hipLaunchKernelGGL(grand::fused_kernel, dim3(grid), dim3(64), lds, st, code, call);
     return hipGetLastError();
 }

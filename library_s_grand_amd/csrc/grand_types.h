@@ -39,13 +39,21 @@ struct DevCode {
     const uint32_t *Lp;               // [m][NL32] logical flips per mechanism (cln)
     const float *llr;                 // [m] (ML)
     const unsigned long long *nCr;    // [(kMaxTam + 1) * kNcrStride]
+    // Small codes are staged in the search kernel's LDS.  The search kernel gets
+    // its own copy of this descriptor with lds_words > 0; every other kernel's
+    // copy has 0 and reads global memory.  Staged words, in order: column words
+    // [D][m], score offsets [m], signatures [m], then (lds_post = 1, cc) the RREF
+    // row of each pivot column [m] and the RREF rows [rank][M32].
+    int lds_words;
+    int lds_post;
 };
 
 // A weight class too large for one wave, shared through the task table.  The
 // owner wave publishes it; any wave claims chunks of 64 * R prefixes with one
 // fetch-and-add on ctl = (generation << 52) | (nchunks << 32) | next_chunk.
 constexpr int kTaskSlots = 128;
-constexpr int kCodeLdsMaxWords = 4096;   // codes up to 16 KB of (D + 2) * m words are staged in LDS
+constexpr int kCodeLdsMaxWords = 4096;   // codes up to 16 KB of staged words (DevCode::lds_words) go to LDS
+constexpr int kMaxBatch = 8;             // most shots per worklist claim
 
 // Claim words of the persistent search kernel: kClaimWords 64-bit words, one
 // per 128-byte line, starting at 32-bit counter index 2 * kClaimBase.
@@ -78,7 +86,9 @@ struct DevCall {
     int mask_order[kMaxMasks][3];
     int cap;                          // active-list capacity held in LDS (<= kMaxTam)
     int rmax;                         // patterns per lane per chunk in the range path
-    int claim_batch;                  // shots taken per worklist claim
+    int claim_batch;                  // shots taken per worklist claim (<= kMaxBatch)
+    int stage_words;                  // search kernel: LDS words per claimed shot (id, signature, syndrome,
+                                      // error words for cc with errors, observed logicals for cln)
     int num_shots;
     int input_errors;                 // 1: input rows are error patterns (cc), 0: syndromes/detectors
     double ms_per_tick;               // device wall clock period
@@ -105,7 +115,6 @@ struct DevCall {
     int lstride;
     int lper_column;                  // 1: lscores indexed by column, 0: by position in the first active list
     int lmaxw;                        // pattern weights 1..lmaxw are tried (<= kMaxWin)
-    int code_lds_words;               // search kernel: (D + 2) * m words of the code staged in LDS, 0 = read global
     unsigned long long lbudget;       // patterns the learned pass may test before plain search takes over
 };
 
