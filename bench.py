@@ -232,9 +232,10 @@ def main():
               f"copy", file=sys.stderr)
         del src, dst
 
+    from library_s_grand_amd import autotune
     tuning = None
+    tuning_defaults = autotune.get(lib)
     if not a.no_tune:
-        from library_s_grand_amd import autotune
 
         def tune_run():
             step()
@@ -356,6 +357,9 @@ def main():
         out["cpu_baseline_all_cores"] = cbm
         out["bit_exact_vs_oracle"] = ok
     sess.close()
+    if tuning is not None:
+        # the tuned knobs belong to this workload: the other legs run with the built-in defaults
+        autotune.apply(lib, tuning_defaults)
     if not a.no_legs:
         oracle_path = None if a.no_cpu else os.path.join(ROOT, "oracle", "libgrand_oracle.so")
         out["bb72_cc"] = leg_bb72_cc(lib, oracle_path)

@@ -123,7 +123,9 @@ typedef struct {
     int waves_per_cu;    /* persistent-grid occupancy; 0 = autotuned */
     int claim_batch;     /* shots claimed per queue access; 0 = autotuned */
     int split_threshold; /* weight classes larger than this are shared across waves; 0 = autotuned, -1 = never */
-    int reserved[4];
+    int triage_search;   /* fused engine: shots whose first iteration is small are searched one per lane in the
+                            triage kernel; 0 = on where it applies, -1 = off (every shot goes to the search kernel) */
+    int reserved[3];
 } GrandDecoderExConfig;
 
 int grand_decode_cc_ex(const int *Hx, const int *Hz, int n, int m, int *errors, int *syndromes, int num_shots,

@@ -118,6 +118,25 @@ size_t grand_session_footprint(GrandSession *s, int num_shots);
 
 void grand_session_destroy(GrandSession *s);
 
+/* Compute units, peak engine clock and device wall-clock rate (kHz) of the
+ * current device.  Returns 0, or -1 without a usable GPU. */
+int grand_device_info(int *cus, int *clock_khz, int *wall_clock_khz);
+
+/* Process-wide knobs of the persistent search kernel, read by every decode
+ * (values <= 0 keep the current setting): workgroups per CU, shots per
+ * worklist claim (at most 8), prefixes per lane per chunk, and the class size
+ * in prefixes above which a weight class is shared by all waves. */
+void grand_set_tuning(int waves_per_cu, int claim_batch, int rmax, int split_threshold);
+void grand_get_tuning(int *waves_per_cu, int *claim_batch, int *rmax, int *split_threshold);
+
+/* Degeneracy words of a CSS pair (host only, no GPU needed): H is the matrix
+ * the syndromes come from and other the one whose row space decides
+ * `degenerate` (n x m, row-major int 0/1 each).  For v in ker(H), v is in
+ * rowspace(other) iff the XOR of words[c] over c in supp(v) is 0.  Returns the
+ * number of functionals (at most 32) and fills words[0..m), or -1 when more
+ * than 32 would be needed (the decoder then walks the RREF rows instead). */
+int grand_degeneracy_words(const int *H, const int *other, int n, int m, unsigned *words);
+
 #ifdef __cplusplus
 }
 #endif

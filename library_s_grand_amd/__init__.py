@@ -63,7 +63,8 @@ class GrandDecoderExConfig(ctypes.Structure):
         ("waves_per_cu", ctypes.c_int),
         ("claim_batch", ctypes.c_int),
         ("split_threshold", ctypes.c_int),
-        ("reserved", ctypes.c_int * 4),
+        ("triage_search", ctypes.c_int),
+        ("reserved", ctypes.c_int * 3),
     ]
 
 
@@ -139,9 +140,11 @@ def make_config(max_error_weight=4, threads_per_block=64, use_gpu_sum=1, max_siz
                               threshold_for_hybrid, mode_of_search)
 
 
-def make_ex(engine=GRAND_ENGINE_FUSED, waves_per_cu=0, claim_batch=0, split_threshold=0) -> GrandDecoderExConfig:
+def make_ex(engine=GRAND_ENGINE_FUSED, waves_per_cu=0, claim_batch=0, split_threshold=0,
+            triage_search=0) -> GrandDecoderExConfig:
     ex = GrandDecoderExConfig()
     ex.engine, ex.waves_per_cu, ex.claim_batch, ex.split_threshold = engine, waves_per_cu, claim_batch, split_threshold
+    ex.triage_search = triage_search
     return ex
 
 

@@ -30,21 +30,21 @@
 #include "../../include/grand_session.h"
 
 extern "C" {
-hipError_t grand_launch_fill(unsigned *p, unsigned v, int n, hipStream_t st);
-hipError_t grand_launch_prep(const grand::DevCall *hcall, grand::DevCall *dcall, unsigned *counters, hipStream_t st);
-hipError_t grand_launch_triage(const grand::DevCode *code, const grand::DevCall *call, int grid, hipStream_t st);
-hipError_t grand_launch_fused(const grand::DevCode *code, const grand::DevCall *call, const grand::DevCall *hcall,
+GRAND_INTERNAL hipError_t grand_launch_fill(unsigned *p, unsigned v, int n, hipStream_t st);
+GRAND_INTERNAL hipError_t grand_launch_prep(const grand::DevCall *hcall, grand::DevCall *dcall, unsigned *counters, hipStream_t st);
+GRAND_INTERNAL hipError_t grand_launch_triage(const grand::DevCode *code, const grand::DevCall *call, int grid, hipStream_t st);
+GRAND_INTERNAL hipError_t grand_launch_fused(const grand::DevCode *code, const grand::DevCall *call, const grand::DevCall *hcall,
                               int code_lds_words, int grid, hipStream_t st);
-hipError_t grand_launch_expand(const uint32_t *packed, int words, int cols, int rows, int *out, hipStream_t st);
-size_t grand_plain_state_bytes();
-size_t grand_plain_record_bytes();
-hipError_t grand_launch_plain_active(const grand::DevCode *code, const grand::DevCall *call, int cap, int shot,
+GRAND_INTERNAL hipError_t grand_launch_expand(const uint32_t *packed, int words, int cols, int rows, int *out, hipStream_t st);
+GRAND_INTERNAL size_t grand_plain_state_bytes();
+GRAND_INTERNAL size_t grand_plain_record_bytes();
+GRAND_INTERNAL hipError_t grand_launch_plain_active(const grand::DevCode *code, const grand::DevCall *call, int cap, int shot,
                                      int thr, int sort_tam, void *scratch, void *state, hipStream_t st);
-hipError_t grand_launch_plain_class(const grand::DevCode *code, const grand::DevCall *call, int cap, int shot, int w,
+GRAND_INTERNAL hipError_t grand_launch_plain_class(const grand::DevCode *code, const grand::DevCall *call, int cap, int shot, int w,
                                     int tam, int R, unsigned long long nchunks, int grid, int ml, void *scratch,
                                     void *state, void *records, hipStream_t st);
-hipError_t grand_launch_plain_ml_reduce(void *records, int nrec, void *out, hipStream_t st);
-hipError_t grand_launch_plain_finalize(const grand::DevCode *code, const grand::DevCall *call, int shot,
+GRAND_INTERNAL hipError_t grand_launch_plain_ml_reduce(void *records, int nrec, void *out, hipStream_t st);
+GRAND_INTERNAL hipError_t grand_launch_plain_finalize(const grand::DevCode *code, const grand::DevCall *call, int shot,
                                        const grand::ShotOutcome *o, void *scratch, unsigned long long hit_rank,
                                        hipStream_t st);
 }
@@ -196,6 +196,80 @@ uint64_t splitmix64(uint64_t &x) {
     return z ^ (z >> 31);
 }
 
+// Degeneracy words of a CSS pair.  For a found shot, v = e ^ e_hat lies in
+// ker(H) (both have syndrome H e), and v is in rowspace(other) iff v is
+// orthogonal to ker(other).  A vector of ker(other) that lies in rowspace(H) +
+// span(earlier kept ones) adds nothing for v in ker(H), so only the ones
+// independent of rowspace(H) are kept: as many as the code's logical qubits for
+// a CSS code.  With at most 32 of them, dg[c] holds their bits at column c and
+// v is in rowspace(other) iff the XOR of dg over supp(e) equals the XOR over
+// supp(e_hat).  Hrow: rows of H packed [n][M32]; R, piv: RREF of the other
+// matrix (rows [rank][M32] and pivot columns).  Returns the number of kept
+// vectors and fills dg, or returns -1 (dg empty) when more than 32 are needed.
+int degeneracy_words(const std::vector<uint32_t> &Hrow, const std::vector<uint32_t> &R, const std::vector<int> &piv,
+                     int n, int m, std::vector<uint32_t> &dg) {
+    const int M32 = (m + 31) / 32;
+    const int rank = (int)piv.size();
+    dg.clear();
+    std::vector<std::vector<uint32_t>> basis;     // reduced: each pivot column set in its own vector only
+    std::vector<int> bpiv;
+    auto bit = [](const std::vector<uint32_t> &v, int c) { return (v[c >> 5] >> (c & 31)) & 1u; };
+    auto insert = [&](std::vector<uint32_t> x) {
+        for (size_t i = 0; i < basis.size(); ++i)
+            if (bit(x, bpiv[i])) for (int j = 0; j < M32; ++j) x[j] ^= basis[i][j];
+        int p = -1;
+        for (int j = 0; j < M32 && p < 0; ++j) if (x[j]) p = j * 32 + __builtin_ctz(x[j]);
+        if (p < 0) return false;
+        for (size_t i = 0; i < basis.size(); ++i)
+            if (bit(basis[i], p)) for (int j = 0; j < M32; ++j) basis[i][j] ^= x[j];
+        basis.push_back(std::move(x));
+        bpiv.push_back(p);
+        return true;
+    };
+    for (int r = 0; r < n; ++r)
+        (void)insert(std::vector<uint32_t>(Hrow.begin() + (size_t)r * M32, Hrow.begin() + (size_t)(r + 1) * M32));
+    std::vector<char> is_piv(m, 0);
+    for (int r = 0; r < rank; ++r) is_piv[piv[r]] = 1;
+    std::vector<std::vector<uint32_t>> kept;
+    for (int f = 0; f < m; ++f) {
+        if (is_piv[f]) continue;
+        std::vector<uint32_t> k(M32, 0);          // null vector of the RREF: e_f plus the pivots of rows with bit f
+        k[f >> 5] |= 1u << (f & 31);
+        for (int r = 0; r < rank; ++r)
+            if ((R[(size_t)r * M32 + (f >> 5)] >> (f & 31)) & 1u) k[piv[r] >> 5] |= 1u << (piv[r] & 31);
+        if (insert(k)) {
+            if (kept.size() == 32) return -1;
+            kept.push_back(std::move(k));
+        }
+    }
+    dg.assign(m, 0u);
+    for (size_t j = 0; j < kept.size(); ++j)
+        for (int c = 0; c < m; ++c) if (bit(kept[j], c)) dg[c] |= 1u << j;
+    return (int)kept.size();
+}
+
+// RREF of rows [n][M32] over GF(2) (grand_decoder.cu:418-454): the reduced rows
+// [rank][M32] and their pivot columns.
+void gf2_rref(std::vector<uint32_t> rows, int n, int m, std::vector<uint32_t> &R, std::vector<int> &piv) {
+    const int M32 = (m + 31) / 32;
+    piv.clear();
+    int rr = 0;
+    for (int c = 0; c < m && rr < n; ++c) {
+        const int wd = c >> 5;
+        const uint32_t bit = 1u << (c & 31);
+        int p = -1;
+        for (int i = rr; i < n; ++i) if (rows[(size_t)i * M32 + wd] & bit) { p = i; break; }
+        if (p < 0) continue;
+        if (p != rr) for (int j = 0; j < M32; ++j) std::swap(rows[(size_t)rr * M32 + j], rows[(size_t)p * M32 + j]);
+        for (int i = 0; i < n; ++i)
+            if (i != rr && (rows[(size_t)i * M32 + wd] & bit))
+                for (int j = 0; j < M32; ++j) rows[(size_t)i * M32 + j] ^= rows[(size_t)rr * M32 + j];
+        piv.push_back(c);
+        ++rr;
+    }
+    R.assign(rows.begin(), rows.begin() + (size_t)rr * M32);
+}
+
 struct CodeEntry {
     uint64_t key = 0;
     DevCode dev{};
@@ -264,25 +338,13 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
         for (int r = 0; r < n; ++r)
             for (int c = 0; c < m; ++c)
                 if (other[(size_t)r * m + c]) rows[(size_t)r * M32 + (c >> 5)] |= 1u << (c & 31);
-        int rr = 0;
-        for (int c = 0; c < m && rr < n; ++c) {
-            const int wd = c >> 5;
-            const uint32_t bit = 1u << (c & 31);
-            int p = -1;
-            for (int i = rr; i < n; ++i) if (rows[(size_t)i * M32 + wd] & bit) { p = i; break; }
-            if (p < 0) continue;
-            if (p != rr) for (int j = 0; j < M32; ++j) std::swap(rows[(size_t)rr * M32 + j], rows[(size_t)p * M32 + j]);
-            for (int i = 0; i < n; ++i)
-                if (i != rr && (rows[(size_t)i * M32 + wd] & bit))
-                    for (int j = 0; j < M32; ++j) rows[(size_t)i * M32 + j] ^= rows[(size_t)rr * M32 + j];
-            piv.push_back(c);
-            ++rr;
-        }
-        rank = rr;
-        R.assign(rows.begin(), rows.begin() + (size_t)rank * M32);
+        gf2_rref(std::move(rows), n, m, R, piv);
+        rank = (int)piv.size();
     }
     std::vector<int> pivrow(m, -1);
     for (int r = 0; r < rank; ++r) pivrow[piv[r]] = r;
+    std::vector<uint32_t> dg;
+    if (family == kCC) (void)degeneracy_words(Hrow, R, piv, n, m, dg);
     std::vector<uint32_t> Lp;
     if (family == kCLN) {
         Lp.assign((size_t)m * NL32, 0);
@@ -304,7 +366,7 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
                  iHrow = add(Hrow.data(), Hrow.size() * 4), iSig = add(sig.data(), sig.size() * 4),
                  iRs = add(rowsig.data(), rowsig.size() * 4), iOff = add(off.data(), off.size() * 4),
                  iR = add(R.data(), R.size() * 4), iPiv = add(pivrow.data(), pivrow.size() * 4),
-                 iL = add(Lp.data(), Lp.size() * 4),
+                 iL = add(Lp.data(), Lp.size() * 4), iDg = add(dg.data(), dg.size() * 4),
                  iLlr = add(llr ? llr->data() : nullptr, llr ? llr->size() * 4 : 0);
     if (!ce->mem.ensure(total)) return nullptr;
     char *base = ce->mem.as<char>();
@@ -325,6 +387,7 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
     dc.R = (const uint32_t *)(base + segs[iR].off);
     dc.pivrow = (const int *)(base + segs[iPiv].off);
     dc.Lp = (const uint32_t *)(base + segs[iL].off);
+    dc.dg = dg.empty() ? nullptr : (const uint32_t *)(base + segs[iDg].off);
     dc.llr = (const float *)(base + segs[iLlr].off);
     dc.nCr = rt().nCr.as<unsigned long long>();
     dc.lds_words = 0;
@@ -333,14 +396,14 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
         !hip_check(hipMemcpy(ce->dcode.p, &dc, sizeof(DevCode), hipMemcpyHostToDevice), "hipMemcpy DevCode"))
         return nullptr;
     // the search kernel stages the column words, offsets and signatures of a
-    // small code in LDS, and the degeneracy data (pivot rows, RREF rows) too
-    // when they also fit
+    // small code in LDS, and the degeneracy data (degeneracy words, or pivot
+    // rows and RREF rows) too when they also fit
     DevCode ds = dc;
     const long long cw = (long long)(D + 2) * m;
-    const long long post = (family == kCC) ? (long long)m + (long long)rank * M32 : 0;
+    const long long post = (family != kCC) ? 0 : dc.dg ? (long long)m : (long long)m + (long long)rank * M32;
     if (cw + post <= kCodeLdsMaxWords) {
         ds.lds_words = (int)(cw + post);
-        ds.lds_post = post > 0 ? 1 : 0;
+        ds.lds_post = post == 0 ? 0 : dc.dg ? 2 : 1;
     } else if (cw <= kCodeLdsMaxWords) {
         ds.lds_words = (int)cw;
     }
@@ -374,13 +437,14 @@ void remember_code(const std::shared_ptr<CodeEntry> &e) {
 
 // ================================================================= sessions
 struct Workspace {
-    DevMem s_packed, e_packed, lg_packed, s_sig, worklist, counters, dec, declog, tasks, task_arrays;
+    DevMem s_packed, e_packed, e_aux, lg_packed, s_sig, worklist, counters, dec, declog, tasks, task_arrays;
     int shots = 0;
     bool tasks_fresh = false;   // the task table was just allocated and must be cleared once
     bool ensure(const DevCode &c, int shots_, int cap) {
         const size_t S = (size_t)std::max(shots_, 1);
         if (tasks.bytes == 0) tasks_fresh = true;
-        if (!s_packed.ensure(S * c.D * 4) || !e_packed.ensure(S * c.M32 * 4) || !lg_packed.ensure(S * c.NL32 * 4) ||
+        if (!s_packed.ensure(S * c.D * 4) || !e_packed.ensure(S * c.M32 * 4) || !e_aux.ensure(S * 8) ||
+            !lg_packed.ensure(S * c.NL32 * 4) ||
             !s_sig.ensure(S * 4) || !worklist.ensure(S * 4) || !counters.ensure(kCounterBytes) || !dec.ensure(S * c.M32 * 4) ||
             !declog.ensure(S * c.NL32 * 4) || !tasks.ensure(sizeof(TaskSlot) * kTaskSlots) ||
             !task_arrays.ensure((size_t)kTaskSlots * 3 * std::max(cap, 1) * 4))
@@ -390,7 +454,7 @@ struct Workspace {
     }
     size_t bytes_for(const DevCode &c, int shots_, int cap) const {
         const size_t S = (size_t)std::max(shots_, 1);
-        return S * 4 * ((size_t)c.D + 2 * c.M32 + 2 * c.NL32 + 2) + kCounterBytes + sizeof(TaskSlot) * kTaskSlots +
+        return S * 4 * ((size_t)c.D + 2 * c.M32 + 2 * c.NL32 + 4) + kCounterBytes + sizeof(TaskSlot) * kTaskSlots +
                (size_t)kTaskSlots * 3 * std::max(cap, 1) * 4;
     }
 };
@@ -668,6 +732,7 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     c.logicals = d_logicals;
     c.s_packed = s->ws.s_packed.as<uint32_t>();
     c.e_packed = s->ws.e_packed.as<uint32_t>();
+    c.e_aux = s->ws.e_aux.as<uint32_t>();
     c.lg_packed = s->ws.lg_packed.as<uint32_t>();
     c.s_sig = s->ws.s_sig.as<uint32_t>();
     c.worklist = s->ws.worklist.as<int>();
@@ -697,9 +762,15 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     // tuning knobs are read at every decode, so the autotuner can vary them on a live session
     c.rmax = g_tuning.rmax;
     c.claim_batch = std::max(1, std::min((s->ex.claim_batch > 0) ? s->ex.claim_batch : g_tuning.claim_batch, kMaxBatch));
+    // shots whose first iteration is small are searched one per lane by the
+    // triage kernel (fused engine, first-hit kinds, small codes); with errors
+    // as input the degeneracy words must exist
+    c.triage_search = (s->ex.engine != GRAND_ENGINE_PLAIN && s->ex.triage_search >= 0 && s->search != kML &&
+                       c.lscores == nullptr && code.D <= kLaneMaxD && (long long)code.m * code.D <= kLaneMaxMD &&
+                       !(s->family == kCC && c.input_errors && code.dg == nullptr)) ? 1 : 0;
     // one claimed shot in the search kernel's LDS: id, signature, syndrome words,
     // then the packed errors (cc with errors) or the observed logicals (cln)
-    c.stage_words = 2 + code.D + (c.input_errors ? code.M32 : 0) + (s->family == kCLN ? code.NL32 : 0);
+    c.stage_words = 2 + code.D + (c.input_errors ? 2 + code.M32 : 0) + (s->family == kCLN ? code.NL32 : 0);
 
     // The task table is cleared once when allocated: every launch leaves all
     // slots idle (each owner waits for its chunks and frees its slot).  The
@@ -1053,6 +1124,24 @@ size_t grand_session_footprint(GrandSession *s, int num_shots) {
 void grand_session_destroy(GrandSession *s) {
     std::lock_guard<std::recursive_mutex> lock(rt().mu);
     delete s;
+}
+
+// Degeneracy words of a CSS pair (host only; the same words the decoder uses).
+int grand_degeneracy_words(const int *H, const int *other, int n, int m, unsigned *words) {
+    if (!H || !other || n < 1 || m < 1) return -1;
+    const int M32 = (m + 31) / 32;
+    std::vector<uint32_t> Hrow((size_t)n * M32, 0), rows((size_t)n * M32, 0);
+    for (int r = 0; r < n; ++r)
+        for (int c = 0; c < m; ++c) {
+            if (H[(size_t)r * m + c]) Hrow[(size_t)r * M32 + (c >> 5)] |= 1u << (c & 31);
+            if (other[(size_t)r * m + c]) rows[(size_t)r * M32 + (c >> 5)] |= 1u << (c & 31);
+        }
+    std::vector<uint32_t> R, dg;
+    std::vector<int> piv;
+    grand::gf2_rref(std::move(rows), n, m, R, piv);
+    const int t = grand::degeneracy_words(Hrow, R, piv, n, m, dg);
+    if (t >= 0 && words) for (int c = 0; c < m; ++c) words[c] = dg[c];
+    return t;
 }
 
 // Compute units and peak engine clock of the current device (roofline inputs).

@@ -244,6 +244,9 @@ __device__ __forceinline__ const int LDS *code_lds_pivrow(const DevCode &code) {
 __device__ __forceinline__ const uint32_t LDS *code_lds_R(const DevCode &code) {
     return code_lds() + (size_t)(code.D + 3) * code.m;
 }
+__device__ __forceinline__ const uint32_t LDS *code_lds_dg(const DevCode &code) {
+    return code_lds() + (size_t)(code.D + 2) * code.m;
+}
 // word d of column c
 __device__ __forceinline__ uint32_t col_word(const DevCode &code, int c, int d) {
     return code.lds_words ? code_lds()[(size_t)d * code.m + c] : code.Hcm[(size_t)c * code.D + d];
@@ -1014,6 +1017,40 @@ __device__ void write_result(GrandDecoderResult *dst, int found, int fw, int ow,
     if (lane < 18) reinterpret_cast<uint32_t *>(dst)[lane] = v;
 }
 
+// e ^ e_hat (words dl0, dl1 of lanes lane and lane + 64) is in the row space of
+// the other matrix iff it equals the XOR of the RREF rows whose pivot bit it
+// has set: in reduced echelon form a pivot column is zero outside its own row,
+// so adding rows never changes which other pivots are set.  The residue is
+// sparse; its set bits are looked up in pivrow and the selected rows XORed
+// (independent loads, no chain through the rank).
+__device__ __attribute__((noinline)) bool rref_degenerate(const DevCode &code, uint32_t dl0, uint32_t dl1) {
+    const int lane = lane_id();
+    const int M32 = code.M32;
+    uint32_t acc[2] = {0u, 0u};
+    uint32_t pend[2] = {dl0, dl1};
+    for (;;) {
+        int c = -1;
+        if (pend[0]) { c = lane * 32 + __ffs(pend[0]) - 1; pend[0] &= pend[0] - 1u; }
+        else if (pend[1]) { c = (lane + 64) * 32 + __ffs(pend[1]) - 1; pend[1] &= pend[1] - 1u; }
+        if (ballot(c >= 0) == 0ull) break;
+        const int r = (c >= 0) ? (code.lds_post == 1 ? code_lds_pivrow(code)[c] : code.pivrow[c]) : -1;
+        uint64_t sel = ballot(r >= 0);
+        while (sel) {
+            const int src = __ffsll((long long)sel) - 1;
+            sel &= sel - 1;
+            const int rr = bcast_i(r, src);
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int w2 = lane + 64 * q;
+                if (w2 < M32)
+                    acc[q] ^= code.lds_post == 1 ? code_lds_R(code)[(size_t)rr * M32 + w2]
+                                                 : code.R[(size_t)rr * M32 + w2];
+            }
+        }
+    }
+    return ballot(((dl0 ^ acc[0]) | (dl1 ^ acc[1])) != 0u) == 0ull;
+}
+
 // Post-processing and result record of a searched shot (grand_decoder.cu:1359-1370
 // for the CSS family, :1623-1648 for the DEM family).  The shot's packed errors
 // (cc) or observed logicals (cln) come from its staged copy in LDS (`slot`, laid
@@ -1036,14 +1073,43 @@ __device__ void finalize_shot(const DevCode &code, const DevCall &call, int shot
         if (wd < M32) call.dec[(size_t)shot * M32 + wd] = v;
     }
     int ow = 0, eq = 0, deg = 0;
-    if (call.family == kCC && call.input_errors) {
+    if (call.family == kCC && call.input_errors && slot) {
+        // staged shot: original weight and degeneracy word from the triage
+        const uint32_t LDS *ews = slot + 2 + code.D + 2;
+        ow = (int)uni(slot[2 + code.D]);
+        if (o.found) {
+            // e_hat == e iff both have the same weight and every decoded column is set in e
+            bool same = (ow == o.fw);
+            uint32_t gh = 0;
+#pragma unroll
+            for (int l = 0; l < kMaxWin; ++l) {
+                if (l < o.fw) {
+                    const int c = o.win[l];
+                    same = same && ((ews[c >> 5] >> (c & 31)) & 1u);
+                    if (code.dg) gh ^= (code.lds_post == 2) ? code_lds_dg(code)[c] : code.dg[c];
+                }
+            }
+            eq = same ? 1 : 0;
+            if (code.dg) {
+                deg = (uni(slot[2 + code.D + 1]) == uni(gh)) ? 1 : 0;
+            } else {
+                uint32_t ew[2] = {0u, 0u};
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const int wd = lane + 64 * q;
+                    if (wd < M32) ew[q] = ews[wd];
+                }
+                deg = rref_degenerate(code, ew[0] ^ dw[0], ew[1] ^ dw[1]) ? 1 : 0;
+            }
+        }
+    } else if (call.family == kCC && call.input_errors) {
         uint32_t ew[2] = {0u, 0u};
         int pc = 0;
         bool same = true;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const int wd = lane + 64 * q;
-            if (wd < M32) ew[q] = slot ? slot[2 + code.D + wd] : call.e_packed[(size_t)shot * M32 + wd];
+            if (wd < M32) ew[q] = call.e_packed[(size_t)shot * M32 + wd];
             pc += __popc(ew[q]);
             same = same && (ew[q] == dw[q]);
         }
@@ -1052,36 +1118,7 @@ __device__ void finalize_shot(const DevCode &code, const DevCall &call, int shot
         ow = pc;
         if (o.found) {
             eq = ballot(!same) == 0ull;
-            // e ^ e_hat is in the row space of the other matrix iff it equals the
-            // XOR of the RREF rows whose pivot bit it has set: in reduced echelon
-            // form a pivot column is zero outside its own row, so adding rows never
-            // changes which other pivots are set.  The residue is sparse; its set
-            // bits are looked up in pivrow and the selected rows XORed (independent
-            // loads, no chain through the rank).
-            const uint32_t dl[2] = {ew[0] ^ dw[0], ew[1] ^ dw[1]};
-            uint32_t acc[2] = {0u, 0u};
-            uint32_t pend[2] = {dl[0], dl[1]};
-            for (;;) {
-                int c = -1;
-                if (pend[0]) { c = lane * 32 + __ffs(pend[0]) - 1; pend[0] &= pend[0] - 1u; }
-                else if (pend[1]) { c = (lane + 64) * 32 + __ffs(pend[1]) - 1; pend[1] &= pend[1] - 1u; }
-                if (ballot(c >= 0) == 0ull) break;
-                const int r = (c >= 0) ? (code.lds_post ? code_lds_pivrow(code)[c] : code.pivrow[c]) : -1;
-                uint64_t sel = ballot(r >= 0);
-                while (sel) {
-                    const int src = __ffsll((long long)sel) - 1;
-                    sel &= sel - 1;
-                    const int rr = bcast_i(r, src);
-#pragma unroll
-                    for (int q = 0; q < 2; ++q) {
-                        const int w2 = lane + 64 * q;
-                        if (w2 < M32)
-                            acc[q] ^= code.lds_post ? code_lds_R(code)[(size_t)rr * M32 + w2]
-                                                    : code.R[(size_t)rr * M32 + w2];
-                    }
-                }
-            }
-            deg = ballot(((dl[0] ^ acc[0]) | (dl[1] ^ acc[1])) != 0u) == 0ull;
+            deg = rref_degenerate(code, ew[0] ^ dw[0], ew[1] ^ dw[1]) ? 1 : 0;
         }
     } else if (call.family == kCLN) {
         const int NL32 = code.NL32;
@@ -1226,7 +1263,8 @@ __device__ bool learned_pass(const DevCode &code, const DevCall &call, int shot,
 // The whole per-shot pipeline in one wave (grand_decoder.cu:1284-1343 and the
 // equivalent loops of the other four drivers).
 // One claimed shot, staged in LDS by stage_claim: slot = [shot, signature,
-// syndrome words, error words or observed logicals].
+// syndrome words (D), then original weight, degeneracy word and error words
+// (M32) for cc with errors, or the observed logicals (NL32) for cln].
 __device__ void decode_shot_wave(const DevCode &code, const DevCall &call, const uint32_t LDS *slot, const WaveLds &L) {
     ShotOutcome o;
     o.t0 = now_ticks();
@@ -1289,7 +1327,7 @@ __device__ void stage_claim(const DevCode &code, const DevCall &call, unsigned p
     const int per = SW - 1;                        // loaded words per shot: signature, syndrome, errors or logicals
     const int myshot = (lane < nb) ? call.worklist[p0 + (unsigned)kClaimWords * (unsigned)lane] : 0;
     if (lane < nb) L.stage[lane * SW] = (uint32_t)myshot;
-    const int E = call.input_errors ? code.M32 : 0;
+    const int E = call.input_errors ? 2 + code.M32 : 0;
     const int tot = nb * per;
     for (int k0 = 0; k0 < tot; k0 += 256) {
         uint32_t v[4];
@@ -1304,7 +1342,8 @@ __device__ void stage_claim(const DevCode &code, const DevCall &call, unsigned p
             if (k < tot) {
                 if (f == 0) x = call.s_sig[shot];
                 else if (f <= code.D) x = call.s_packed[(size_t)shot * code.D + (f - 1)];
-                else if (f <= code.D + E) x = call.e_packed[(size_t)shot * code.M32 + (f - 1 - code.D)];
+                else if (f <= code.D + 2 && E > 0) x = call.e_aux[2 * (size_t)shot + (f - 1 - code.D)];
+                else if (f <= code.D + E) x = call.e_packed[(size_t)shot * code.M32 + (f - 3 - code.D)];
                 else x = call.lg_packed[(size_t)shot * code.NL32 + (f - 1 - code.D - E)];
             }
             v[q] = x;
@@ -1332,6 +1371,198 @@ __device__ __forceinline__ void write_zero_result(GrandDecoderResult *dst, int o
 // Stages a 64-shot x 64-column tile of int32 rows through LDS (coalesced loads,
 // one row per step) and returns this lane's 64 values packed into two words;
 // `bad` collects any value other than 0/1.
+// ------------------------------------------------------------ lane search
+// A shot searched by its own lane in the triage kernel: the first iteration of
+// decode_shot_wave (threshold max_soma) when its active list has at most
+// kLaneMax positions and the leading weight classes of its list are of weight
+// <= 3 with at most kLaneBudget patterns in all.  At threshold max_soma a
+// column is active iff its support lies inside supp(s) (score = |supp(h_c) &
+// supp(s)| + max_soma - colw[c], grand_decoder.cu:406-416, :495-504).  Classes
+// are enumerated in the order of the wave path (lexicographic within a class,
+// classes in list order), each candidate filtered by its signature and checked
+// exactly, so the hit, its count and its columns are the wave path's.  Returns
+// false, having written nothing, when the shot needs the search kernel (list
+// too long, a class too large, a cap of the DEM family, or no match in the
+// first iteration).
+__device__ __forceinline__ unsigned lane_ncr(int n, int k) {
+    if (k < 0 || k > n) return 0u;
+    unsigned r = 1u;
+    for (int i = 1; i <= k; ++i) r = r * (unsigned)(n - k + i) / (unsigned)i;
+    return r;
+}
+
+template <int D>
+__device__ __forceinline__ bool lane_exact(const DevCode &code, const uint32_t (&s)[kMaxWords], int a, int b, int c,
+                                           int w) {
+    uint32_t bad = 0u;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        uint32_t x = s[d] ^ code.Hcm[(size_t)a * D + d];
+        if (w > 1) x ^= code.Hcm[(size_t)b * D + d];
+        if (w > 2) x ^= code.Hcm[(size_t)c * D + d];
+        bad |= x;
+    }
+    return bad == 0u;
+}
+
+template <int D>
+__device__ bool lane_search(const DevCode &code, const DevCall &call, int shot, const uint32_t (&s)[kMaxWords],
+                            uint32_t sg, int ow, uint32_t dgv, int LDS *LA, uint32_t LDS *LS,
+                            unsigned long long t0) {
+    const int lane = lane_id();
+    const int m = code.m;
+    int tam = 0;
+    // eight columns per step (wave-uniform loads, issued together)
+    int cb = 0;
+    for (; cb + 8 <= m; cb += 8) {
+        uint32_t out[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) out[j] = (code.off[cb + j] == kNegScore) ? 1u : 0u;
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) out[j] |= code.Hcw[(size_t)d * m + cb + j] & ~s[d];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (out[j] == 0u) {
+                if (tam < kLaneMax) { LA[tam * 64 + lane] = cb + j; LS[tam * 64 + lane] = code.sig[cb + j]; }
+                ++tam;
+            }
+    }
+    for (int c = cb; c < m; ++c) {
+        uint32_t out = (code.off[c] == kNegScore) ? 1u : 0u;
+#pragma unroll
+        for (int d = 0; d < D; ++d) out |= code.Hcw[(size_t)d * m + c] & ~s[d];
+        if (out == 0u) {
+            if (tam < kLaneMax) { LA[tam * 64 + lane] = c; LS[tam * 64 + lane] = code.sig[c]; }
+            ++tam;
+        }
+    }
+    if (tam == 0 || tam > kLaneMax || tam > call.cap) return false;
+    if (call.family == kCLN && tam > call.max_size) return false;   // tam falls back to 0: second iteration
+    const int br = branch_of(call, tam);
+    if (br == 0) return false;
+    // the weight list of decode_shot_wave, walked class by class
+    if (call.kind != kUniform && call.n_masks <= 0) return false;
+    int mi = call.n_masks - 1;
+    if (call.kind != kUniform)
+        for (int i = 0; i < call.n_masks; ++i) if (tam <= call.mask_tam_upper[i]) { mi = i; break; }
+    const int nslots = (call.kind == kUniform) ? call.W : call.num_phases * 5;
+    unsigned cum = 0, wsum = 0;
+    int hw = 0, ha = 0, hb = 0, hc = 0;
+    unsigned hidx = 0;
+    bool giveup = false;
+    for (int q = 0; q < nslots && hw == 0 && !giveup; ++q) {
+        int w;
+        if (call.kind == kUniform) {
+            w = q + 1;
+        } else {
+            const int ph = call.mask_order[mi][q / 5], j = q % 5;
+            w = (ph == 0) ? ((j < 5 && j + 1 <= tam && j + 1 <= kMaxW) ? j + 1 : 0)
+                          : (ph == 1 && j == 0 && tam >= 6) ? 6 : 0;
+        }
+        if (w == 0) continue;
+        const unsigned N = lane_ncr(tam, w);
+        if (N == 0u) continue;
+        if (w > 3 || cum + N > (unsigned)kLaneBudget) { giveup = true; break; }
+        unsigned r = 0;
+        if (w == 1) {
+            for (int a = 0; a < tam && hw == 0; ++a, ++r)
+                if (LS[a * 64 + lane] == sg && lane_exact<D>(code, s, LA[a * 64 + lane], 0, 0, 1)) {
+                    hw = 1; ha = a; hidx = cum + r;
+                }
+        } else if (w == 2) {
+            for (int a = 0; a + 1 < tam && hw == 0; ++a) {
+                const uint32_t ta = sg ^ LS[a * 64 + lane];
+                for (int b = a + 1; b < tam && hw == 0; ++b, ++r)
+                    if (LS[b * 64 + lane] == ta &&
+                        lane_exact<D>(code, s, LA[a * 64 + lane], LA[b * 64 + lane], 0, 2)) {
+                        hw = 2; ha = a; hb = b; hidx = cum + r;
+                    }
+            }
+        } else {
+            for (int a = 0; a + 2 < tam && hw == 0; ++a) {
+                const uint32_t ta = sg ^ LS[a * 64 + lane];
+                for (int b = a + 1; b + 1 < tam && hw == 0; ++b) {
+                    const uint32_t tb = ta ^ LS[b * 64 + lane];
+                    for (int c = b + 1; c < tam && hw == 0; ++c, ++r)
+                        if (LS[c * 64 + lane] == tb &&
+                            lane_exact<D>(code, s, LA[a * 64 + lane], LA[b * 64 + lane], LA[c * 64 + lane], 3)) {
+                            hw = 3; ha = a; hb = b; hc = c; hidx = cum + r;
+                        }
+                }
+            }
+        }
+        if (hw == 0) { cum += N; wsum += N * (unsigned)(w + 1); }
+    }
+    if (hw == 0) return false;
+    const unsigned long long t1 = now_ticks();
+    const int c0 = LA[ha * 64 + lane];
+    const int c1 = hw > 1 ? LA[hb * 64 + lane] : c0;
+    const int c2 = hw > 2 ? LA[hc * 64 + lane] : c0;
+    const unsigned count = hidx + 1u;
+    wsum += (hidx + 1u - cum) * (unsigned)(hw + 1);
+    const int M32 = code.M32;
+    for (int wd = 0; wd < M32; ++wd) {
+        uint32_t v = 0u;
+        if ((c0 >> 5) == wd) v |= 1u << (c0 & 31);
+        if (hw > 1 && (c1 >> 5) == wd) v |= 1u << (c1 & 31);
+        if (hw > 2 && (c2 >> 5) == wd) v |= 1u << (c2 & 31);
+        call.dec[(size_t)shot * M32 + wd] = v;
+    }
+    int eq = 0, deg = 0;
+    if (call.family == kCC && call.input_errors) {
+        const uint32_t *e = call.e_packed + (size_t)shot * M32;
+        bool all = (ow == hw) && ((e[c0 >> 5] >> (c0 & 31)) & 1u);
+        if (hw > 1) all = all && ((e[c1 >> 5] >> (c1 & 31)) & 1u);
+        if (hw > 2) all = all && ((e[c2 >> 5] >> (c2 & 31)) & 1u);
+        eq = all ? 1 : 0;
+        uint32_t gh = code.dg[c0];
+        if (hw > 1) gh ^= code.dg[c1];
+        if (hw > 2) gh ^= code.dg[c2];
+        deg = (gh == dgv) ? 1 : 0;
+    } else if (call.family == kCLN) {
+        const int NL32 = code.NL32;
+        const int nlw = (code.NL + 31) / 32;
+        bool same = true;
+        for (int wd = 0; wd < nlw; ++wd) {
+            uint32_t le = code.Lp[(size_t)c0 * NL32 + wd];
+            if (hw > 1) le ^= code.Lp[(size_t)c1 * NL32 + wd];
+            if (hw > 2) le ^= code.Lp[(size_t)c2 * NL32 + wd];
+            call.declog[(size_t)shot * NL32 + wd] = le;
+            same = same && (le == call.lg_packed[(size_t)shot * NL32 + wd]);
+        }
+        deg = same ? 1 : 0;
+    }
+    const double ts = (double)(t1 - t0) * call.ms_per_tick;
+    const uint64_t tsb = (uint64_t)__double_as_longlong(ts);
+    uint32_t *p = reinterpret_cast<uint32_t *>(&call.results[shot]);
+    p[0] = 1u; p[1] = (uint32_t)hw; p[2] = (uint32_t)ow; p[3] = (uint32_t)deg; p[4] = (uint32_t)eq;
+    p[5] = (uint32_t)tam; p[6] = count; p[7] = 0u; p[8] = (uint32_t)((br > 0) ? 0 : -1); p[9] = 1u;
+    p[10] = 0u; p[11] = 0u; p[12] = (uint32_t)tsb; p[13] = (uint32_t)(tsb >> 32); p[14] = 0u; p[15] = 0u;
+    p[16] = (uint32_t)tsb; p[17] = (uint32_t)(tsb >> 32);
+    if (call.stats) {
+        call.stats[3 * (size_t)shot] = t0;
+        call.stats[3 * (size_t)shot + 1] = t1;
+        call.stats[3 * (size_t)shot + 2] = wsum;
+    }
+    return true;
+}
+
+__device__ __forceinline__ bool lane_search_any(const DevCode &code, const DevCall &call, int shot,
+                                             const uint32_t (&s)[kMaxWords], uint32_t sg, int ow, uint32_t dgv,
+                                             int LDS *LA, uint32_t LDS *LS, unsigned long long t0) {
+    switch (code.D) {
+        case 1: return lane_search<1>(code, call, shot, s, sg, ow, dgv, LA, LS, t0);
+        case 2: return lane_search<2>(code, call, shot, s, sg, ow, dgv, LA, LS, t0);
+        case 3: return lane_search<3>(code, call, shot, s, sg, ow, dgv, LA, LS, t0);
+        case 4: return lane_search<4>(code, call, shot, s, sg, ow, dgv, LA, LS, t0);
+        case 5: return lane_search<5>(code, call, shot, s, sg, ow, dgv, LA, LS, t0);
+        case 6: return lane_search<6>(code, call, shot, s, sg, ow, dgv, LA, LS, t0);
+        default: return false;
+    }
+}
+
 __device__ __forceinline__ void pack_tile(const int *rows, int stride, int base, int cnt, int c0, int cols,
                                           uint32_t LDS *tile, uint32_t &w0, uint32_t &w1, unsigned &bad) {
     const int lane = lane_id();
@@ -1376,6 +1607,8 @@ __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ 
     const DevCall &call = *callp;
     const int lane = lane_id();
     uint32_t LDS *tile = lds_base();   // [64][65]
+    int LDS *LA = (int LDS *)(tile + 64 * 65);          // lane search: [kLaneMax][64] positions
+    uint32_t LDS *LS = tile + 64 * 65 + kLaneMax * 64;  //              [kLaneMax][64] signatures
     const int groups = (call.num_shots + 63) / 64;
     unsigned bad = 0;
     for (int g = blockIdx.x; g < groups; g += gridDim.x) {
@@ -1386,7 +1619,7 @@ __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ 
         uint32_t s[kMaxWords];
 #pragma unroll
         for (int d = 0; d < kMaxWords; ++d) s[d] = 0u;
-        uint32_t sg = 0;
+        uint32_t sg = 0, dgv = 0;
         int ow = 0;
         if (call.input_errors) {
             for (int c0 = 0; c0 < code.m; c0 += 64) {
@@ -1405,6 +1638,7 @@ __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ 
                         const int c = c0 + 32 * h + __ffs(bits) - 1;
                         bits &= bits - 1u;
                         sg ^= code.sig[c];
+                        if (code.dg) dgv ^= code.dg[c];
 #pragma unroll
                         for (int d = 0; d < kMaxWords; ++d)
                             if (d < code.D) s[d] ^= code.Hcm[(size_t)c * code.D + d];
@@ -1447,11 +1681,21 @@ __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ 
 #pragma unroll
         for (int d = 0; d < kMaxWords; ++d) any |= s[d];
         const bool nz = valid && any != 0u;
+        bool solved = false;
+        if (call.triage_search && ballot(nz)) {
+            const unsigned long long t0 = now_ticks();
+            if (nz) solved = lane_search_any(code, call, shot, s, sg, ow, dgv, LA, LS, t0);
+        }
+        const bool queued = nz && !solved;
         if (valid) {
 #pragma unroll
             for (int d = 0; d < kMaxWords; ++d)
                 if (d < code.D) call.s_packed[(size_t)shot * code.D + d] = s[d];
             call.s_sig[shot] = sg;
+            if (call.input_errors && queued) {
+                call.e_aux[2 * (size_t)shot] = (uint32_t)ow;
+                call.e_aux[2 * (size_t)shot + 1] = dgv;
+            }
         }
         if (valid && !nz) {
             write_zero_result(&call.results[shot], (call.family == kCC) ? ow : 0);
@@ -1464,12 +1708,12 @@ __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ 
                 call.stats[3 * (size_t)shot + 2] = 0;
             }
         }
-        const uint64_t mask = ballot(nz);
+        const uint64_t mask = ballot(queued);
         if (mask) {
             unsigned pos = 0;
             if (lane == 0) pos = atomicAdd(&call.counters[0], (unsigned)__popcll(mask));
             pos = uni(pos);
-            if (nz) call.worklist[pos + lanes_below(mask)] = shot;
+            if (queued) call.worklist[pos + lanes_below(mask)] = shot;
         }
     }
     if (ballot(bad != 0u) && lane == 0) atomicOr(&call.counters[2], kErrNonBinary);
@@ -1491,7 +1735,11 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
             dst[nw + i] = (uint32_t)code.off[i];
             dst[nw + code.m + i] = code.sig[i];
         }
-        if (code.lds_post) {
+        if (code.lds_post == 2) {
+            uint32_t LDS *post = dst + nw + 2 * code.m;
+#pragma unroll 4
+            for (int i = lane_id(); i < code.m; i += 64) post[i] = code.dg[i];
+        } else if (code.lds_post == 1) {
             uint32_t LDS *post = dst + nw + 2 * code.m;
             const int nr = code.rank * code.M32;
 #pragma unroll 4
@@ -1829,27 +2077,28 @@ using grand::DevCall;
 using grand::DevCode;
 
 // LDS of one wave without the staged code and claimed shots (carve_lds).
-size_t grand_fused_lds_bytes(int cap, int ml) {
+GRAND_INTERNAL size_t grand_fused_lds_bytes(int cap, int ml) {
     return sizeof(uint32_t) * ((size_t)grand::kLdsCodeOffset + 2 * (size_t)grand::ht_capacity(cap) +
                                (size_t)grand::bitmap_words(cap) + (size_t)cap * (ml ? 6 : 2));
 }
 
-hipError_t grand_launch_prep(const DevCall *hcall, DevCall *dcall, unsigned *counters, hipStream_t st) {
+GRAND_INTERNAL hipError_t grand_launch_prep(const DevCall *hcall, DevCall *dcall, unsigned *counters, hipStream_t st) {
     hipLaunchKernelGGL(grand::prep_kernel, dim3(1), dim3(256), 0, st, *hcall, dcall, counters);
     return hipGetLastError();
 }
 
-hipError_t grand_launch_fill(unsigned *p, unsigned v, int n, hipStream_t st) {
+GRAND_INTERNAL hipError_t grand_launch_fill(unsigned *p, unsigned v, int n, hipStream_t st) {
     hipLaunchKernelGGL(grand::fill_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, v, n);
     return hipGetLastError();
 }
 
-hipError_t grand_launch_triage(const DevCode *code, const DevCall *call, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(grand::triage_kernel, dim3(grid), dim3(64), sizeof(uint32_t) * 64 * 65, st, code, call);
+GRAND_INTERNAL hipError_t grand_launch_triage(const DevCode *code, const DevCall *call, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(grand::triage_kernel, dim3(grid), dim3(64),
+                       sizeof(uint32_t) * (64 * 65 + 2 * grand::kLaneMax * 64), st, code, call);
     return hipGetLastError();
 }
 
-hipError_t grand_launch_fused(const DevCode *code, const DevCall *call, const DevCall *hcall, int code_lds_words,
+GRAND_INTERNAL hipError_t grand_launch_fused(const DevCode *code, const DevCall *call, const DevCall *hcall, int code_lds_words,
                               int grid, hipStream_t st) {
     const size_t lds = grand_fused_lds_bytes(hcall->cap, hcall->kind == grand::kML || hcall->lscores != nullptr) +
                        sizeof(uint32_t) * ((size_t)code_lds_words + (size_t)hcall->claim_batch * hcall->stage_words);
@@ -1857,7 +2106,7 @@ hipError_t grand_launch_fused(const DevCode *code, const DevCall *call, const De
     return hipGetLastError();
 }
 
-hipError_t grand_launch_expand(const uint32_t *packed, int words, int cols, int rows, int *out, hipStream_t st) {
+GRAND_INTERNAL hipError_t grand_launch_expand(const uint32_t *packed, int words, int cols, int rows, int *out, hipStream_t st) {
     const size_t total = (size_t)rows * cols;
     int grid = (int)((total + 255) / 256);
     if (grid > 65536) grid = 65536;
@@ -1866,10 +2115,10 @@ hipError_t grand_launch_expand(const uint32_t *packed, int words, int cols, int 
     return hipGetLastError();
 }
 
-size_t grand_plain_state_bytes() { return sizeof(grand::PlainState); }
-size_t grand_plain_record_bytes() { return sizeof(grand::MlRecord); }
+GRAND_INTERNAL size_t grand_plain_state_bytes() { return sizeof(grand::PlainState); }
+GRAND_INTERNAL size_t grand_plain_record_bytes() { return sizeof(grand::MlRecord); }
 
-hipError_t grand_launch_plain_active(const DevCode *code, const DevCall *call, int cap, int shot, int thr,
+GRAND_INTERNAL hipError_t grand_launch_plain_active(const DevCode *code, const DevCall *call, int cap, int shot, int thr,
                                      int sort_tam, void *scratch, void *state, hipStream_t st) {
     int *gA = (int *)scratch;
     uint32_t *gsig = (uint32_t *)(gA + cap);
@@ -1882,7 +2131,7 @@ hipError_t grand_launch_plain_active(const DevCode *code, const DevCall *call, i
     return hipGetLastError();
 }
 
-hipError_t grand_launch_plain_class(const DevCode *code, const DevCall *call, int cap, int shot, int w, int tam, int R,
+GRAND_INTERNAL hipError_t grand_launch_plain_class(const DevCode *code, const DevCall *call, int cap, int shot, int w, int tam, int R,
                                     unsigned long long nchunks, int grid, int ml, void *scratch, void *state,
                                     void *records, hipStream_t st) {
     int *gA = (int *)scratch;
@@ -1897,13 +2146,13 @@ hipError_t grand_launch_plain_class(const DevCode *code, const DevCall *call, in
     return hipGetLastError();
 }
 
-hipError_t grand_launch_plain_ml_reduce(void *records, int nrec, void *out, hipStream_t st) {
+GRAND_INTERNAL hipError_t grand_launch_plain_ml_reduce(void *records, int nrec, void *out, hipStream_t st) {
     hipLaunchKernelGGL(grand::plain_ml_reduce_kernel, dim3(1), dim3(64), 0, st, (const grand::MlRecord *)records, nrec,
                        (grand::MlRecord *)out);
     return hipGetLastError();
 }
 
-hipError_t grand_launch_plain_finalize(const DevCode *code, const DevCall *call, int shot,
+GRAND_INTERNAL hipError_t grand_launch_plain_finalize(const DevCode *code, const DevCall *call, int shot,
                                        const grand::ShotOutcome *o, void *scratch, unsigned long long hit_rank,
                                        hipStream_t st) {
     hipLaunchKernelGGL(grand::plain_finalize_kernel, dim3(1), dim3(64), 0, st, code, call, shot, *o,

@@ -4,6 +4,10 @@
 #include <stdint.h>
 #include "../../include/grand_decoder.h"
 
+// Symbols shared between the library's own translation units only (kernel
+// launchers); they stay out of the shared library's exported interface.
+#define GRAND_INTERNAL __attribute__((visibility("hidden")))
+
 namespace grand {
 
 constexpr int kMaxW = GRAND_MAX_W;   // largest enumerated weight of the plain search
@@ -37,13 +41,15 @@ struct DevCode {
     const uint32_t *R;                // [rank][M32] RREF of the other CSS matrix (cc)
     const int *pivrow;                // [m] RREF row whose pivot is this column, -1 for other columns
     const uint32_t *Lp;               // [m][NL32] logical flips per mechanism (cln)
+    const uint32_t *dg;               // [m] degeneracy words (cc), or null when more than 32 are needed
     const float *llr;                 // [m] (ML)
     const unsigned long long *nCr;    // [(kMaxTam + 1) * kNcrStride]
     // Small codes are staged in the search kernel's LDS.  The search kernel gets
     // its own copy of this descriptor with lds_words > 0; every other kernel's
     // copy has 0 and reads global memory.  Staged words, in order: column words
-    // [D][m], score offsets [m], signatures [m], then (lds_post = 1, cc) the RREF
-    // row of each pivot column [m] and the RREF rows [rank][M32].
+    // [D][m], score offsets [m], signatures [m], then for cc either (lds_post = 2)
+    // the degeneracy words [m] or (lds_post = 1) the RREF row of each pivot
+    // column [m] and the RREF rows [rank][M32].
     int lds_words;
     int lds_post;
 };
@@ -54,6 +60,11 @@ struct DevCode {
 constexpr int kTaskSlots = 128;
 constexpr int kCodeLdsMaxWords = 4096;   // codes up to 16 KB of staged words (DevCode::lds_words) go to LDS
 constexpr int kMaxBatch = 8;             // most shots per worklist claim
+// Search of a shot by its own lane in the triage kernel (first iteration only)
+constexpr int kLaneMax = 12;             // active-list entries a lane keeps
+constexpr int kLaneBudget = 300;         // patterns a lane may test
+constexpr int kLaneMaxD = 6;             // codes with at most 6 syndrome words ...
+constexpr int kLaneMaxMD = 2048;         // ... and m * D at most this (the lane scans every column)
 
 // Claim words of the persistent search kernel: kClaimWords 64-bit words, one
 // per 128-byte line, starting at 32-bit counter index 2 * kClaimBase.
@@ -87,8 +98,10 @@ struct DevCall {
     int cap;                          // active-list capacity held in LDS (<= kMaxTam)
     int rmax;                         // patterns per lane per chunk in the range path
     int claim_batch;                  // shots taken per worklist claim (<= kMaxBatch)
-    int stage_words;                  // search kernel: LDS words per claimed shot (id, signature, syndrome,
-                                      // error words for cc with errors, observed logicals for cln)
+    int triage_search;                // 1: the triage kernel searches small first iterations one shot per lane
+    int stage_words;                  // search kernel: LDS words per claimed shot: id, signature, syndrome,
+                                      // then original weight, degeneracy word and error words (cc with
+                                      // errors) or the observed logicals (cln)
     int num_shots;
     int input_errors;                 // 1: input rows are error patterns (cc), 0: syndromes/detectors
     double ms_per_tick;               // device wall clock period
@@ -96,6 +109,7 @@ struct DevCall {
     const int *logicals;              // [shots][NL] (cln)
     uint32_t *s_packed;               // [shots][D]
     uint32_t *e_packed;               // [shots][M32] (cc with errors)
+    uint32_t *e_aux;                  // [shots][2] original weight and XOR of dg over supp(e) (cc with errors)
     uint32_t *lg_packed;              // [shots][NL32] (cln)
     uint32_t *s_sig;                  // [shots]
     int *worklist;                    // nonzero-syndrome shots

@@ -43,6 +43,7 @@ def test_header_compiles_as_c_and_layout(tmp_path):
 #include "grand_decoder.h"
 #include "grand_session.h"
 #include "grand_server.h"
+#include "grand_mlp.h"
 int main(void) {
   printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(GrandDecoderConfig), sizeof(GrandDecoderResult),
          offsetof(GrandDecoderResult, tam), offsetof(GrandDecoderResult, total_tested_combinations),
@@ -69,6 +70,15 @@ def test_product_exports_every_declared_function():
     assert len(names) >= 18
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_every_exported_function_is_declared():
+    out = subprocess.run(["nm", "-D", "--defined-only", G.DEFAULT_LIB], capture_output=True, text=True,
+                         check=True).stdout
+    exported = sorted({ln.split()[-1] for ln in out.splitlines() if ln.split()[-2:-1] == ["T"] and
+                       ln.split()[-1].startswith("grand_")})
+    declared = set(declared_functions())
+    assert exported and not [n for n in exported if n not in declared]
 
 
 def test_oracle_exports_abi():

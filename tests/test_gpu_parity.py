@@ -136,3 +136,52 @@ def test_shared_classes_cln_deep(oracle, product, split):
     o = T.run_cln(oracle, G.KIND_CLN, H, L, det, lg, cfg=cfg)
     p = T.run_cln(product, G.KIND_CLN, H, L, det, lg, cfg=cfg, ex=ex)
     T.assert_same(o, p, f"shared cln split={split}")
+
+
+# ---------------------------------------------------------------- triage search
+# Shots with a small first iteration are searched one per lane in the triage
+# kernel (fused engine, first-hit kinds, codes of at most 6 syndrome words).
+# Results must be those of the oracle and of the search kernel alone
+# (triage_search = -1), including the cap, hybrid-mode and phase-order paths.
+def _triage_cases(tmp_path):
+    pf = T.phase_file(tmp_path)
+    masks6 = [(5, 0, 1, 2, 1, 0, 0), (15, 1, 0, 2, .3, .7, 0), (300, 0, 1, 2, .6, .4, 0)]
+    pf6 = T.phase_file(tmp_path, masks6, name="phase6.txt")
+    hx72, hz72, e72 = T.cc_workload(72, 3000, 0.006, seed=31)
+    hx144, hz144, e144 = T.cc_workload(144, 4000, "loguniform", seed=32)
+    hx288, hz288, e288 = T.cc_workload(288, 1500, 0.004, seed=33)
+    Hs, Ls, ps, dets, lgs = T.dem_workload(2000, seed=34, dem_seed=5, n_det=72, n_mech=400)
+    cases = [
+        ("cc BB72 errors", T.run_cc, (G.KIND_CC, hx72, hz72), dict(errors=e72, cfg=G.make_config(max_error_weight=3))),
+        ("cc BB72 syndromes", T.run_cc, (G.KIND_CC, hx72, hz72),
+         dict(syndromes=T.codes.syndromes(hz72, e72), cfg=G.make_config(max_error_weight=3))),
+        ("cc BB72 z W1", T.run_cc, (G.KIND_CC, hx72, hz72), dict(errors=e72, etype="z",
+                                                                 cfg=G.make_config(max_error_weight=1))),
+        ("prob_cc BB144", T.run_cc, (G.KIND_PROB_CC, hx144, hz144),
+         dict(errors=e144, cfg=G.make_config(max_error_weight=4), prob_file=pf)),
+        ("prob_cc BB72 phase orders", T.run_cc, (G.KIND_PROB_CC, hx72, hz72),
+         dict(errors=e72, cfg=G.make_config(max_error_weight=6), prob_file=pf6)),
+        ("cc BB288", T.run_cc, (G.KIND_CC, hx288, hz288), dict(errors=e288, cfg=G.make_config(max_error_weight=4))),
+        ("cc BB72 hybrid", T.run_cc, (G.KIND_CC, hx72, hz72),
+         dict(errors=e72, cfg=G.make_config(max_error_weight=4, mode_of_search=1, threshold_for_hybrid=2,
+                                            max_size_active_positions=30))),
+        ("cc BB72 cpu mode", T.run_cc, (G.KIND_CC, hx72, hz72),
+         dict(errors=e72, cfg=G.make_config(max_error_weight=4, mode_of_search=-1))),
+        ("cc BB72 no search", T.run_cc, (G.KIND_CC, hx72, hz72),
+         dict(errors=e72, cfg=G.make_config(max_error_weight=4, mode_of_search=7))),
+        ("cln small DEM", T.run_cln, (G.KIND_CLN, Hs, Ls, dets, lgs),
+         dict(cfg=G.make_config(max_error_weight=3, max_size_active_positions=48))),
+        ("prob_cln small DEM, cap 2", T.run_cln, (G.KIND_PROB_CLN, Hs, Ls, dets, lgs),
+         dict(cfg=G.make_config(max_error_weight=4, max_size_active_positions=2), file=pf)),
+    ]
+    return cases
+
+
+def test_triage_search_matches_oracle_and_search_kernel(oracle, product, tmp_path):
+    for what, runner, args, kw in _triage_cases(tmp_path):
+        o = runner(oracle, *args, **kw)
+        on = runner(product, *args, ex=G.make_ex(triage_search=0), **kw)
+        off = runner(product, *args, ex=G.make_ex(triage_search=-1), **kw)
+        T.assert_same(o, on, what + " (triage search on)")
+        T.assert_same(on, off, what + " (on vs off)")
+        print(what, "found", int(o[1]["found"].sum()), flush=True)
