@@ -96,9 +96,19 @@ def cpu_baseline(hx, hz, errors, prob_file, fused_res):
     return single, multi, bad == 0, o
 
 
-def leg_bb72_cc(lib, oracle_path):
+def tune_leg(lib, run, tune):
+    """Autotunes the knobs on one leg's workload (run() decodes it once and returns its kernel times); returns the
+    chosen configuration, or None when tuning is off.  The caller restores the defaults afterwards."""
+    if not tune:
+        return None
+    from library_s_grand_amd import autotune
+    best, best_ms, table = autotune.tune(lib, run, reps=3, max_rounds=1)
+    return {"chosen": best, "search_ms": best_ms, "configs_measured": len(table)}
+
+
+def leg_bb72_cc(lib, oracle_path, tune=True):
     """BB72 grand_decode_cc, 10,000 shots of X errors (p = 0.005), max_error_weight = 3: device-resident
-    throughput and a bit-exact check of every shot against the oracle."""
+    throughput (knobs tuned on this workload) and a bit-exact check of every shot against the oracle."""
     import torch
     hx, hz = codes.bb(72)
     errors = codes.sample_errors(np.random.default_rng(SEED + 72), 10_000, 0.005, hx.shape[1])
@@ -109,6 +119,12 @@ def leg_bb72_cc(lib, oracle_path):
     d_r = torch.empty(errors.shape[0] * 72, dtype=torch.uint8, device="cuda")
     s.decode(d_e, True, d_r)
     torch.cuda.synchronize()
+
+    def run():
+        s.decode(d_e, True, d_r)
+        return s.last_timing()
+
+    tuning = tune_leg(lib, run, tune)
     t = time.perf_counter()
     for _ in range(10):
         s.decode(d_e, True, d_r)
@@ -117,7 +133,7 @@ def leg_bb72_cc(lib, oracle_path):
     res = results_from_bytes(d_r.cpu().numpy())
     s.close()
     out = {"workload": "BB72 [[72,12,6]] grand_decode_cc, 10000 shots, X errors p=0.005, max_error_weight=3",
-           "shots_per_s": errors.shape[0] / (ms * 1e-3), "ms_per_step": ms}
+           "shots_per_s": errors.shape[0] / (ms * 1e-3), "ms_per_step": ms, "tuning": tuning}
     if oracle_path:
         olib = G.bind(ctypes.CDLL(oracle_path))
         rc, o, _ = G.run_grand_shots_cc_ex(olib, hx, hz, errors, None, cfg, "x", want_decoded=False)
@@ -126,9 +142,10 @@ def leg_bb72_cc(lib, oracle_path):
     return out
 
 
-def leg_dem_ml(lib, oracle_path, shots=1_000_000, sample=2000):
+def leg_dem_ml(lib, oracle_path, shots=1_000_000, sample=2000, tune=True):
     """Synthetic DEM 216 x 3000 grand_decode_ML_cln, max_error_weight = 4, max_size = 64: throughput on
-    `shots` shots sampled on the device, and a bit-exact check against the oracle on the first `sample`."""
+    `shots` shots sampled on the device (knobs tuned on the first 100,000), and a bit-exact check against the
+    oracle on the first `sample`."""
     import torch
     from library_s_grand_amd import learned
     task = learned.DemTask(W=4, max_size=64)
@@ -140,6 +157,12 @@ def leg_dem_ml(lib, oracle_path, shots=1_000_000, sample=2000):
     d_r = torch.empty(shots * 72, dtype=torch.uint8, device="cuda")
     s.decode(det, False, d_r, d_logicals=lg, num_shots=sample)
     torch.cuda.synchronize()
+
+    def run():
+        s.decode(det, False, d_r, d_logicals=lg, num_shots=min(shots, 100_000))
+        return s.last_timing()
+
+    tuning = tune_leg(lib, run, tune)
     t = time.perf_counter()
     s.decode(det, False, d_r, d_logicals=lg)
     torch.cuda.synchronize()
@@ -152,7 +175,7 @@ def leg_dem_ml(lib, oracle_path, shots=1_000_000, sample=2000):
     out = {"workload": "synthetic DEM 216x3000 (column weight 1..4, 6 windows), 12 logicals, grand_decode_ML_cln, "
                        f"{shots} shots, max_error_weight=4, max_size_active_positions=64",
            "shots_per_s": shots / sec, "seconds": sec, "patterns_per_s": cnt.sum() / sec,
-           "mean_patterns": float(cnt.mean())}
+           "mean_patterns": float(cnt.mean()), "tuning": tuning}
     if oracle_path:
         olib = G.bind(ctypes.CDLL(oracle_path))
         olib.oracle_set_threads.argtypes = [ctypes.c_int]
@@ -362,8 +385,10 @@ def main():
         autotune.apply(lib, tuning_defaults)
     if not a.no_legs:
         oracle_path = None if a.no_cpu else os.path.join(ROOT, "oracle", "libgrand_oracle.so")
-        out["bb72_cc"] = leg_bb72_cc(lib, oracle_path)
-        out["dem_ml_cln"] = leg_dem_ml(lib, oracle_path)
+        out["bb72_cc"] = leg_bb72_cc(lib, oracle_path, tune=not a.no_tune)
+        autotune.apply(lib, tuning_defaults)
+        out["dem_ml_cln"] = leg_dem_ml(lib, oracle_path, tune=not a.no_tune)
+        autotune.apply(lib, tuning_defaults)
     if not a.no_serving:
         from library_s_grand_amd import serving
         out["serving"] = serving.measure(rate_hz=2000.0, n_requests=4000)

@@ -768,6 +768,7 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     c.triage_search = (s->ex.engine != GRAND_ENGINE_PLAIN && s->ex.triage_search >= 0 && s->search != kML &&
                        c.lscores == nullptr && code.D <= kLaneMaxD && (long long)code.m * code.D <= kLaneMaxMD &&
                        !(s->family == kCC && c.input_errors && code.dg == nullptr)) ? 1 : 0;
+    c.split_worklist = (s->ex.engine != GRAND_ENGINE_PLAIN) ? 1 : 0;
     // one claimed shot in the search kernel's LDS: id, signature, syndrome words,
     // then the packed errors (cc with errors) or the observed logicals (cln)
     c.stage_words = 2 + code.D + (c.input_errors ? 2 + code.M32 : 0) + (s->family == kCLN ? code.NL32 : 0);

@@ -178,12 +178,14 @@ __device__ __forceinline__ uint32_t LDS *lds_base() { return (uint32_t LDS *)lds
 
 constexpr int kNcrLdsRows = 129;   // n = 0..128
 
-// Signature bitmap kept after the hash table: at least 32 bits per entry, so
-// a prefix whose target signature is absent is rejected by one LDS read in all
-// but a fraction <= 1/32 of cases, without walking a probe chain.
+// Signature bitmap kept after the hash table: 256 bits per entry (at most 2^16
+// bits), so a prefix whose target signature is absent is rejected by one LDS
+// read in all but a fraction <= 1/256 of cases (1/64 at the largest lists),
+// without walking a probe chain; a wave of 64 prefixes then rarely has any
+// lane on the probe path.
 __host__ __device__ __forceinline__ int bitmap_words(int cap) {
     int bits = 1024;
-    while (bits < 32 * cap) bits <<= 1;
+    while (bits < 256 * cap && bits < 65536) bits <<= 1;
     return bits / 32;
 }
 
@@ -400,10 +402,97 @@ struct PrefHit {
     int j;           // prefix offset of the hit within the lane's range
 };
 
+// Probe of a prefix c[0..WP) whose target signature t passed the bitmap: the
+// smallest position x > last in t's probe chain whose pattern matches exactly,
+// or -1.  Out of line, so that none of its work is scheduled into the scan.
+template <int WP>
+__device__ __attribute__((noinline)) int probe_prefix(const DevCode &code, const uint32_t LDS *s, uint32_t t,
+                                                     int last, int tam, const int LDS *act,
+                                                     const unsigned long long LDS *ht, unsigned hm, int p0, int p1,
+                                                     int p2, int p3, int p4, int p5) {
+    const int pc[6] = {p0, p1, p2, p3, p4, p5};
+    int best = -1;
+    for (unsigned pos = t & hm;; pos = (pos + 1u) & hm) {
+        const unsigned long long e = ht[pos];
+        if (e == kHtEmpty) break;
+        const int x = (int)(uint32_t)e;
+        if ((uint32_t)(e >> 32) == t && x > last && x < tam && (best < 0 || x < best)) {
+            int cols[WP + 1];
+#pragma unroll
+            for (int l = 0; l < WP; ++l) cols[l] = act[pc[l]];
+            cols[WP] = act[x];
+            if (exact_match<WP + 1>(code, s, cols, WP + 1)) best = x;
+        }
+    }
+    return best;
+}
+
+// First-hit walk of len prefixes from rank q0: one bitmap read per prefix on
+// the common path, the next position's signature read while the bitmap read
+// is in flight, candidates handed to probe_prefix.  Stops at the lane's first
+// match (lexicographic order of (prefix, x) is rank order).
+template <int WP>
+__device__ __forceinline__ PrefHit scan_prefix_first(const DevCode &code, const uint32_t LDS *s, uint32_t t0,
+                                                     uint64_t q0, int len, int tam, const int LDS *act,
+                                                     const uint32_t LDS *sg, const unsigned long long LDS *ht,
+                                                     unsigned hmask) {
+    PrefHit h{~0ull, 0.0f, -1};
+    if (len <= 0) return h;
+    const int M = tam - 1;
+    int c[WP];
+    const bool small = tam < kNcrLdsRows;
+    if (small) unrank_t<WP>(q0, M, ncr_lds(), c); else unrank_t<WP>(q0, M, code.nCr, c);
+    uint32_t acc[WP];   // acc[l] = t0 ^ sg[c0] ^ ... ^ sg[c_l]
+    uint32_t a = t0;
+#pragma unroll
+    for (int l = 0; l < WP; ++l) { a ^= sg[c[l]]; acc[l] = a; }
+    uint32_t hi = (WP >= 2) ? acc[(WP >= 2) ? WP - 2 : 0] : t0;   // signature up to the second-last element
+    uint32_t t = acc[WP - 1];
+    const unsigned hm = hmask & 0xFFFFu, bmsk = hmask >> 16;
+    const uint32_t LDS *bm = (const uint32_t LDS *)(ht + hm + 1u);
+    for (int j = 0;;) {
+        const int last = c[WP - 1];
+        const uint32_t nxt = sg[last + 1];            // last + 1 <= M = tam - 1
+        const unsigned bi = t & bmsk;
+        if ((bm[bi >> 5] >> (bi & 31u)) & 1u) {
+            const int x = probe_prefix<WP>(code, s, t, last, tam, act, ht, hm, c[0], WP > 1 ? c[WP > 1 ? 1 : 0] : 0,
+                                           WP > 2 ? c[WP > 2 ? 2 : 0] : 0, WP > 3 ? c[WP > 3 ? 3 : 0] : 0,
+                                           WP > 4 ? c[WP > 4 ? 4 : 0] : 0, WP > 5 ? c[WP > 5 ? 5 : 0] : 0);
+            if (x >= 0) {
+                h.rank = small ? rank_with_last<WP>(c, x, tam, ncr_lds()) : rank_with_last<WP>(c, x, tam, code.nCr);
+                h.j = j;
+                break;
+            }
+        }
+        if (++j == len) break;
+        // lexicographic successor among WP-subsets of [0, M)
+        if (last + 1 < M) {
+            c[WP - 1] = last + 1;
+            t = hi ^ nxt;
+        } else {
+            int i = -1;
+#pragma unroll
+            for (int l = 0; l < WP - 1; ++l) if (c[l] < M - WP + l) i = l;
+#pragma unroll
+            for (int l = 0; l < WP; ++l) {
+                if (l == i) c[l] += 1;
+                else if (l > 0 && l > i) c[l] = c[l - 1] + 1;
+            }
+#pragma unroll
+            for (int l = 0; l < WP; ++l)
+                if (l >= i) acc[l] = ((l == 0) ? t0 : acc[(l == 0) ? 0 : l - 1]) ^ sg[c[l]];
+            hi = (WP >= 2) ? acc[(WP >= 2) ? WP - 2 : 0] : t0;
+            t = acc[WP - 1];
+        }
+    }
+    return h;
+}
+
 template <int WP, int MODE>
 __device__ __forceinline__ PrefHit scan_prefix(const DevCode &code, const uint32_t LDS *s, uint32_t t0, uint64_t q0,
                                                int len, int tam, const int LDS *act, const uint32_t LDS *sg,
                                                const float LDS *llr, const unsigned long long LDS *ht, unsigned hmask) {
+    if constexpr (MODE == 0) return scan_prefix_first<WP>(code, s, t0, q0, len, tam, act, sg, ht, hmask);
     PrefHit h{~0ull, 0.0f, -1};
     if (len <= 0) return h;
     const int M = tam - 1;
@@ -1321,11 +1410,17 @@ __device__ void decode_shot_wave(const DevCode &code, const DevCall &call, const
 // Stages the shots of one claim (worklist positions p0, p0 + 8, ...; nb of them)
 // in LDS: the worklist entries first, then every shot's words in one round of
 // loads, so the shots decode without waiting on global memory between them.
-__device__ void stage_claim(const DevCode &code, const DevCall &call, unsigned p0, int nb, const WaveLds &L) {
+__device__ void stage_claim(const DevCode &code, const DevCall &call, unsigned p0, int nb, unsigned heavy,
+                            const WaveLds &L) {
     const int lane = lane_id();
     const int SW = call.stage_words;
     const int per = SW - 1;                        // loaded words per shot: signature, syndrome, errors or logicals
-    const int myshot = (lane < nb) ? call.worklist[p0 + (unsigned)kClaimWords * (unsigned)lane] : 0;
+    int myshot = 0;
+    if (lane < nb) {
+        // positions below `heavy` are the front segment of the worklist, the rest the back segment
+        const unsigned p = p0 + (unsigned)kClaimWords * (unsigned)lane;
+        myshot = call.worklist[p < heavy ? p : (unsigned)call.num_shots - 1u - (p - heavy)];
+    }
     if (lane < nb) L.stage[lane * SW] = (uint32_t)myshot;
     const int E = call.input_errors ? 2 + code.M32 : 0;
     const int tot = nb * per;
@@ -1708,12 +1803,25 @@ __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ 
                 call.stats[3 * (size_t)shot + 2] = 0;
             }
         }
-        const uint64_t mask = ballot(queued);
-        if (mask) {
+        // search kernel: shots whose syndrome needs more than two columns go to
+        // the front of the worklist (their deep classes then overlap with the
+        // rest of the batch), the others to the back; the plain engine reads one list
+        int sw = 0;
+#pragma unroll
+        for (int d = 0; d < kMaxWords; ++d) sw += __popc(s[d]);
+        const bool front = queued && (!call.split_worklist || sw > 2 * code.max_soma);
+        const uint64_t mf = ballot(front), mb = ballot(queued && !front);
+        if (mf) {
             unsigned pos = 0;
-            if (lane == 0) pos = atomicAdd(&call.counters[0], (unsigned)__popcll(mask));
+            if (lane == 0) pos = atomicAdd(&call.counters[0], (unsigned)__popcll(mf));
             pos = uni(pos);
-            if (queued) call.worklist[pos + lanes_below(mask)] = shot;
+            if (front) call.worklist[pos + lanes_below(mf)] = shot;
+        }
+        if (mb) {
+            unsigned pos = 0;
+            if (lane == 0) pos = atomicAdd(&call.counters[1], (unsigned)__popcll(mb));
+            pos = uni(pos);
+            if (queued && !front) call.worklist[call.num_shots - 1 - (int)(pos + lanes_below(mb))] = shot;
         }
     }
     if (ballot(bad != 0u) && lane == 0) atomicOr(&call.counters[2], kErrNonBinary);
@@ -1748,7 +1856,10 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
             for (int i = lane_id(); i < nr; i += 64) post[code.m + i] = code.R[i];
         }
     }
-    const unsigned total = call.counters[0];
+    // worklist: shots likely to search deep at the front (counters[0] of them),
+    // the others at the back of the buffer (counters[1]), claimed in that order
+    const unsigned heavy = call.counters[0];
+    const unsigned total = heavy + call.counters[1];
     // shots per claim: the tuned batch when the worklist is long, fewer when
     // there are not several claims per wave (small batches finish sooner)
     unsigned B = total / (4u * gridDim.x);
@@ -1805,7 +1916,7 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
             if (got < total && p0 < total) {
                 const unsigned left = (total - 1u - p0) / kClaimWords + 1u;   // positions p0, p0 + 8, ... below total
                 const int nb = (int)(left < B ? left : B);
-                stage_claim(code, call, p0, nb, L);
+                stage_claim(code, call, p0, nb, heavy, L);
                 for (int j = 0; j < nb; ++j) decode_shot_wave(code, call, L.stage + j * call.stage_words, L);
                 ctask = -1;
                 continue;

@@ -99,6 +99,7 @@ struct DevCall {
     int rmax;                         // patterns per lane per chunk in the range path
     int claim_batch;                  // shots taken per worklist claim (<= kMaxBatch)
     int triage_search;                // 1: the triage kernel searches small first iterations one shot per lane
+    int split_worklist;               // 1 (search kernel): worklist in two segments, likely-deep shots first
     int stage_words;                  // search kernel: LDS words per claimed shot: id, signature, syndrome,
                                       // then original weight, degeneracy word and error words (cc with
                                       // errors) or the observed logicals (cln)
@@ -113,7 +114,8 @@ struct DevCall {
     uint32_t *lg_packed;              // [shots][NL32] (cln)
     uint32_t *s_sig;                  // [shots]
     int *worklist;                    // nonzero-syndrome shots
-    unsigned *counters;               // [0] worklist length, [2] error bits, [4..6] task counters, claim words
+    unsigned *counters;               // [0] worklist front length, [1] back length, [2] error bits,
+                                      // [4..6] task counters, claim words
     GrandDecoderResult *results;      // [shots]
     uint32_t *dec;                    // [shots][M32] decoded pattern
     uint32_t *declog;                 // [shots][NL32] decoded logicals (cln)
