@@ -470,17 +470,22 @@ __device__ __forceinline__ PrefHit scan_prefix_first(const DevCode &code, const 
             c[WP - 1] = last + 1;
             t = hi ^ nxt;
         } else {
+            // end of a run: increment the highest element that can move and
+            // restart the ones after it; every signature is reread (reads
+            // issued together, no branch per element)
             int i = -1;
 #pragma unroll
-            for (int l = 0; l < WP - 1; ++l) if (c[l] < M - WP + l) i = l;
-#pragma unroll
-            for (int l = 0; l < WP; ++l) {
-                if (l == i) c[l] += 1;
-                else if (l > 0 && l > i) c[l] = c[l - 1] + 1;
-            }
+            for (int l = 0; l < WP - 1; ++l) i = (c[l] < M - WP + l) ? l : i;
+            int nc[WP];
 #pragma unroll
             for (int l = 0; l < WP; ++l)
-                if (l >= i) acc[l] = ((l == 0) ? t0 : acc[(l == 0) ? 0 : l - 1]) ^ sg[c[l]];
+                nc[l] = (l < i) ? c[l] : (l == i) ? c[l] + 1 : (l == 0) ? c[0] : nc[(l == 0) ? 0 : l - 1] + 1;
+            uint32_t v[WP];
+#pragma unroll
+            for (int l = 0; l < WP; ++l) v[l] = sg[nc[l]];
+            uint32_t a2 = t0;
+#pragma unroll
+            for (int l = 0; l < WP; ++l) { a2 ^= v[l]; acc[l] = a2; c[l] = nc[l]; }
             hi = (WP >= 2) ? acc[(WP >= 2) ? WP - 2 : 0] : t0;
             t = acc[WP - 1];
         }
