@@ -30,6 +30,7 @@
 #include "../../include/grand_session.h"
 
 extern "C" {
+GRAND_INTERNAL size_t grand_fused_lds_bytes(int cap, int ml);
 GRAND_INTERNAL hipError_t grand_launch_fill(unsigned *p, unsigned v, int n, hipStream_t st);
 GRAND_INTERNAL hipError_t grand_launch_prep(const grand::DevCall *hcall, grand::DevCall *dcall, unsigned *counters, hipStream_t st);
 GRAND_INTERNAL hipError_t grand_launch_triage(const grand::DevCode *code, const grand::DevCall *call, int grid, hipStream_t st);
@@ -772,6 +773,22 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     // one claimed shot in the search kernel's LDS: id, signature, syndrome words,
     // then the packed errors (cc with errors) or the observed logicals (cln)
     c.stage_words = 2 + code.D + (c.input_errors ? 2 + code.M32 : 0) + (s->family == kCLN ? code.NL32 : 0);
+    // search-kernel pair table (first-hit kinds): what is left of the wave's share
+    // of the CU's LDS (at most 64 KB per workgroup) after everything else
+    const int wpc = (s->ex.waves_per_cu > 0) ? s->ex.waves_per_cu : g_tuning.waves_per_cu;
+    c.pair_cap = 0;
+    if (s->ex.engine != GRAND_ENGINE_PLAIN && s->search != kML) {
+        const size_t base = grand_fused_lds_bytes(c.cap, c.lscores != nullptr) +
+                            4 * ((size_t)s->code->search_lds_words + (size_t)c.claim_batch * c.stage_words) +
+                            4 * ((size_t)kPairBitmapWords + kPairBuckets);
+        const size_t budget = std::min<size_t>(65536, kLdsPerCu / (size_t)std::max(wpc, 1));
+        const long long all = (long long)c.cap * (c.cap - 1) / 2;
+        if (budget > base) {
+            const long long room = (long long)((budget - base) / 8);
+            const long long pc = std::min(room, std::min(all, 65535LL));
+            if (pc >= 256) c.pair_cap = (int)pc;
+        }
+    }
 
     // The task table is cleared once when allocated: every launch leaves all
     // slots idle (each owner waits for its chunks and frees its slot).  The
@@ -796,7 +813,6 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
         if (s->timing) HIPX(hipEventRecord(s->ev[2], st));
         return rc;
     }
-    const int wpc = (s->ex.waves_per_cu > 0) ? s->ex.waves_per_cu : g_tuning.waves_per_cu;
     const int grid = std::max(1, std::min(R.cu_count * wpc, num_shots));
     HIPX(grand_launch_fused(s->code->dcode_search.as<DevCode>(), dcall, &c, s->code->search_lds_words, grid, st));
     if (s->timing) HIPX(hipEventRecord(s->ev[2], st));

@@ -168,6 +168,10 @@ struct WaveLds {
     float LDS *llrP;
     unsigned long long LDS *ht;   // (signature << 32) | position, empty = kHtEmpty
     uint32_t LDS *stage;       // search kernel: the claimed shots, DevCall::stage_words each
+    uint32_t LDS *pbm;         // pair table (search kernel, pair_cap > 0): key filter [kPairBitmapWords],
+    uint32_t LDS *pend;        //   bucket ends [kPairBuckets], keys and (y | x << 16) values, bucket-sorted
+    uint32_t LDS *pkey;
+    uint32_t LDS *pval;
     uint32_t LDS *sw;  // syndrome words of the shot being searched, kMaxWords
     uint32_t LDS *sw2; // syndrome minus one column (learned pass), kMaxWords
     unsigned long long LDS *ncr; // C(n, k) for n < kNcrLdsRows, k < 8 (search kernel only)
@@ -202,7 +206,8 @@ __host__ __device__ __forceinline__ int ht_capacity(int cap) {
 constexpr int kLdsNcrOffset = 2 * kMaxWords;
 constexpr int kLdsCodeOffset = kLdsNcrOffset + 2 * kNcrLdsRows * kNcrStride;
 
-__device__ __forceinline__ WaveLds carve_lds(int cap, bool ml, int code_words = 0, int batch_words = 0) {
+__device__ __forceinline__ WaveLds carve_lds(int cap, bool ml, int code_words = 0, int batch_words = 0,
+                                             int pair_cap = 0) {
     uint32_t LDS *base = lds_base();
     WaveLds L;
     L.sw = base;
@@ -219,6 +224,14 @@ __device__ __forceinline__ WaveLds carve_lds(int cap, bool ml, int code_words = 
         L.llrP = (float LDS *)p; p += cap;
     } else {
         L.llrA = nullptr; L.P = nullptr; L.sigP = nullptr; L.llrP = nullptr;
+    }
+    if (pair_cap > 0) {
+        L.pbm = p; p += kPairBitmapWords;
+        L.pend = p; p += kPairBuckets;
+        L.pkey = p; p += pair_cap;
+        L.pval = p; p += pair_cap;
+    } else {
+        L.pbm = nullptr; L.pend = nullptr; L.pkey = nullptr; L.pval = nullptr;
     }
     L.stage = batch_words > 0 ? p : nullptr;
     return L;
@@ -594,6 +607,197 @@ __device__ __attribute__((noinline)) PrefHit scan_prefix_learned(int w, const De
     }
 }
 
+// ------------------------------------------------------------ pair table
+// Meet in the middle for first-hit classes of weight w >= 3: a w-subset
+// c_0 < ... < c_{w-1} of A[0..tam) is its (w-2)-stem c_0..c_{w-3} and the pair
+// (y, x) = (c_{w-2}, c_{w-1}) after it.  The table holds every pair y < x of
+// the list keyed by sg[y] ^ sg[x]; a stem with target T = sig(s) ^ sig(stem)
+// looks up the pairs with key T and y past the stem.  Stems are walked in
+// lexicographic order and, within a stem, the lexicographically smallest valid
+// pair is taken, so the hit and its rank are those of the prefix walk (and of
+// a pattern-by-pattern walk); C(tam-2, w-2) stems replace C(tam-1, w-1)
+// prefixes.
+
+// Pairs of A[0..tam) in bucket order (bucket = key bits 16..26): after the
+// build pend[b] is the end of bucket b (its start is pend[b-1], or 0), and
+// pbm has the bit of every key's low 15 bits.  Each lane takes a contiguous
+// block of the C(tam, 2) pairs in lexicographic order.
+__device__ __attribute__((noinline)) void build_pairs_lds(uint32_t LDS *pend, uint32_t LDS *pbm, uint32_t LDS *pkey,
+                                                        uint32_t LDS *pval, const uint32_t LDS *sg, int tam) {
+    const int lane = lane_id();
+    for (int i = lane; i < kPairBuckets; i += 64) pend[i] = 0u;
+    for (int i = lane; i < kPairBitmapWords; i += 64) pbm[i] = 0u;
+    __syncthreads();
+    const int P = tam * (tam - 1) / 2;
+    const int per = (P + 63) / 64;
+    const int k0 = lane * per < P ? lane * per : P;
+    const int k1 = k0 + per < P ? k0 + per : P;
+    // first pair of the block: rows y hold tam - 1 - y pairs
+    int y0 = 0, rs = 0;
+    while (y0 < tam - 1 && rs + (tam - 1 - y0) <= k0) { rs += tam - 1 - y0; ++y0; }
+    const int x0 = y0 + 1 + (k0 - rs);
+    for (int pass = 0; pass < 2; ++pass) {
+        int y = y0, x = x0;
+        uint32_t sy = (k0 < k1) ? sg[y] : 0u;
+        for (int k = k0; k < k1; ++k) {
+            const uint32_t key = sy ^ sg[x];
+            const unsigned b = (key >> 16) & (unsigned)(kPairBuckets - 1);
+            if (pass == 0) {
+                __hip_atomic_fetch_add(&pend[b], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_or(&pbm[(key & 0x7FFFu) >> 5], 1u << (key & 31u), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else {
+                const unsigned pos =
+                    __hip_atomic_fetch_add(&pend[b], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                pkey[pos] = key;
+                pval[pos] = (uint32_t)y | ((uint32_t)x << 16);
+            }
+            if (++x == tam) { ++y; x = y + 1; if (k + 1 < k1) sy = sg[y]; }
+        }
+        __syncthreads();
+        if (pass == 0) {
+            // exclusive prefix sum of the bucket counts: 32 consecutive buckets per lane
+            constexpr int kPer = kPairBuckets / 64;
+            uint32_t sum = 0;
+            for (int i = 0; i < kPer; ++i) sum += pend[lane * kPer + i];
+            uint32_t incl = sum;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t u = (uint32_t)__shfl_up((int)incl, o, 64);
+                if (lane >= o) incl += u;
+            }
+            uint32_t run = incl - sum;
+            for (int i = 0; i < kPer; ++i) {
+                const uint32_t v = pend[lane * kPer + i];
+                pend[lane * kPer + i] = run;
+                run += v;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__device__ __forceinline__ void build_pairs(const WaveLds &L, const uint32_t LDS *sg, int tam) {
+    build_pairs_lds(L.pend, L.pbm, L.pkey, L.pval, sg, tam);
+}
+
+// Lexicographically smallest pair (y, x), y > last, with key T whose pattern
+// (stem p0.., y, x) matches exactly, packed y << 16 | x, or -1.  Out of line.
+template <int WS>
+__device__ __attribute__((noinline)) int probe_pairs(const DevCode &code, const uint32_t LDS *s, uint32_t T, int last,
+                                                    const int LDS *act, const uint32_t LDS *pkey,
+                                                    const uint32_t LDS *pval, const uint32_t LDS *pend, int p0,
+                                                    int p1, int p2, int p3) {
+    const int pc[4] = {p0, p1, p2, p3};
+    const unsigned b = (T >> 16) & (unsigned)(kPairBuckets - 1);
+    const int e0 = b ? (int)pend[b - 1] : 0, e1 = (int)pend[b];
+    int best = -1;
+    for (int e = e0; e < e1; ++e) {
+        if (pkey[e] != T) continue;
+        const uint32_t v = pval[e];
+        const int y = (int)(v & 0xFFFFu), x = (int)(v >> 16);
+        if (y <= last) continue;
+        if (best >= 0) {
+            const int by = best >> 16, bx = best & 0xFFFF;
+            if (y > by || (y == by && x >= bx)) continue;
+        }
+        int cols[WS + 2];
+#pragma unroll
+        for (int l = 0; l < WS; ++l) cols[l] = act[pc[l]];
+        cols[WS] = act[y];
+        cols[WS + 1] = act[x];
+        if (exact_match<WS + 2>(code, s, cols, WS + 2)) best = (y << 16) | x;
+    }
+    return best;
+}
+
+// First-hit walk of len stems from stem rank q0 (WS-subsets of [0, tam-2)),
+// with the loop structure of scan_prefix_first.  The hit's j is its stem
+// offset within the lane's range.
+template <int WS>
+__device__ __forceinline__ PrefHit scan_stems_first(const DevCode &code, const uint32_t LDS *s, uint32_t t0,
+                                                    uint64_t q0, int len, int tam, const int LDS *act,
+                                                    const uint32_t LDS *sg, const uint32_t LDS *pkey,
+                                                    const uint32_t LDS *pval, const uint32_t LDS *pend,
+                                                    const uint32_t LDS *pbm) {
+    PrefHit h{~0ull, 0.0f, -1};
+    if (len <= 0) return h;
+    const int M = tam - 2;
+    int c[WS];
+    const bool small = tam < kNcrLdsRows;
+    if (small) unrank_t<WS>(q0, M, ncr_lds(), c); else unrank_t<WS>(q0, M, code.nCr, c);
+    uint32_t acc[WS];
+    uint32_t a = t0;
+#pragma unroll
+    for (int l = 0; l < WS; ++l) { a ^= sg[c[l]]; acc[l] = a; }
+    uint32_t hi = (WS >= 2) ? acc[(WS >= 2) ? WS - 2 : 0] : t0;
+    uint32_t t = acc[WS - 1];
+    for (int j = 0;;) {
+        const int last = c[WS - 1];
+        const uint32_t nxt = sg[last + 1];            // last + 1 <= M = tam - 2
+        if ((pbm[(t & 0x7FFFu) >> 5] >> (t & 31u)) & 1u) {
+            const int yx = probe_pairs<WS>(code, s, t, last, act, pkey, pval, pend, c[0],
+                                           WS > 1 ? c[WS > 1 ? 1 : 0] : 0, WS > 2 ? c[WS > 2 ? 2 : 0] : 0,
+                                           WS > 3 ? c[WS > 3 ? 3 : 0] : 0);
+            if (yx >= 0) {
+                int cc[WS + 1];
+#pragma unroll
+                for (int l = 0; l < WS; ++l) cc[l] = c[l];
+                cc[WS] = yx >> 16;
+                const int x = yx & 0xFFFF;
+                h.rank = small ? rank_with_last<WS + 1>(cc, x, tam, ncr_lds())
+                               : rank_with_last<WS + 1>(cc, x, tam, code.nCr);
+                h.j = j;
+                break;
+            }
+        }
+        if (++j == len) break;
+        if (last + 1 < M) {
+            c[WS - 1] = last + 1;
+            t = hi ^ nxt;
+        } else {
+            int i = -1;
+#pragma unroll
+            for (int l = 0; l < WS - 1; ++l) i = (c[l] < M - WS + l) ? l : i;
+            int nc[WS];
+#pragma unroll
+            for (int l = 0; l < WS; ++l)
+                nc[l] = (l < i) ? c[l] : (l == i) ? c[l] + 1 : (l == 0) ? c[0] : nc[(l == 0) ? 0 : l - 1] + 1;
+            uint32_t v[WS];
+#pragma unroll
+            for (int l = 0; l < WS; ++l) v[l] = sg[nc[l]];
+            uint32_t a2 = t0;
+#pragma unroll
+            for (int l = 0; l < WS; ++l) { a2 ^= v[l]; acc[l] = a2; c[l] = nc[l]; }
+            hi = (WS >= 2) ? acc[(WS >= 2) ? WS - 2 : 0] : t0;
+            t = acc[WS - 1];
+        }
+    }
+    return h;
+}
+
+__device__ __attribute__((noinline)) PrefHit scan_stems_w(int w, const DevCode &code, const uint32_t LDS *s,
+                                                         uint32_t t0, uint64_t q0, int len, int tam,
+                                                         const int LDS *act, const uint32_t LDS *sg,
+                                                         const uint32_t LDS *pkey, const uint32_t LDS *pval,
+                                                         const uint32_t LDS *pend, const uint32_t LDS *pbm) {
+    switch (w) {
+        case 3: return scan_stems_first<1>(code, s, t0, q0, len, tam, act, sg, pkey, pval, pend, pbm);
+        case 4: return scan_stems_first<2>(code, s, t0, q0, len, tam, act, sg, pkey, pval, pend, pbm);
+        case 5: return scan_stems_first<3>(code, s, t0, q0, len, tam, act, sg, pkey, pval, pend, pbm);
+        default: return scan_stems_first<4>(code, s, t0, q0, len, tam, act, sg, pkey, pval, pend, pbm);
+    }
+}
+
+// Whether a first-hit class of weight w over tam positions is walked by stems:
+// the table must fit and the class must have many more prefixes than the list
+// has pairs (the table is built once per list and reused by its later classes).
+__device__ __forceinline__ bool use_pairs(const DevCode &code, const DevCall &call, int w, int tam) {
+    if (call.pair_cap <= 0 || w < 3 || w > 6 || tam < 4) return false;
+    const uint64_t P = (uint64_t)tam * (uint64_t)(tam - 1) / 2u;
+    return P <= (uint64_t)call.pair_cap && ncr_w(code, tam - 1, w - 1) > 2u * P;
+}
+
 // Weight 1: every position with sig == t0, scanned by the wave 64 at a time.
 // MODE 0 returns the lowest matching position; MODE 1 the cheapest (ties: lowest).
 template <int MODE>
@@ -629,6 +833,8 @@ __device__ PrefHit weight_one(const DevCode &code, const uint32_t LDS *s, uint32
 // nothing (the counter may run past nchunks by at most one step per wave, far
 // from carrying into the nchunks field).
 constexpr unsigned kMaxChunks = (1u << 20) - 1;
+constexpr int kStemCost = 4;    // prefix steps one stem step is counted as (split decision, chunk size)
+constexpr int kMinStemR = 4;
 __device__ __forceinline__ unsigned ctl_next(unsigned long long c) { return (unsigned)c; }
 __device__ __forceinline__ unsigned ctl_nch(unsigned long long c) { return (unsigned)((c >> 32) & 0xFFFFFu); }
 __device__ __forceinline__ unsigned ctl_gen(unsigned long long c) { return (unsigned)(c >> 52); }
@@ -668,28 +874,71 @@ __device__ __forceinline__ unsigned claim_chunk(TaskSlot *T, unsigned *open_coun
     return ch;
 }
 
-// One chunk (64 lanes x R prefixes) of a shared class; the class's arrays and
-// hash table are in this wave's LDS.  First-hit: atomicMin of the lowest
-// matching rank (and of its prefix rank, so later chunks are skipped).  ML:
-// atomicMin of (cost key << 32 | rank).  Returns the chunk's result (~0 = none).
-__device__ unsigned long long task_chunk(const DevCode &code, TaskSlot *T, const uint32_t LDS *s, uint32_t t0, int w,
-                                        int tam, int R, uint64_t NP, bool ml, unsigned ch, const int LDS *act,
-                                        const uint32_t LDS *sg, const float LDS *llr, const unsigned long long LDS *ht,
-                                        unsigned hmask) {
+// Units of one class of a task: (w-2)-stems with the pair table, else (w-1)-prefixes.
+__device__ __forceinline__ uint64_t class_units(const DevCode &code, int tam, int w, bool pairs) {
+    return pairs ? ncr_w(code, tam - 2, w - 2) : ncr_w(code, tam - 1, w - 1);
+}
+
+// Chunks of a task's class: units per lane per chunk, bounded so that the
+// chunk count fits the 20-bit ctl field.  A stem step costs several prefix
+// steps, so stem classes get a quarter of the units per lane (at least
+// kMinStemR).
+__device__ __forceinline__ int class_chunk_r(const DevCall &call, uint64_t units, bool pairs) {
+    int R = pairs ? (call.rmax / kStemCost > kMinStemR ? call.rmax / kStemCost : kMinStemR) : call.rmax;
+    const uint64_t per = 64ull * (uint64_t)kMaxChunks;
+    if (units / per >= (uint64_t)R) R = (int)(units / per + 1);
+    return R;
+}
+
+// One chunk (64 lanes x R units) of a shared task; the list's arrays and
+// tables are in this wave's LDS.  A task holds consecutive weight classes of
+// one list; chunk ch belongs to the class whose chunk range holds it.
+// First-hit: atomic-min of the matching pattern index over the task's classes
+// concatenated in list order (earlier classes' sizes plus the rank), and of
+// its unit index (so that chunks past a known hit are skipped).  ML (one
+// class): atomic-min of (cost key << 32 | rank).  Returns the chunk's result
+// (~0 = none).
+__device__ unsigned long long task_chunk(const DevCode &code, TaskSlot *T, const uint32_t LDS *s, uint32_t t0,
+                                        int tam, bool ml, unsigned ch, const int LDS *act, const uint32_t LDS *sg,
+                                        const float LDS *llr, const unsigned long long LDS *ht, unsigned hmask,
+                                        const uint32_t LDS *pkey, const uint32_t LDS *pval,
+                                        const uint32_t LDS *pend, const uint32_t LDS *pbm) {
     const int lane = lane_id();
-    const uint64_t base = (uint64_t)ch * 64ull * (uint64_t)R;
-    if (!ml && base > uni64(ld_relaxed(&T->best_q))) return ~0ull - 1;
+    const int nclass = (int)uni((unsigned)ld_relaxed(&T->nclass));
+    const unsigned wpack = uni((unsigned)ld_relaxed(&T->wpack)), ppack = uni((unsigned)ld_relaxed(&T->ppack));
+    int w = 0, R = 1, k = 0;
+    bool pairs = false;
+    uint64_t ubase = 0, pbase = 0, NP = 0;
+    unsigned ch0 = 0;
+    for (int i = 0; i < nclass; ++i) {
+        const int wi = (int)((wpack >> (4 * i)) & 15u);
+        const bool pi = ((ppack >> i) & 1u) != 0;
+        const unsigned c1 = uni(ld_relaxed(&T->cch[i + 1]));
+        if (ch < c1) {
+            k = i; w = wi; pairs = pi;
+            ch0 = uni(ld_relaxed(&T->cch[i]));
+            R = (int)uni((unsigned)ld_relaxed(&T->cR[i]));
+            NP = class_units(code, tam, wi, pi);
+            break;
+        }
+        ubase += class_units(code, tam, wi, pi);
+        pbase += ncr_w(code, tam, wi);
+    }
+    (void)k;
+    const uint64_t base = (uint64_t)(ch - ch0) * 64ull * (uint64_t)R;
+    if (!ml && ubase + base > uni64(ld_relaxed(&T->best_q))) return ~0ull - 1;
     const uint64_t q0 = base + (uint64_t)lane * (uint64_t)R;
     const int len = q0 >= NP ? 0 : (int)((NP - q0) < (uint64_t)R ? (NP - q0) : (uint64_t)R);
     const PrefHit h = ml ? scan_prefix_w<1>(w, code, s, t0, q0, len, tam, act, sg, llr, ht, hmask)
-                         : scan_prefix_w<0>(w, code, s, t0, q0, len, tam, act, sg, llr, ht, hmask);
+                    : pairs ? scan_stems_w(w, code, s, t0, q0, len, tam, act, sg, pkey, pval, pend, pbm)
+                            : scan_prefix_w<0>(w, code, s, t0, q0, len, tam, act, sg, llr, ht, hmask);
     unsigned long long out = ~0ull;
     if (!ml) {
         const uint64_t b = ballot(h.j >= 0);
         if (b) {
             const int hl = __ffsll((long long)b) - 1;
-            const uint64_t r = (uint64_t)__shfl((long long)h.rank, hl, 64);
-            const uint64_t q = q0 + (uint64_t)(h.j < 0 ? 0 : h.j);
+            const uint64_t r = pbase + (uint64_t)__shfl((long long)h.rank, hl, 64);
+            const uint64_t q = ubase + q0 + (uint64_t)(h.j < 0 ? 0 : h.j);
             const uint64_t qh = (uint64_t)__shfl((long long)q, hl, 64);
             if (lane == 0) { atomicMin(&T->best, (unsigned long long)r); atomicMin(&T->best_q, (unsigned long long)qh); }
             out = r;
@@ -724,18 +973,18 @@ __device__ void trace_chunk(const DevCall &call, int who, int slot, unsigned gen
     r[15] = (unsigned)now_ticks();
 }
 
-// Owner side: publish the weight-w class over act/sg/llr (this wave's LDS) as a
-// shared task, work on it, wait for every chunk and return the class result in
-// best (~0 = no match).  Returns false when no slot is free (caller searches alone).
+// Owner side: publish nclass consecutive weight classes of the list act/sg/llr
+// (this wave's LDS; weights in wpack, 4 bits each; bit i of ppack: class i by
+// stems) as one shared task, work on it, wait for every chunk and return the
+// result in best (~0 = no match): first-hit, the lowest matching pattern index
+// over the classes concatenated in order; ML (one class), the lowest (cost key,
+// rank).  Returns false when no slot is free (the caller searches alone).
 __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, const uint32_t LDS *s, uint32_t t0,
-                           int w, int tam, uint64_t Nw, uint64_t NP, bool ml, const int LDS *act,
+                           int tam, bool ml, int nclass, unsigned wpack, unsigned ppack, const int LDS *act,
                            const uint32_t LDS *sg, const float LDS *llr, const unsigned long long LDS *ht, unsigned hmask,
-                           unsigned long long &best) {
+                           const uint32_t LDS *pkey, const uint32_t LDS *pval,
+                           const uint32_t LDS *pend, const uint32_t LDS *pbm, unsigned long long &best) {
     const int lane = lane_id();
-    int R = call.rmax;
-    const uint64_t per = 64ull * (uint64_t)kMaxChunks;
-    if (NP / per >= (uint64_t)R) R = (int)(NP / per + 1);
-    const unsigned nch = (unsigned)((NP + 64ull * R - 1) / (64ull * R));
     // find a free slot: 64 candidates per step, lane 0 takes the first free one
     int slot = -1;
     const int start = (int)((blockIdx.x * 131u) & (kTaskSlots - 1));
@@ -759,16 +1008,29 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
         st_relaxed(&arr[call.cap + i], (uint32_t)sg[i]);
         if (ml) st_relaxed(&arr[2 * call.cap + i], __float_as_uint(llr[i]));
     }
+    // chunk ranges of the classes (lane i < nclass holds class i)
+    unsigned nch = 0, myc = 0;
+    int myR = 1;
+    for (int i = 0; i < nclass; ++i) {
+        const int wi = (int)((wpack >> (4 * i)) & 15u);
+        const bool pi = ((ppack >> i) & 1u) != 0;
+        const uint64_t u = class_units(code, tam, wi, pi);
+        const int Ri = class_chunk_r(call, u, pi);
+        if (lane == i) { myc = nch; myR = Ri; }
+        nch += (unsigned)((u + 64ull * (uint64_t)Ri - 1) / (64ull * (uint64_t)Ri));
+    }
     const unsigned gen = (ctl_gen(ld_relaxed(&T->ctl)) + 1u) & 0xFFFu;
     unsigned uid = 0;
     if (lane == 0) uid = atomicAdd(&call.task_counters[2], 1u);
     uid = uni(uid);
     if (lane == 0) {
-        st_relaxed(&T->gen, uid); st_relaxed(&T->nchunks, nch); st_relaxed(&T->shot, shot); st_relaxed(&T->w, w);
-        st_relaxed(&T->tam, tam); st_relaxed(&T->R, R); st_relaxed(&T->ml, ml ? 1 : 0); st_relaxed(&T->t0, t0);
-        st_relaxed(&T->nw, (unsigned long long)Nw); st_relaxed(&T->np, (unsigned long long)NP);
+        st_relaxed(&T->gen, uid); st_relaxed(&T->nchunks, nch); st_relaxed(&T->shot, shot);
+        st_relaxed(&T->tam, tam); st_relaxed(&T->ml, ml ? 1 : 0); st_relaxed(&T->t0, t0);
+        st_relaxed(&T->nclass, nclass); st_relaxed(&T->wpack, (int)wpack); st_relaxed(&T->ppack, (int)ppack);
+        st_relaxed(&T->cch[nclass], nch);
         st_relaxed(&T->done, 0u); st_relaxed(&T->best, ~0ull); st_relaxed(&T->best_q, ~0ull);
     }
+    if (lane < nclass) { st_relaxed(&T->cch[lane], myc); st_relaxed(&T->cR[lane], myR); }
     if (lane < kMaxWords) st_relaxed(&T->s[lane], (uint32_t)s[lane]);
     wait_mem();                                   // the task is complete in memory before it is published
     if (lane == 0) {
@@ -779,8 +1041,9 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
         unsigned g;
         const unsigned ch = claim_chunk(T, &call.task_counters[0], g);
         if (ch == ~0u) break;
-        const unsigned long long res = task_chunk(code, T, s, t0, w, tam, R, NP, ml, ch, act, sg, llr, ht, hmask);
-        trace_chunk(call, 0, slot, uid, ch, w, tam, R, NP, t0, s, act, sg, res);
+        const unsigned long long res =
+            task_chunk(code, T, s, t0, tam, ml, ch, act, sg, llr, ht, hmask, pkey, pval, pend, pbm);
+        trace_chunk(call, 0, slot, uid, ch, (int)(wpack & 15u), tam, nclass, nch, t0, s, act, sg, res);
         wait_mem();                               // the chunk's result atomics complete before it counts as done
         if (lane == 0) atomicAdd(&T->done, 1u);
     }
@@ -792,8 +1055,10 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
 }
 
 // Helper side: claim one chunk of any open task and process it.  The task's
-// arrays are copied into this wave's LDS and its hash table built once per task.
-__device__ bool help_task(const DevCode &code, const DevCall &call, const WaveLds &L, int &ctask, unsigned &hmask) {
+// arrays are copied into this wave's LDS and its tables (hash table and/or pair
+// table, as its classes need) built once per task.
+__device__ __forceinline__ bool help_task(const DevCode &code, const DevCall &call, const WaveLds &L, int &ctask,
+                                          unsigned &hmask) {
     const int lane = lane_id();
     if (__builtin_amdgcn_readfirstlane(ld_relaxed(&call.task_counters[0])) == 0u) return false;
     static_assert(kTaskSlots == 128, "the open-slot scan reads two 64-slot halves");
@@ -812,10 +1077,11 @@ __device__ bool help_task(const DevCode &code, const DevCall &call, const WaveLd
     if (ch == ~0u) return true;
     // the task was complete in memory before its ctl was published; device-coherent
     // loads read it after the claim
-    const int w = ld_relaxed(&T->w), tam = ld_relaxed(&T->tam), R = ld_relaxed(&T->R);
+    const int tam = ld_relaxed(&T->tam);
     const unsigned uid = uni(ld_relaxed(&T->gen));
     const bool ml = ld_relaxed(&T->ml) != 0;
-    const uint64_t NP = ld_relaxed(&T->np);
+    const int nclass = ld_relaxed(&T->nclass);
+    const unsigned wpack = (unsigned)ld_relaxed(&T->wpack), ppack = (unsigned)ld_relaxed(&T->ppack);
     const uint32_t t0 = ld_relaxed(&T->t0);
     if (lane < kMaxWords) L.sw[lane] = ld_relaxed(&T->s[lane]);
     __syncthreads();
@@ -830,12 +1096,15 @@ __device__ bool help_task(const DevCode &code, const DevCall &call, const WaveLd
             if (ml) L.llrP[i] = __uint_as_float(ld_relaxed(&arr[2 * call.cap + i]));
         }
         __syncthreads();
-        hmask = build_ht(L.ht, sg, tam);
+        const unsigned all = (nclass >= 32) ? ~0u : ((1u << nclass) - 1u);
+        if ((ppack & all) != 0u) build_pairs(L, sg, tam);
+        if ((ppack & all) != all) hmask = build_ht(L.ht, sg, tam);
         ctask = (int)uid;
     }
-    const unsigned long long res =
-        task_chunk(code, T, s, t0, w, tam, R, NP, ml, ch, act, sg, ml ? L.llrP : nullptr, L.ht, hmask);
-    trace_chunk(call, 1, slot, uid, ch, w, tam, R, NP, t0, s, act, sg, res);
+    const unsigned long long res = task_chunk(code, T, s, t0, tam, ml, ch, act, sg, ml ? L.llrP : nullptr, L.ht,
+                                              hmask, L.pkey, L.pval, L.pend, L.pbm);
+    trace_chunk(call, 1, slot, uid, ch, (int)(wpack & 15u), tam, nclass, ld_relaxed(&T->nchunks), t0, s, act, sg,
+                res);
     wait_mem();                                   // the chunk's result atomics complete before it counts as done
     if (lane == 0) atomicAdd(&T->done, 1u);
     return true;
@@ -844,17 +1113,17 @@ __device__ bool help_task(const DevCode &code, const DevCall &call, const WaveLd
 // ------------------------------------------------------------ class searches
 constexpr uint64_t kFlatMax = 128;   // classes up to this size go pattern by pattern, one per lane
 
-// First matching rank in the weight-w class over act/sg (hash table ht), or ~0.
+// First matching rank in the weight-w class over act/sg (hash table ht, or the
+// pair table with pairs), searched by this wave alone, or ~0.
 __device__ uint64_t class_first_hit(const DevCode &code, const DevCall &call, int shot, const uint32_t LDS *s,
                                     uint32_t t0, int w, int tam, uint64_t Nw, const int LDS *act,
-                                    const uint32_t LDS *sg, const unsigned long long LDS *ht, unsigned hmask) {
+                                    const uint32_t LDS *sg, const unsigned long long LDS *ht, unsigned hmask,
+                                    bool pairs, const uint32_t LDS *pkey, const uint32_t LDS *pval,
+                                    const uint32_t LDS *pend, const uint32_t LDS *pbm) {
     const int lane = lane_id();
     if (w == 1) return weight_one<0>(code, s, t0, tam, act, sg, nullptr).rank;
-    const uint64_t NP = ncr_w(code, tam - 1, w - 1);
-    unsigned long long shared_best;
-    if (call.split_threshold && NP > call.split_threshold &&
-        run_shared(code, call, shot, s, t0, w, tam, Nw, NP, false, act, sg, nullptr, ht, hmask, shared_best))
-        return shared_best;
+    // units: stems with the pair table, else prefixes
+    const uint64_t NP = class_units(code, tam, w, pairs);
     uint64_t pos = 0;
     while (pos < NP) {
         const uint64_t rem = NP - pos;
@@ -862,7 +1131,8 @@ __device__ uint64_t class_first_hit(const DevCode &code, const DevCall &call, in
         const int R = (int)(r64 < (uint64_t)call.rmax ? r64 : (uint64_t)call.rmax);
         const uint64_t q0 = pos + (uint64_t)lane * (uint64_t)R;
         const int len = q0 >= NP ? 0 : (int)((NP - q0) < (uint64_t)R ? (NP - q0) : (uint64_t)R);
-        const PrefHit h = scan_prefix_w<0>(w, code, s, t0, q0, len, tam, act, sg, nullptr, ht, hmask);
+        const PrefHit h = pairs ? scan_stems_w(w, code, s, t0, q0, len, tam, act, sg, pkey, pval, pend, pbm)
+                                : scan_prefix_w<0>(w, code, s, t0, q0, len, tam, act, sg, nullptr, ht, hmask);
         const uint64_t b = ballot(h.j >= 0);
         if (b) return (uint64_t)__shfl((long long)h.rank, __ffsll((long long)b) - 1, 64);
         pos += 64ull * (uint64_t)R;
@@ -894,6 +1164,7 @@ __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, int sho
     const int lane = lane_id();
     unsigned hmask = 0;          // hash table of A, built before the first class that needs it
     bool have_table = false;
+    bool have_pairs = false;     // pair table of A, likewise
     uint64_t N[kMaxW];
     uint64_t total = 0;
 #pragma unroll
@@ -947,8 +1218,60 @@ __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, int sho
             if (!hit) { class_start = flat_end; kc = kend; }
         } else {
             const int w = wl[kc];
-            if (!have_table && w > 1) { hmask = build_ht(L.ht, L.sigA, tam); have_table = true; }
-            const uint64_t r = class_first_hit(code, call, shot, s, t0, w, tam, N[kc], L.A, L.sigA, L.ht, hmask);
+            const bool pairs = use_pairs(code, call, w, tam);
+            const uint64_t units = class_units(code, tam, w, pairs);
+            if (w >= 2 && call.split_threshold && units * (pairs ? (uint64_t)kStemCost : 1u) > call.split_threshold) {
+                // this class and the following ones of weight >= 2 as one shared task:
+                // no barrier between the classes, and the task's lowest matching
+                // pattern index is the first hit over them in list order
+                int nc = 0, kend = kc;
+                unsigned wpack = 0, ppack = 0;
+                bool need_ht = false, need_pairs = false;
+                while (kend < nw && nc < kTaskClasses && wl[kend] >= 2) {
+                    if (N[kend] > 0) {
+                        const bool pk = use_pairs(code, call, wl[kend], tam);
+                        wpack |= (unsigned)wl[kend] << (4 * nc);
+                        ppack |= (pk ? 1u : 0u) << nc;
+                        need_pairs = need_pairs || pk;
+                        need_ht = need_ht || !pk;
+                        ++nc;
+                    }
+                    ++kend;
+                }
+                if (need_pairs && !have_pairs) { build_pairs(L, L.sigA, tam); have_pairs = true; }
+                if (need_ht && !have_table) { hmask = build_ht(L.ht, L.sigA, tam); have_table = true; }
+                unsigned long long best;
+                if (run_shared(code, call, shot, s, t0, tam, false, nc, wpack, ppack, L.A, L.sigA, nullptr, L.ht, hmask,
+                               L.pkey, L.pval, L.pend, L.pbm, best)) {
+                    uint64_t span = 0;
+                    for (int k = kc; k < kend; ++k) span += N[k];
+                    if (best != ~0ull) {
+                        // class of the hit, and its rank within the class
+                        uint64_t r = best;
+                        int k = kc;
+                        while (r >= N[k]) { r -= N[k]; ++k; }
+                        hit = true;
+                        hit_lane = 0;
+                        hit_w = wl[k];
+                        hit_idx = class_start + best;
+                        if (tam < kNcrLdsRows) unrank_dyn(r, tam, hit_w, ncr_lds(), c);
+                        else unrank_dyn(r, tam, hit_w, code.nCr, c);
+                    } else {
+                        class_start += span;
+                        kc = kend;
+                    }
+                    continue;
+                }
+            }
+            if (pairs) {
+                if (!have_pairs) { build_pairs(L, L.sigA, tam); have_pairs = true; }
+            } else if (!have_table && w > 1) {
+                hmask = build_ht(L.ht, L.sigA, tam);
+                have_table = true;
+            }
+            const uint64_t r =
+                class_first_hit(code, call, shot, s, t0, w, tam, N[kc], L.A, L.sigA, L.ht, hmask, pairs, L.pkey,
+                                L.pval, L.pend, L.pbm);
             if (r != ~0ull) {
                 hit = true;
                 hit_lane = 0;
@@ -1010,8 +1333,8 @@ __device__ bool ml_wave(const DevCode &code, const DevCall &call, int shot, cons
             const uint64_t NP = ncr_w(code, tam - 1, w - 1);
             unsigned long long shared_best;
             if (call.split_threshold && NP > call.split_threshold && Nw < (1ull << 32) &&
-                run_shared(code, call, shot, s, t0, w, tam, Nw, NP, true, L.P, L.sigP, L.llrP, L.ht, hmask,
-                           shared_best)) {
+                run_shared(code, call, shot, s, t0, tam, true, 1, (unsigned)w, 0u, L.P, L.sigP, L.llrP, L.ht, hmask,
+                           nullptr, nullptr, nullptr, nullptr, shared_best)) {
                 if (shared_best != ~0ull) {
                     chit = true;
                     cbest = key_to_float((uint32_t)(shared_best >> 32));
@@ -1359,7 +1682,8 @@ __device__ bool learned_pass(const DevCode &code, const DevCall &call, int shot,
 // One claimed shot, staged in LDS by stage_claim: slot = [shot, signature,
 // syndrome words (D), then original weight, degeneracy word and error words
 // (M32) for cc with errors, or the observed logicals (NL32) for cln].
-__device__ void decode_shot_wave(const DevCode &code, const DevCall &call, const uint32_t LDS *slot, const WaveLds &L) {
+__device__ __forceinline__ void decode_shot_wave(const DevCode &code, const DevCall &call, const uint32_t LDS *slot,
+                                                 const WaveLds &L) {
     ShotOutcome o;
     o.t0 = now_ticks();
     const int shot = (int)uni(slot[0]);
@@ -1837,7 +2161,7 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
     const DevCode &code = *codep;
     const DevCall &call = *callp;
     const WaveLds L = carve_lds(call.cap, call.kind == kML || call.lscores != nullptr, code.lds_words,
-                                call.claim_batch * call.stage_words);
+                                call.claim_batch * call.stage_words, call.pair_cap);
     if (code.lds_words > 0) {
         uint32_t LDS *dst = lds_base() + kLdsCodeOffset;
         const int nw = code.D * code.m;
@@ -2217,7 +2541,10 @@ GRAND_INTERNAL hipError_t grand_launch_triage(const DevCode *code, const DevCall
 GRAND_INTERNAL hipError_t grand_launch_fused(const DevCode *code, const DevCall *call, const DevCall *hcall, int code_lds_words,
                               int grid, hipStream_t st) {
     const size_t lds = grand_fused_lds_bytes(hcall->cap, hcall->kind == grand::kML || hcall->lscores != nullptr) +
-                       sizeof(uint32_t) * ((size_t)code_lds_words + (size_t)hcall->claim_batch * hcall->stage_words);
+                       sizeof(uint32_t) * ((size_t)code_lds_words + (size_t)hcall->claim_batch * hcall->stage_words +
+                                           (hcall->pair_cap > 0 ? (size_t)grand::kPairBitmapWords + grand::kPairBuckets +
+                                                                      2 * (size_t)hcall->pair_cap
+                                                                : 0));
     hipLaunchKernelGGL(grand::fused_kernel, dim3(grid), dim3(64), lds, st, code, call);
     return hipGetLastError();
 }

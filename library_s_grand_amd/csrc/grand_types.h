@@ -60,6 +60,10 @@ struct DevCode {
 constexpr int kTaskSlots = 128;
 constexpr int kCodeLdsMaxWords = 4096;   // codes up to 16 KB of staged words (DevCode::lds_words) go to LDS
 constexpr int kMaxBatch = 8;             // most shots per worklist claim
+// Pair table of the search kernel (first-hit classes of weight >= 3)
+constexpr int kPairBuckets = 2048;       // buckets, by signature bits 16..26
+constexpr int kPairBitmapWords = 1024;   // 2^15-bit filter, by signature bits 0..14
+constexpr size_t kLdsPerCu = 160 * 1024; // gfx950
 // Search of a shot by its own lane in the triage kernel (first iteration only)
 constexpr int kLaneMax = 12;             // active-list entries a lane keeps
 constexpr int kLaneBudget = 300;         // patterns a lane may test
@@ -72,17 +76,23 @@ constexpr int kClaimWords = 8;
 constexpr int kClaimBase = 16;        // in 64-bit words: byte offset 128
 constexpr int kClaimStride = 16;      // in 64-bit words: 128 bytes
 constexpr int kCounterBytes = 8 * (kClaimBase + kClaimWords * kClaimStride);
+constexpr int kTaskClasses = 6;          // weight classes one shared task can hold
 struct TaskSlot {
     unsigned long long ctl;           // generation | nchunks | next chunk to claim
-    unsigned long long best;          // first-hit: lowest matching rank; ML: (cost key << 32) | rank
-    unsigned long long best_q;        // first-hit: prefix rank of that hit (later chunks are skipped)
-    unsigned long long nw;            // class size C(tam, w)
-    unsigned long long np;            // prefixes C(tam - 1, w - 1)
+    unsigned long long best;          // first-hit: lowest matching pattern index over the task's classes
+                                      // (classes concatenated in list order); ML: (cost key << 32) | rank
+    unsigned long long best_q;        // first-hit: unit index of that hit over the task's classes (later
+                                      // chunks are skipped)
     unsigned done;                    // chunks finished
     unsigned busy;                    // 0 free, 1 owned
     unsigned gen;                     // launch-unique task id (helpers' LDS cache key)
     unsigned nchunks;
-    int shot, w, tam, R, ml;
+    int shot, tam, ml;
+    int nclass;                       // weight classes of the task, consecutive in the list (ML: 1)
+    int wpack;                        // their weights, 4 bits each, first class in the low bits
+    int ppack;                        // bit i: class i is walked by (w-2)-stems and the pair table
+    unsigned cch[kTaskClasses + 1];   // first chunk of each class; cch[nclass] = nchunks
+    int cR[kTaskClasses];             // units (prefixes or stems) per lane per chunk of each class
     uint32_t t0;                      // signature of the syndrome
     uint32_t s[kMaxWords];            // syndrome words
 };
@@ -100,6 +110,7 @@ struct DevCall {
     int claim_batch;                  // shots taken per worklist claim (<= kMaxBatch)
     int triage_search;                // 1: the triage kernel searches small first iterations one shot per lane
     int split_worklist;               // 1 (search kernel): worklist in two segments, likely-deep shots first
+    int pair_cap;                     // search kernel: pairs the LDS pair table holds (0 = no pair table)
     int stage_words;                  // search kernel: LDS words per claimed shot: id, signature, syndrome,
                                       // then original weight, degeneracy word and error words (cc with
                                       // errors) or the observed logicals (cln)

@@ -185,3 +185,37 @@ def test_triage_search_matches_oracle_and_search_kernel(oracle, product, tmp_pat
         T.assert_same(o, on, what + " (triage search on)")
         T.assert_same(on, off, what + " (on vs off)")
         print(what, "found", int(o[1]["found"].sum()), flush=True)
+
+
+# ---------------------------------------------------------------- pair table
+# Deep first-hit classes (weight >= 3, many more prefixes than pairs) are walked
+# by (w-2)-stems and the LDS pair table when it fits the wave's share of the
+# CU's LDS: at 1 and 2 waves per CU for these lists, not at 8.  Shared (split
+# 256) and unshared classes must give the oracle's results bit for bit.
+_PAIR_ORACLE = {}
+
+
+def _pair_cases(oracle, tmp_path):
+    if not _PAIR_ORACLE:
+        pf = T.phase_file(tmp_path)
+        hx, hz, e = T.cc_workload(144, 400, 0.015, seed=41)          # tam up to 87, 3.9e7 patterns
+        _PAIR_ORACLE["prob_cc BB144"] = (T.run_cc, (G.KIND_PROB_CC, hx, hz),
+                                         dict(errors=e, cfg=G.make_config(max_error_weight=4), prob_file=pf))
+        hx72, hz72, e72 = T.cc_workload(72, 600, 0.03, seed=42)     # weight-5 classes, tam up to 59
+        _PAIR_ORACLE["cc BB72 W5"] = (T.run_cc, (G.KIND_CC, hx72, hz72),
+                                      dict(errors=e72, cfg=G.make_config(max_error_weight=5)))
+        H, L, pr, det, lg = T.dem_workload(300, seed=43)
+        _PAIR_ORACLE["cln DEM"] = (T.run_cln, (G.KIND_CLN, H, L, det, lg),
+                                   dict(cfg=G.make_config(max_error_weight=4, max_size_active_positions=60)))
+        for k, (runner, args, kw) in list(_PAIR_ORACLE.items()):
+            _PAIR_ORACLE[k] = (runner, args, kw, runner(oracle, *args, **kw))
+    return _PAIR_ORACLE
+
+
+@pytest.mark.parametrize("wpc", [1, 2, 8])
+def test_pair_table_deep_classes(oracle, product, tmp_path, wpc):
+    for what, (runner, args, kw, o) in _pair_cases(oracle, tmp_path).items():
+        for split in (256, -1):
+            p = runner(product, *args, ex=G.make_ex(waves_per_cu=wpc, split_threshold=split), **kw)
+            T.assert_same(o, p, f"{what} wpc={wpc} split={split}")
+        print(what, "wpc", wpc, "max count", int(o[1]["total_tested_combinations"].max()), flush=True)
