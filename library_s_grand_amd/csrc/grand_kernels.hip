@@ -636,30 +636,57 @@ __device__ __attribute__((noinline)) void build_pairs_lds(uint32_t LDS *pend, ui
     int y0 = 0, rs = 0;
     while (y0 < tam - 1 && rs + (tam - 1 - y0) <= k0) { rs += tam - 1 - y0; ++y0; }
     const int x0 = y0 + 1 + (k0 - rs);
+    constexpr int U = 8;                         // pairs per step, their LDS reads and atomics issued together
     for (int pass = 0; pass < 2; ++pass) {
         int y = y0, x = x0;
-        uint32_t sy = (k0 < k1) ? sg[y] : 0u;
-        for (int k = k0; k < k1; ++k) {
-            const uint32_t key = sy ^ sg[x];
-            const unsigned b = (key >> 16) & (unsigned)(kPairBuckets - 1);
-            if (pass == 0) {
-                __hip_atomic_fetch_add(&pend[b], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                __hip_atomic_fetch_or(&pbm[(key & 0x7FFFu) >> 5], 1u << (key & 31u), __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_WORKGROUP);
-            } else {
-                const unsigned pos =
-                    __hip_atomic_fetch_add(&pend[b], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                pkey[pos] = key;
-                pval[pos] = (uint32_t)y | ((uint32_t)x << 16);
+        for (int k = k0; k < k1; k += U) {
+            int yy[U], xx[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                yy[u] = y; xx[u] = x;
+                const bool wrap = x + 1 == tam;
+                y = wrap ? y + 1 : y;
+                x = wrap ? y + 1 : x + 1;
             }
-            if (++x == tam) { ++y; x = y + 1; if (k + 1 < k1) sy = sg[y]; }
+            uint32_t key[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const bool ok = k + u < k1;
+                key[u] = ok ? (sg[yy[u]] ^ sg[xx[u]]) : 0u;
+            }
+            if (pass == 0) {
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (k + u < k1) {
+                        __hip_atomic_fetch_add(&pend[(key[u] >> 16) & (unsigned)(kPairBuckets - 1)], 1u,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        __hip_atomic_fetch_or(&pbm[(key[u] & 0x7FFFu) >> 5], 1u << (key[u] & 31u), __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+            } else {
+                unsigned pos[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    pos[u] = (k + u < k1) ? __hip_atomic_fetch_add(&pend[(key[u] >> 16) & (unsigned)(kPairBuckets - 1)],
+                                                                   1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                                          : 0u;
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (k + u < k1) {
+                        pkey[pos[u]] = key[u];
+                        pval[pos[u]] = (uint32_t)yy[u] | ((uint32_t)xx[u] << 16);
+                    }
+            }
         }
         __syncthreads();
         if (pass == 0) {
             // exclusive prefix sum of the bucket counts: 32 consecutive buckets per lane
+            // (unrolled, so the reads are issued together)
             constexpr int kPer = kPairBuckets / 64;
+            uint32_t v[kPer];
             uint32_t sum = 0;
-            for (int i = 0; i < kPer; ++i) sum += pend[lane * kPer + i];
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) { v[i] = pend[lane * kPer + i]; sum += v[i]; }
             uint32_t incl = sum;
 #pragma unroll
             for (int o = 1; o < 64; o <<= 1) {
@@ -667,11 +694,8 @@ __device__ __attribute__((noinline)) void build_pairs_lds(uint32_t LDS *pend, ui
                 if (lane >= o) incl += u;
             }
             uint32_t run = incl - sum;
-            for (int i = 0; i < kPer; ++i) {
-                const uint32_t v = pend[lane * kPer + i];
-                pend[lane * kPer + i] = run;
-                run += v;
-            }
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) { pend[lane * kPer + i] = run; run += v[i]; }
             __syncthreads();
         }
     }
