@@ -779,13 +779,13 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     c.pair_cap = 0;
     if (s->ex.engine != GRAND_ENGINE_PLAIN && s->search != kML) {
         const size_t base = grand_fused_lds_bytes(c.cap, c.lscores != nullptr) +
-                            4 * ((size_t)s->code->search_lds_words + (size_t)c.claim_batch * c.stage_words) +
+                            4 * ((size_t)((s->code->search_lds_words + 3) & ~3) + (size_t)c.claim_batch * c.stage_words) +
                             4 * ((size_t)kPairBitmapWords + kPairBuckets);
         const size_t budget = std::min<size_t>(65536, kLdsPerCu / (size_t)std::max(wpc, 1));
         const long long all = (long long)c.cap * (c.cap - 1) / 2;
         if (budget > base) {
             const long long room = (long long)((budget - base) / 8);
-            const long long pc = std::min(room, std::min(all, 65535LL));
+            const long long pc = std::min(room, std::min(all, (long long)kPairNone));
             if (pc >= 256) c.pair_cap = (int)pc;
         }
     }

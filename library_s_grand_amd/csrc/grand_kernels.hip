@@ -169,7 +169,7 @@ struct WaveLds {
     unsigned long long LDS *ht;   // (signature << 32) | position, empty = kHtEmpty
     uint32_t LDS *stage;       // search kernel: the claimed shots, DevCall::stage_words each
     uint32_t LDS *pbm;         // pair table (search kernel, pair_cap > 0): key filter [kPairBitmapWords],
-    uint32_t LDS *pend;        //   bucket ends [kPairBuckets], keys and (y | x << 16) values, bucket-sorted
+    uint32_t LDS *phead;       //   bucket chain heads [kPairBuckets], keys and (y | x << 10 | next << 20)
     uint32_t LDS *pkey;
     uint32_t LDS *pval;
     uint32_t LDS *sw;  // syndrome words of the shot being searched, kMaxWords
@@ -213,7 +213,9 @@ __device__ __forceinline__ WaveLds carve_lds(int cap, bool ml, int code_words = 
     L.sw = base;
     L.sw2 = L.sw + kMaxWords;
     L.ncr = (unsigned long long LDS *)(base + kLdsNcrOffset);
-    uint32_t LDS *p = base + kLdsCodeOffset + code_words;
+    // the staged code is padded to 4 words, so the 64-bit hash table and the
+    // pair table that follow are 16-byte aligned
+    uint32_t LDS *p = base + kLdsCodeOffset + ((code_words + 3) & ~3);
     L.ht = (unsigned long long LDS *)p; p += 2 * ht_capacity(cap) + bitmap_words(cap);
     L.A = (int LDS *)p; p += cap;
     L.sigA = p; p += cap;
@@ -227,11 +229,11 @@ __device__ __forceinline__ WaveLds carve_lds(int cap, bool ml, int code_words = 
     }
     if (pair_cap > 0) {
         L.pbm = p; p += kPairBitmapWords;
-        L.pend = p; p += kPairBuckets;
+        L.phead = p; p += kPairBuckets;
         L.pkey = p; p += pair_cap;
         L.pval = p; p += pair_cap;
     } else {
-        L.pbm = nullptr; L.pend = nullptr; L.pkey = nullptr; L.pval = nullptr;
+        L.pbm = nullptr; L.phead = nullptr; L.pkey = nullptr; L.pval = nullptr;
     }
     L.stage = batch_words > 0 ? p : nullptr;
     return L;
@@ -618,91 +620,81 @@ __device__ __attribute__((noinline)) PrefHit scan_prefix_learned(int w, const De
 // a pattern-by-pattern walk); C(tam-2, w-2) stems replace C(tam-1, w-1)
 // prefixes.
 
-// Pairs of A[0..tam) in bucket order (bucket = key bits 16..26): after the
-// build pend[b] is the end of bucket b (its start is pend[b-1], or 0), and
-// pbm has the bit of every key's low 15 bits.  Each lane takes a contiguous
-// block of the C(tam, 2) pairs in lexicographic order.
-__device__ __attribute__((noinline)) void build_pairs_lds(uint32_t LDS *pend, uint32_t LDS *pbm, uint32_t LDS *pkey,
+// Pairs of A[0..tam) in chained buckets (bucket = key bits 16..26): phead[b] is
+// the first entry of bucket b (kPairNone = empty); entry p holds the key in
+// pkey[p] and y | x << 10 | next << 20 in pval[p] (tam <= 1024, at most
+// kPairNone entries); pbm has the bit of every key's low 15 bits.  Each lane
+// takes a contiguous block of the C(tam, 2) pairs in lexicographic order and
+// pushes each on its bucket's chain with one atomic exchange.
+__device__ __attribute__((noinline)) void build_pairs_lds(uint32_t LDS *phead, uint32_t LDS *pbm, uint32_t LDS *pkey,
                                                         uint32_t LDS *pval, const uint32_t LDS *sg, int tam) {
     const int lane = lane_id();
-    for (int i = lane; i < kPairBuckets; i += 64) pend[i] = 0u;
-    for (int i = lane; i < kPairBitmapWords; i += 64) pbm[i] = 0u;
+    static_assert(kPairBuckets % 256 == 0 && kPairBitmapWords % 256 == 0, "clearing in 4-word stores");
+    const uint64_t none2 = ((uint64_t)kPairNone << 32) | kPairNone;
+#pragma unroll
+    for (int i = 0; i < kPairBuckets / 256; ++i)
+        *reinterpret_cast<uint64_t LDS *>(phead + 4 * (lane + 64 * i)) = none2,
+        *reinterpret_cast<uint64_t LDS *>(phead + 4 * (lane + 64 * i) + 2) = none2;
+#pragma unroll
+    for (int i = 0; i < kPairBitmapWords / 256; ++i)
+        *reinterpret_cast<uint64_t LDS *>(pbm + 4 * (lane + 64 * i)) = 0ull,
+        *reinterpret_cast<uint64_t LDS *>(pbm + 4 * (lane + 64 * i) + 2) = 0ull;
     __syncthreads();
     const int P = tam * (tam - 1) / 2;
     const int per = (P + 63) / 64;
     const int k0 = lane * per < P ? lane * per : P;
     const int k1 = k0 + per < P ? k0 + per : P;
-    // first pair of the block: rows y hold tam - 1 - y pairs
-    int y0 = 0, rs = 0;
-    while (y0 < tam - 1 && rs + (tam - 1 - y0) <= k0) { rs += tam - 1 - y0; ++y0; }
-    const int x0 = y0 + 1 + (k0 - rs);
+    // first pair of the block: rows y hold tam - 1 - y pairs and row y starts at
+    // pair rs(y) = y (2 tam - 1 - y) / 2; y0 = the last row starting at or before k0
+    // (a square root, then exact integer correction)
+    auto rs = [tam](int y) { return y * (2 * tam - 1 - y) / 2; };
+    const float bq = (float)(2 * tam - 1);
+    int y0 = (int)((bq - sqrtf(fmaxf(bq * bq - 8.0f * (float)k0, 0.0f))) * 0.5f);
+    y0 = y0 < 0 ? 0 : (y0 > tam - 2 ? tam - 2 : y0);
+    while (y0 > 0 && rs(y0) > k0) --y0;
+    while (y0 < tam - 2 && rs(y0 + 1) <= k0) ++y0;
+    int y = y0, x = y0 + 1 + (k0 - rs(y0));
     constexpr int U = 8;                         // pairs per step, their LDS reads and atomics issued together
-    for (int pass = 0; pass < 2; ++pass) {
-        int y = y0, x = x0;
-        for (int k = k0; k < k1; k += U) {
-            int yy[U], xx[U];
+    for (int k = k0; k < k1; k += U) {
+        int yy[U], xx[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                yy[u] = y; xx[u] = x;
-                const bool wrap = x + 1 == tam;
-                y = wrap ? y + 1 : y;
-                x = wrap ? y + 1 : x + 1;
-            }
-            uint32_t key[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const bool ok = k + u < k1;
-                key[u] = ok ? (sg[yy[u]] ^ sg[xx[u]]) : 0u;
-            }
-            if (pass == 0) {
-#pragma unroll
-                for (int u = 0; u < U; ++u)
-                    if (k + u < k1) {
-                        __hip_atomic_fetch_add(&pend[(key[u] >> 16) & (unsigned)(kPairBuckets - 1)], 1u,
-                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        __hip_atomic_fetch_or(&pbm[(key[u] & 0x7FFFu) >> 5], 1u << (key[u] & 31u), __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_WORKGROUP);
-                    }
-            } else {
-                unsigned pos[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u)
-                    pos[u] = (k + u < k1) ? __hip_atomic_fetch_add(&pend[(key[u] >> 16) & (unsigned)(kPairBuckets - 1)],
-                                                                   1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
-                                          : 0u;
-#pragma unroll
-                for (int u = 0; u < U; ++u)
-                    if (k + u < k1) {
-                        pkey[pos[u]] = key[u];
-                        pval[pos[u]] = (uint32_t)yy[u] | ((uint32_t)xx[u] << 16);
-                    }
-            }
+        for (int u = 0; u < U; ++u) {
+            yy[u] = y; xx[u] = x;
+            const bool wrap = x + 1 == tam;
+            y = wrap ? y + 1 : y;
+            x = wrap ? y + 1 : x + 1;
         }
-        __syncthreads();
-        if (pass == 0) {
-            // exclusive prefix sum of the bucket counts: 32 consecutive buckets per lane
-            // (unrolled, so the reads are issued together)
-            constexpr int kPer = kPairBuckets / 64;
-            uint32_t v[kPer];
-            uint32_t sum = 0;
+        // unconditional reads (clamped indices) so that nothing waits inside a divergent branch
+        uint32_t key[U];
 #pragma unroll
-            for (int i = 0; i < kPer; ++i) { v[i] = pend[lane * kPer + i]; sum += v[i]; }
-            uint32_t incl = sum;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t u = (uint32_t)__shfl_up((int)incl, o, 64);
-                if (lane >= o) incl += u;
-            }
-            uint32_t run = incl - sum;
-#pragma unroll
-            for (int i = 0; i < kPer; ++i) { pend[lane * kPer + i] = run; run += v[i]; }
-            __syncthreads();
+        for (int u = 0; u < U; ++u) {
+            const bool ok = k + u < k1;
+            key[u] = sg[ok ? yy[u] : 0] ^ sg[ok ? xx[u] : 1];
         }
+        uint32_t nxt[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool ok = k + u < k1;
+            if (ok)
+                __hip_atomic_fetch_or(&pbm[(key[u] & 0x7FFFu) >> 5], 1u << (key[u] & 31u), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+            // past the block's end the exchange puts back what it reads (no change)
+            uint32_t LDS *hp = &phead[ok ? (key[u] >> 16) & (unsigned)(kPairBuckets - 1) : 0u];
+            nxt[u] = ok ? __hip_atomic_exchange(hp, (uint32_t)(k + u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                        : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (k + u < k1) {
+                pkey[k + u] = key[u];
+                pval[k + u] = (uint32_t)yy[u] | ((uint32_t)xx[u] << 10) | (nxt[u] << 20);
+            }
     }
+    __syncthreads();
 }
 
 __device__ __forceinline__ void build_pairs(const WaveLds &L, const uint32_t LDS *sg, int tam) {
-    build_pairs_lds(L.pend, L.pbm, L.pkey, L.pval, sg, tam);
+    build_pairs_lds(L.phead, L.pbm, L.pkey, L.pval, sg, tam);
 }
 
 // Lexicographically smallest pair (y, x), y > last, with key T whose pattern
@@ -710,16 +702,17 @@ __device__ __forceinline__ void build_pairs(const WaveLds &L, const uint32_t LDS
 template <int WS>
 __device__ __attribute__((noinline)) int probe_pairs(const DevCode &code, const uint32_t LDS *s, uint32_t T, int last,
                                                     const int LDS *act, const uint32_t LDS *pkey,
-                                                    const uint32_t LDS *pval, const uint32_t LDS *pend, int p0,
+                                                    const uint32_t LDS *pval, const uint32_t LDS *phead, int p0,
                                                     int p1, int p2, int p3) {
     const int pc[4] = {p0, p1, p2, p3};
-    const unsigned b = (T >> 16) & (unsigned)(kPairBuckets - 1);
-    const int e0 = b ? (int)pend[b - 1] : 0, e1 = (int)pend[b];
+    unsigned e = phead[(T >> 16) & (unsigned)(kPairBuckets - 1)];
     int best = -1;
-    for (int e = e0; e < e1; ++e) {
-        if (pkey[e] != T) continue;
+    while (e != kPairNone) {
         const uint32_t v = pval[e];
-        const int y = (int)(v & 0xFFFFu), x = (int)(v >> 16);
+        const uint32_t key = pkey[e];
+        e = v >> 20;
+        if (key != T) continue;
+        const int y = (int)(v & 0x3FFu), x = (int)((v >> 10) & 0x3FFu);
         if (y <= last) continue;
         if (best >= 0) {
             const int by = best >> 16, bx = best & 0xFFFF;
@@ -742,7 +735,7 @@ template <int WS>
 __device__ __forceinline__ PrefHit scan_stems_first(const DevCode &code, const uint32_t LDS *s, uint32_t t0,
                                                     uint64_t q0, int len, int tam, const int LDS *act,
                                                     const uint32_t LDS *sg, const uint32_t LDS *pkey,
-                                                    const uint32_t LDS *pval, const uint32_t LDS *pend,
+                                                    const uint32_t LDS *pval, const uint32_t LDS *phead,
                                                     const uint32_t LDS *pbm) {
     PrefHit h{~0ull, 0.0f, -1};
     if (len <= 0) return h;
@@ -760,7 +753,7 @@ __device__ __forceinline__ PrefHit scan_stems_first(const DevCode &code, const u
         const int last = c[WS - 1];
         const uint32_t nxt = sg[last + 1];            // last + 1 <= M = tam - 2
         if ((pbm[(t & 0x7FFFu) >> 5] >> (t & 31u)) & 1u) {
-            const int yx = probe_pairs<WS>(code, s, t, last, act, pkey, pval, pend, c[0],
+            const int yx = probe_pairs<WS>(code, s, t, last, act, pkey, pval, phead, c[0],
                                            WS > 1 ? c[WS > 1 ? 1 : 0] : 0, WS > 2 ? c[WS > 2 ? 2 : 0] : 0,
                                            WS > 3 ? c[WS > 3 ? 3 : 0] : 0);
             if (yx >= 0) {
@@ -804,12 +797,12 @@ __device__ __attribute__((noinline)) PrefHit scan_stems_w(int w, const DevCode &
                                                          uint32_t t0, uint64_t q0, int len, int tam,
                                                          const int LDS *act, const uint32_t LDS *sg,
                                                          const uint32_t LDS *pkey, const uint32_t LDS *pval,
-                                                         const uint32_t LDS *pend, const uint32_t LDS *pbm) {
+                                                         const uint32_t LDS *phead, const uint32_t LDS *pbm) {
     switch (w) {
-        case 3: return scan_stems_first<1>(code, s, t0, q0, len, tam, act, sg, pkey, pval, pend, pbm);
-        case 4: return scan_stems_first<2>(code, s, t0, q0, len, tam, act, sg, pkey, pval, pend, pbm);
-        case 5: return scan_stems_first<3>(code, s, t0, q0, len, tam, act, sg, pkey, pval, pend, pbm);
-        default: return scan_stems_first<4>(code, s, t0, q0, len, tam, act, sg, pkey, pval, pend, pbm);
+        case 3: return scan_stems_first<1>(code, s, t0, q0, len, tam, act, sg, pkey, pval, phead, pbm);
+        case 4: return scan_stems_first<2>(code, s, t0, q0, len, tam, act, sg, pkey, pval, phead, pbm);
+        case 5: return scan_stems_first<3>(code, s, t0, q0, len, tam, act, sg, pkey, pval, phead, pbm);
+        default: return scan_stems_first<4>(code, s, t0, q0, len, tam, act, sg, pkey, pval, phead, pbm);
     }
 }
 
@@ -926,7 +919,7 @@ __device__ unsigned long long task_chunk(const DevCode &code, TaskSlot *T, const
                                         int tam, bool ml, unsigned ch, const int LDS *act, const uint32_t LDS *sg,
                                         const float LDS *llr, const unsigned long long LDS *ht, unsigned hmask,
                                         const uint32_t LDS *pkey, const uint32_t LDS *pval,
-                                        const uint32_t LDS *pend, const uint32_t LDS *pbm) {
+                                        const uint32_t LDS *phead, const uint32_t LDS *pbm) {
     const int lane = lane_id();
     const int nclass = (int)uni((unsigned)ld_relaxed(&T->nclass));
     const unsigned wpack = uni((unsigned)ld_relaxed(&T->wpack)), ppack = uni((unsigned)ld_relaxed(&T->ppack));
@@ -954,7 +947,7 @@ __device__ unsigned long long task_chunk(const DevCode &code, TaskSlot *T, const
     const uint64_t q0 = base + (uint64_t)lane * (uint64_t)R;
     const int len = q0 >= NP ? 0 : (int)((NP - q0) < (uint64_t)R ? (NP - q0) : (uint64_t)R);
     const PrefHit h = ml ? scan_prefix_w<1>(w, code, s, t0, q0, len, tam, act, sg, llr, ht, hmask)
-                    : pairs ? scan_stems_w(w, code, s, t0, q0, len, tam, act, sg, pkey, pval, pend, pbm)
+                    : pairs ? scan_stems_w(w, code, s, t0, q0, len, tam, act, sg, pkey, pval, phead, pbm)
                             : scan_prefix_w<0>(w, code, s, t0, q0, len, tam, act, sg, llr, ht, hmask);
     unsigned long long out = ~0ull;
     if (!ml) {
@@ -1007,7 +1000,7 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
                            int tam, bool ml, int nclass, unsigned wpack, unsigned ppack, const int LDS *act,
                            const uint32_t LDS *sg, const float LDS *llr, const unsigned long long LDS *ht, unsigned hmask,
                            const uint32_t LDS *pkey, const uint32_t LDS *pval,
-                           const uint32_t LDS *pend, const uint32_t LDS *pbm, unsigned long long &best) {
+                           const uint32_t LDS *phead, const uint32_t LDS *pbm, unsigned long long &best) {
     const int lane = lane_id();
     // find a free slot: 64 candidates per step, lane 0 takes the first free one
     int slot = -1;
@@ -1066,7 +1059,7 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
         const unsigned ch = claim_chunk(T, &call.task_counters[0], g);
         if (ch == ~0u) break;
         const unsigned long long res =
-            task_chunk(code, T, s, t0, tam, ml, ch, act, sg, llr, ht, hmask, pkey, pval, pend, pbm);
+            task_chunk(code, T, s, t0, tam, ml, ch, act, sg, llr, ht, hmask, pkey, pval, phead, pbm);
         trace_chunk(call, 0, slot, uid, ch, (int)(wpack & 15u), tam, nclass, nch, t0, s, act, sg, res);
         wait_mem();                               // the chunk's result atomics complete before it counts as done
         if (lane == 0) atomicAdd(&T->done, 1u);
@@ -1126,7 +1119,7 @@ __device__ __forceinline__ bool help_task(const DevCode &code, const DevCall &ca
         ctask = (int)uid;
     }
     const unsigned long long res = task_chunk(code, T, s, t0, tam, ml, ch, act, sg, ml ? L.llrP : nullptr, L.ht,
-                                              hmask, L.pkey, L.pval, L.pend, L.pbm);
+                                              hmask, L.pkey, L.pval, L.phead, L.pbm);
     trace_chunk(call, 1, slot, uid, ch, (int)(wpack & 15u), tam, nclass, ld_relaxed(&T->nchunks), t0, s, act, sg,
                 res);
     wait_mem();                                   // the chunk's result atomics complete before it counts as done
@@ -1143,7 +1136,7 @@ __device__ uint64_t class_first_hit(const DevCode &code, const DevCall &call, in
                                     uint32_t t0, int w, int tam, uint64_t Nw, const int LDS *act,
                                     const uint32_t LDS *sg, const unsigned long long LDS *ht, unsigned hmask,
                                     bool pairs, const uint32_t LDS *pkey, const uint32_t LDS *pval,
-                                    const uint32_t LDS *pend, const uint32_t LDS *pbm) {
+                                    const uint32_t LDS *phead, const uint32_t LDS *pbm) {
     const int lane = lane_id();
     if (w == 1) return weight_one<0>(code, s, t0, tam, act, sg, nullptr).rank;
     // units: stems with the pair table, else prefixes
@@ -1155,7 +1148,7 @@ __device__ uint64_t class_first_hit(const DevCode &code, const DevCall &call, in
         const int R = (int)(r64 < (uint64_t)call.rmax ? r64 : (uint64_t)call.rmax);
         const uint64_t q0 = pos + (uint64_t)lane * (uint64_t)R;
         const int len = q0 >= NP ? 0 : (int)((NP - q0) < (uint64_t)R ? (NP - q0) : (uint64_t)R);
-        const PrefHit h = pairs ? scan_stems_w(w, code, s, t0, q0, len, tam, act, sg, pkey, pval, pend, pbm)
+        const PrefHit h = pairs ? scan_stems_w(w, code, s, t0, q0, len, tam, act, sg, pkey, pval, phead, pbm)
                                 : scan_prefix_w<0>(w, code, s, t0, q0, len, tam, act, sg, nullptr, ht, hmask);
         const uint64_t b = ballot(h.j >= 0);
         if (b) return (uint64_t)__shfl((long long)h.rank, __ffsll((long long)b) - 1, 64);
@@ -1266,7 +1259,7 @@ __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, int sho
                 if (need_ht && !have_table) { hmask = build_ht(L.ht, L.sigA, tam); have_table = true; }
                 unsigned long long best;
                 if (run_shared(code, call, shot, s, t0, tam, false, nc, wpack, ppack, L.A, L.sigA, nullptr, L.ht, hmask,
-                               L.pkey, L.pval, L.pend, L.pbm, best)) {
+                               L.pkey, L.pval, L.phead, L.pbm, best)) {
                     uint64_t span = 0;
                     for (int k = kc; k < kend; ++k) span += N[k];
                     if (best != ~0ull) {
@@ -1295,7 +1288,7 @@ __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, int sho
             }
             const uint64_t r =
                 class_first_hit(code, call, shot, s, t0, w, tam, N[kc], L.A, L.sigA, L.ht, hmask, pairs, L.pkey,
-                                L.pval, L.pend, L.pbm);
+                                L.pval, L.phead, L.pbm);
             if (r != ~0ull) {
                 hit = true;
                 hit_lane = 0;
@@ -1468,11 +1461,11 @@ __device__ __attribute__((noinline)) bool rref_degenerate(const DevCode &code, u
     const int lane = lane_id();
     const int M32 = code.M32;
     uint32_t acc[2] = {0u, 0u};
-    uint32_t pend[2] = {dl0, dl1};
+    uint32_t rest[2] = {dl0, dl1};
     for (;;) {
         int c = -1;
-        if (pend[0]) { c = lane * 32 + __ffs(pend[0]) - 1; pend[0] &= pend[0] - 1u; }
-        else if (pend[1]) { c = (lane + 64) * 32 + __ffs(pend[1]) - 1; pend[1] &= pend[1] - 1u; }
+        if (rest[0]) { c = lane * 32 + __ffs(rest[0]) - 1; rest[0] &= rest[0] - 1u; }
+        else if (rest[1]) { c = (lane + 64) * 32 + __ffs(rest[1]) - 1; rest[1] &= rest[1] - 1u; }
         if (ballot(c >= 0) == 0ull) break;
         const int r = (c >= 0) ? (code.lds_post == 1 ? code_lds_pivrow(code)[c] : code.pivrow[c]) : -1;
         uint64_t sel = ballot(r >= 0);
@@ -2565,7 +2558,7 @@ GRAND_INTERNAL hipError_t grand_launch_triage(const DevCode *code, const DevCall
 GRAND_INTERNAL hipError_t grand_launch_fused(const DevCode *code, const DevCall *call, const DevCall *hcall, int code_lds_words,
                               int grid, hipStream_t st) {
     const size_t lds = grand_fused_lds_bytes(hcall->cap, hcall->kind == grand::kML || hcall->lscores != nullptr) +
-                       sizeof(uint32_t) * ((size_t)code_lds_words + (size_t)hcall->claim_batch * hcall->stage_words +
+                       sizeof(uint32_t) * ((size_t)((code_lds_words + 3) & ~3) + (size_t)hcall->claim_batch * hcall->stage_words +
                                            (hcall->pair_cap > 0 ? (size_t)grand::kPairBitmapWords + grand::kPairBuckets +
                                                                       2 * (size_t)hcall->pair_cap
                                                                 : 0));
