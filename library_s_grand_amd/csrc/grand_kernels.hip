@@ -1171,6 +1171,21 @@ __device__ __forceinline__ uint64_t weighted_prefix(const uint64_t *N, const int
     return acc;
 }
 
+// Work left in a run of classes (weights in wl4, 4 bits each, from the current
+// class on; the run ends at the list's end or at weight 1), in prefix steps:
+// the classes after the current one are searched anyway unless it hits, so the
+// run is shared as one task when this exceeds the split threshold.
+__device__ __attribute__((noinline)) uint64_t rest_work(const DevCode &code, const DevCall &call, uint64_t wl4, int tam) {
+    uint64_t rest = 0;
+    for (; (wl4 & 15u) >= 2u; wl4 >>= 4) {
+        const int w = (int)(wl4 & 15u);
+        if (w > tam) continue;
+        const bool pk = use_pairs(code, call, w, tam);
+        rest += class_units(code, tam, w, pk) * (pk ? (uint64_t)kStemCost : 1u);
+    }
+    return rest;
+}
+
 // ------------------------------------------------------------ first-hit search
 // Weight list wl[0..nw) searched in order over A[0..tam) (grand_decoder.cu:295-350,
 // :534-588); on a hit writes the decoded columns to win and returns true; count
@@ -1189,6 +1204,9 @@ __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, int sho
         N[k] = (k < nw) ? ncr_w(code, tam, wl[k]) : 0ull;
         total += N[k];
     }
+    uint64_t wl_packed = 0;        // the weight list, 4 bits per class
+#pragma unroll
+    for (int k = 0; k < 2 * kMaxW; ++k) if (k < nw) wl_packed |= (uint64_t)wl[k] << (4 * k);
     uint64_t class_start = 0;
     int kc = 0;
     bool hit = false;
@@ -1236,8 +1254,7 @@ __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, int sho
         } else {
             const int w = wl[kc];
             const bool pairs = use_pairs(code, call, w, tam);
-            const uint64_t units = class_units(code, tam, w, pairs);
-            if (w >= 2 && call.split_threshold && units * (pairs ? (uint64_t)kStemCost : 1u) > call.split_threshold) {
+            if (w >= 2 && call.split_threshold && rest_work(code, call, wl_packed >> (4 * kc), tam) > call.split_threshold) {
                 // this class and the following ones of weight >= 2 as one shared task:
                 // no barrier between the classes, and the task's lowest matching
                 // pattern index is the first hit over them in list order
@@ -1489,8 +1506,17 @@ __device__ __attribute__((noinline)) bool rref_degenerate(const DevCode &code, u
 // for the CSS family, :1623-1648 for the DEM family).  The shot's packed errors
 // (cc) or observed logicals (cln) come from its staged copy in LDS (`slot`, laid
 // out as DevCall::stage_words describes) or, with slot null, from global memory.
-__device__ void finalize_shot(const DevCode &code, const DevCall &call, int shot, const ShotOutcome &o,
-                              const uint32_t LDS *slot) {
+__device__ __attribute__((noinline)) void finalize_shot_v(const DevCode &code, const DevCall &call, int shot, int found,
+                                                          int fw, int iteration, int tam, int hibrido, int w0, int w1,
+                                                          int w2, int w3, int w4, int w5, int w6, int w7,
+                                                          unsigned long long count, unsigned long long wops,
+                                                          unsigned long long t0, unsigned long long t1,
+                                                          const uint32_t LDS *slot) {
+    ShotOutcome o;
+    o.found = found; o.fw = fw; o.iteration = iteration; o.tam = tam; o.hibrido = hibrido;
+    o.win[0] = w0; o.win[1] = w1; o.win[2] = w2; o.win[3] = w3; o.win[4] = w4; o.win[5] = w5; o.win[6] = w6;
+    o.win[7] = w7;
+    o.count = count; o.wops = wops; o.t0 = t0; o.t1 = t1;
     const int lane = lane_id();
     const int M32 = code.M32;
     uint32_t dw[2] = {0u, 0u};   // decoded pattern words lane + 64 q, q < 2 (m <= 4096)
@@ -1579,6 +1605,15 @@ __device__ void finalize_shot(const DevCode &code, const DevCall &call, int shot
         call.stats[3 * (size_t)shot + 1] = o.t1;
         call.stats[3 * (size_t)shot + 2] = o.wops;
     }
+}
+
+// The outcome goes to the out-of-line finalizer as scalars (registers), so the
+// shot's outcome never needs stack memory whatever the inliner decides.
+static_assert(kMaxWin == 8, "finalize_shot_v takes eight decoded columns");
+__device__ __forceinline__ void finalize_shot(const DevCode &code, const DevCall &call, int shot, const ShotOutcome &o,
+                                              const uint32_t LDS *slot) {
+    finalize_shot_v(code, call, shot, o.found, o.fw, o.iteration, o.tam, o.hibrido, o.win[0], o.win[1], o.win[2],
+                    o.win[3], o.win[4], o.win[5], o.win[6], o.win[7], o.count, o.wops, o.t0, o.t1, slot);
 }
 
 // ------------------------------------------------------------ learned pass
