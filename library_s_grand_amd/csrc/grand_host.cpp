@@ -33,7 +33,8 @@ extern "C" {
 GRAND_INTERNAL size_t grand_fused_lds_bytes(int cap, int ml);
 GRAND_INTERNAL hipError_t grand_launch_fill(unsigned *p, unsigned v, int n, hipStream_t st);
 GRAND_INTERNAL hipError_t grand_launch_prep(const grand::DevCall *hcall, grand::DevCall *dcall, unsigned *counters, hipStream_t st);
-GRAND_INTERNAL hipError_t grand_launch_triage(const grand::DevCode *code, const grand::DevCall *call, int grid, hipStream_t st);
+GRAND_INTERNAL hipError_t grand_launch_triage(const grand::DevCode *code, const grand::DevCall *call, int grid,
+                                              int block_words, hipStream_t st);
 GRAND_INTERNAL hipError_t grand_launch_fused(const grand::DevCode *code, const grand::DevCall *call, const grand::DevCall *hcall,
                               int code_lds_words, int grid, hipStream_t st);
 GRAND_INTERNAL hipError_t grand_launch_expand(const uint32_t *packed, int words, int cols, int rows, int *out, hipStream_t st);
@@ -805,7 +806,8 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     const DevCode *dcode = s->code->dcode.as<DevCode>();
     const DevCall *dcall = s->dcall.as<DevCall>();
     if (s->timing) HIPX(hipEventRecord(s->ev[0], st));
-    HIPX(grand_launch_triage(dcode, dcall, tgrid, st));
+    const int block_words = std::max(c.input_errors ? code.M32 : code.D, s->family == kCLN ? code.NL32 : 0);
+    HIPX(grand_launch_triage(dcode, dcall, tgrid, block_words, st));
     if (s->timing) HIPX(hipEventRecord(s->ev[1], st));
     if (s->ex.engine == GRAND_ENGINE_PLAIN) {
         int rc = run_plain(s, st, num_shots);

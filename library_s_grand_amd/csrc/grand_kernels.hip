@@ -1844,9 +1844,6 @@ __device__ __forceinline__ void write_zero_result(GrandDecoderResult *dst, int o
     p[8] = (uint32_t)-2;            // hibrido: no search
 }
 
-// Stages a 64-shot x 64-column tile of int32 rows through LDS (coalesced loads,
-// one row per step) and returns this lane's 64 values packed into two words;
-// `bad` collects any value other than 0/1.
 // ------------------------------------------------------------ lane search
 // A shot searched by its own lane in the triage kernel: the first iteration of
 // decode_shot_wave (threshold max_soma) when its active list has at most
@@ -2039,37 +2036,75 @@ __device__ __forceinline__ bool lane_search_any(const DevCode &code, const DevCa
     }
 }
 
-__device__ __forceinline__ void pack_tile(const int *rows, int stride, int base, int cnt, int c0, int cols,
-                                          uint32_t LDS *tile, uint32_t &w0, uint32_t &w1, unsigned &bad) {
+// Row and column of entry e of a row-major block with `ncols` columns, without
+// an integer division: e < 2^18, so e * (1/ncols) in float is off by at most one.
+__device__ __forceinline__ void entry_rc(int e, int ncols, float inv, int &r, int &c) {
+    r = (int)((float)e * inv);
+    c = e - r * ncols;
+    if (c < 0) { --r; c += ncols; }
+    if (c >= ncols) { ++r; c -= ncols; }
+}
+
+// Packs `cnt` consecutive int32 rows of `ncols` entries (row-major, contiguous,
+// starting at row `base`) into bits[64][W32]: bit c of word c / 32 of row r is
+// set iff entry (r, c) is odd; `bad` collects any value other than 0/1.  The
+// block is read front to back as 16-byte vectors (fully coalesced), a batch of
+// kPackBatch per lane issued together.  Set entries are rare, so only a vector
+// holding one works out its row and column and sets its bits with one LDS OR
+// (the four entries of a vector share a row and a word when ncols % 4 == 0).
+constexpr int kPackBatch = 12;
+__device__ __forceinline__ void pack_rows(const int *rows, int ncols, int base, int cnt, int W32, uint32_t LDS *bits,
+                                          unsigned &bad) {
     const int lane = lane_id();
-    const bool vec = c0 + 64 <= cols && (stride & 3) == 0 && (((uintptr_t)rows) & 15) == 0;
-    if (vec) {
-        // 16 lanes per row, 16 bytes per lane: 16 independent loads cover the tile
-        const int rr = lane >> 4, cc = (lane & 15) * 4;
+    __syncthreads();                              // earlier readers of `bits` are done
+    for (int i = lane; i < 64 * W32; i += 64) bits[i] = 0u;
+    __syncthreads();
+    const int *blk = rows + (size_t)base * ncols;
+    const int total = cnt * ncols;                // <= 64 * kMaxM
+    const float inv = 1.0f / (float)ncols;
+    if ((ncols & 3) == 0 && (((uintptr_t)blk) & 15) == 0) {
+        const int4 *v4 = reinterpret_cast<const int4 *>(blk);
+        const int nv = total >> 2;
+        for (int q0 = 0; q0 < nv; q0 += 64 * kPackBatch) {
+            int4 v[kPackBatch];
 #pragma unroll
-        for (int it = 0; it < 16; ++it) {
-            const int j = it * 4 + rr;
-            int4 v = make_int4(0, 0, 0, 0);
-            if (j < cnt) v = *reinterpret_cast<const int4 *>(rows + (size_t)(base + j) * stride + c0 + cc);
-            uint32_t LDS *d = tile + j * 65 + cc;
-            d[0] = (uint32_t)v.x; d[1] = (uint32_t)v.y; d[2] = (uint32_t)v.z; d[3] = (uint32_t)v.w;
+            for (int k = 0; k < kPackBatch; ++k) {
+                const int q = q0 + 64 * k + lane;
+                v[k] = (q < nv) ? v4[q] : make_int4(0, 0, 0, 0);
+            }
+#pragma unroll
+            for (int k = 0; k < kPackBatch; ++k) {
+                const uint32_t o = (uint32_t)(v[k].x | v[k].y | v[k].z | v[k].w);
+                bad |= o & ~1u;
+                if (o & 1u) {
+                    int r, c;
+                    entry_rc(4 * (q0 + 64 * k + lane), ncols, inv, r, c);
+                    const uint32_t nib = ((uint32_t)v[k].x & 1u) | (((uint32_t)v[k].y & 1u) << 1) |
+                                         (((uint32_t)v[k].z & 1u) << 2) | (((uint32_t)v[k].w & 1u) << 3);
+                    __hip_atomic_fetch_or(&bits[r * W32 + (c >> 5)], nib << (c & 31), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
         }
     } else {
-#pragma unroll 8
-        for (int j = 0; j < 64; ++j) {
-            int v = 0;
-            if (j < cnt && c0 + lane < cols) v = rows[(size_t)(base + j) * stride + c0 + lane];
-            tile[j * 65 + lane] = (uint32_t)v;
-        }
-    }
-    __syncthreads();
-    w0 = 0u; w1 = 0u;
+        for (int q0 = 0; q0 < total; q0 += 64 * kPackBatch) {
+            int v[kPackBatch];
 #pragma unroll
-    for (int k = 0; k < 32; ++k) {
-        const uint32_t a = tile[lane * 65 + k], b = tile[lane * 65 + 32 + k];
-        bad |= (a | b) & ~1u;
-        w0 |= (a & 1u) << k;
-        w1 |= (b & 1u) << k;
+            for (int k = 0; k < kPackBatch; ++k) {
+                const int q = q0 + 64 * k + lane;
+                v[k] = (q < total) ? blk[q] : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < kPackBatch; ++k) {
+                bad |= (uint32_t)v[k] & ~1u;
+                if (v[k] & 1) {
+                    int r, c;
+                    entry_rc(q0 + 64 * k + lane, ncols, inv, r, c);
+                    __hip_atomic_fetch_or(&bits[r * W32 + (c >> 5)], 1u << (c & 31), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+        }
     }
     __syncthreads();
 }
@@ -2078,13 +2113,16 @@ __device__ __forceinline__ void pack_tile(const int *rows, int stride, int base,
 // syndromes, and logicals), forms s = H e and sig(s) as XORs over the set error
 // bits (both maps are GF(2)-linear), writes the result of zero-syndrome shots
 // and appends the others to the worklist with one atomic per 64 shots.
+// LDS: the lane search's [kLaneMax][64] positions and signatures, then the
+// packed block, 64 rows of the larger of the input row's words (M32 for errors,
+// D for syndromes) and, for the DEM family, the NL32 words of the logicals.
 __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ codep, const DevCall *__restrict__ callp) {
     const DevCode &code = *codep;
     const DevCall &call = *callp;
     const int lane = lane_id();
-    uint32_t LDS *tile = lds_base();   // [64][65]
-    int LDS *LA = (int LDS *)(tile + 64 * 65);          // lane search: [kLaneMax][64] positions
-    uint32_t LDS *LS = tile + 64 * 65 + kLaneMax * 64;  //              [kLaneMax][64] signatures
+    int LDS *LA = (int LDS *)lds_base();
+    uint32_t LDS *LS = lds_base() + kLaneMax * 64;
+    uint32_t LDS *bits = lds_base() + 2 * kLaneMax * 64;
     const int groups = (call.num_shots + 63) / 64;
     unsigned bad = 0;
     for (int g = blockIdx.x; g < groups; g += gridDim.x) {
@@ -2098,60 +2136,42 @@ __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ 
         uint32_t sg = 0, dgv = 0;
         int ow = 0;
         if (call.input_errors) {
-            for (int c0 = 0; c0 < code.m; c0 += 64) {
-                uint32_t w[2];
-                pack_tile(call.input, code.m, base, cnt, c0, code.m, tile, w[0], w[1], bad);
-                const int wd = c0 >> 5;
-                if (valid) {
-                    call.e_packed[(size_t)shot * code.M32 + wd] = w[0];
-                    if (wd + 1 < code.M32) call.e_packed[(size_t)shot * code.M32 + wd + 1] = w[1];
-                }
-                ow += __popc(w[0]) + __popc(w[1]);
+            const int M32 = code.M32;
+            pack_rows(call.input, code.m, base, cnt, M32, bits, bad);
+            // the block's packed rows are consecutive in e_packed
+            for (int i = lane; i < cnt * M32; i += 64) call.e_packed[(size_t)base * M32 + i] = bits[i];
+            for (int wd = 0; wd < M32; ++wd) {
+                uint32_t b = bits[lane * M32 + wd];
+                ow += __popc(b);
+                while (b) {
+                    const int c = 32 * wd + __ffs(b) - 1;
+                    b &= b - 1u;
+                    sg ^= code.sig[c];
+                    if (code.dg) dgv ^= code.dg[c];
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    uint32_t bits = w[h];
-                    while (bits) {
-                        const int c = c0 + 32 * h + __ffs(bits) - 1;
-                        bits &= bits - 1u;
-                        sg ^= code.sig[c];
-                        if (code.dg) dgv ^= code.dg[c];
-#pragma unroll
-                        for (int d = 0; d < kMaxWords; ++d)
-                            if (d < code.D) s[d] ^= code.Hcm[(size_t)c * code.D + d];
-                    }
+                    for (int d = 0; d < kMaxWords; ++d)
+                        if (d < code.D) s[d] ^= code.Hcm[(size_t)c * code.D + d];
                 }
             }
         } else {
-            for (int r0 = 0; r0 < code.n; r0 += 64) {
-                uint32_t w[2];
-                pack_tile(call.input, code.n, base, cnt, r0, code.n, tile, w[0], w[1], bad);
-                const int wd = r0 >> 5;
+            const int D = code.D;
+            pack_rows(call.input, code.n, base, cnt, D, bits, bad);
 #pragma unroll
-                for (int d = 0; d < kMaxWords; ++d) {
-                    if (d == wd) s[d] = w[0];
-                    if (d == wd + 1) s[d] = w[1];
-                }
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    uint32_t bits = w[h];
-                    while (bits) {
-                        const int r = r0 + 32 * h + __ffs(bits) - 1;
-                        bits &= bits - 1u;
-                        sg ^= code.rowsig[r];
+            for (int d = 0; d < kMaxWords; ++d) {
+                if (d < D) {
+                    uint32_t b = bits[lane * D + d];
+                    s[d] = b;
+                    while (b) {
+                        sg ^= code.rowsig[32 * d + __ffs(b) - 1];
+                        b &= b - 1u;
                     }
                 }
             }
         }
         if (call.family == kCLN) {
-            for (int q0 = 0; q0 < code.NL; q0 += 64) {
-                uint32_t w0, w1;
-                pack_tile(call.logicals, code.NL, base, cnt, q0, code.NL, tile, w0, w1, bad);
-                const int wd = q0 >> 5;
-                if (valid) {
-                    call.lg_packed[(size_t)shot * code.NL32 + wd] = w0;
-                    if (wd + 1 < code.NL32) call.lg_packed[(size_t)shot * code.NL32 + wd + 1] = w1;
-                }
-            }
+            const int NL32 = code.NL32;
+            pack_rows(call.logicals, code.NL, base, cnt, NL32, bits, bad);
+            for (int i = lane; i < cnt * NL32; i += 64) call.lg_packed[(size_t)base * NL32 + i] = bits[i];
         }
         uint32_t any = 0;
 #pragma unroll
@@ -2584,9 +2604,13 @@ GRAND_INTERNAL hipError_t grand_launch_fill(unsigned *p, unsigned v, int n, hipS
     return hipGetLastError();
 }
 
-GRAND_INTERNAL hipError_t grand_launch_triage(const DevCode *code, const DevCall *call, int grid, hipStream_t st) {
+// `block_words`: words per shot of the triage kernel's packed block, the larger
+// of the input row (M32 words of errors or D words of syndrome) and, for the DEM
+// family, the NL32 words of observed logicals.
+GRAND_INTERNAL hipError_t grand_launch_triage(const DevCode *code, const DevCall *call, int grid, int block_words,
+                                              hipStream_t st) {
     hipLaunchKernelGGL(grand::triage_kernel, dim3(grid), dim3(64),
-                       sizeof(uint32_t) * (64 * 65 + 2 * grand::kLaneMax * 64), st, code, call);
+                       sizeof(uint32_t) * (2 * grand::kLaneMax * 64 + 64 * (size_t)block_words), st, code, call);
     return hipGetLastError();
 }
 
