@@ -34,7 +34,7 @@ GRAND_INTERNAL size_t grand_fused_lds_bytes(int cap, int ml);
 GRAND_INTERNAL hipError_t grand_launch_fill(unsigned *p, unsigned v, int n, hipStream_t st);
 GRAND_INTERNAL hipError_t grand_launch_prep(const grand::DevCall *hcall, grand::DevCall *dcall, unsigned *counters, hipStream_t st);
 GRAND_INTERNAL hipError_t grand_launch_triage(const grand::DevCode *code, const grand::DevCall *call, int grid,
-                                              int block_words, hipStream_t st);
+                                              int block_words, int stage_words, hipStream_t st);
 GRAND_INTERNAL hipError_t grand_launch_fused(const grand::DevCode *code, const grand::DevCall *call, const grand::DevCall *hcall,
                               int code_lds_words, int grid, hipStream_t st);
 GRAND_INTERNAL hipError_t grand_launch_expand(const uint32_t *packed, int words, int cols, int rows, int *out, hipStream_t st);
@@ -447,7 +447,8 @@ struct Workspace {
         if (tasks.bytes == 0) tasks_fresh = true;
         if (!s_packed.ensure(S * c.D * 4) || !e_packed.ensure(S * c.M32 * 4) || !e_aux.ensure(S * 8) ||
             !lg_packed.ensure(S * c.NL32 * 4) ||
-            !s_sig.ensure(S * 4) || !worklist.ensure(S * 4) || !counters.ensure(kCounterBytes) || !dec.ensure(S * c.M32 * 4) ||
+            !s_sig.ensure(S * 4) || !worklist.ensure((S + 63) / 64 * 64 * 4) || !counters.ensure(kCounterBytes) ||
+            !dec.ensure(S * c.M32 * 4) ||
             !declog.ensure(S * c.NL32 * 4) || !tasks.ensure(sizeof(TaskSlot) * kTaskSlots) ||
             !task_arrays.ensure((size_t)kTaskSlots * 3 * std::max(cap, 1) * 4))
             return false;
@@ -456,7 +457,7 @@ struct Workspace {
     }
     size_t bytes_for(const DevCode &c, int shots_, int cap) const {
         const size_t S = (size_t)std::max(shots_, 1);
-        return S * 4 * ((size_t)c.D + 2 * c.M32 + 2 * c.NL32 + 4) + kCounterBytes + sizeof(TaskSlot) * kTaskSlots +
+        return S * 4 * ((size_t)c.D + 2 * c.M32 + 2 * c.NL32 + 4) + 256 + kCounterBytes + sizeof(TaskSlot) * kTaskSlots +
                (size_t)kTaskSlots * 3 * std::max(cap, 1) * 4;
     }
 };
@@ -598,7 +599,8 @@ int run_plain(GrandSession *s, hipStream_t st, int num_shots) {
     const DevCode *dcode = s->code->dcode.as<DevCode>();
     const DevCall *dcall = s->dcall.as<DevCall>();
     unsigned nwork = 0;
-    HIPX(hipMemcpyAsync(&nwork, call.counters, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    // one worklist partition (wl_parts = 1): its fronts are worklist[0 .. nwork)
+    HIPX(hipMemcpyAsync(&nwork, call.counters + kWlBase, sizeof(unsigned), hipMemcpyDeviceToHost, st));
     HIPX(hipStreamSynchronize(st));
     std::vector<int> work(nwork);
     if (nwork) HIPX(hipMemcpyAsync(work.data(), call.worklist, sizeof(int) * nwork, hipMemcpyDeviceToHost, st));
@@ -771,6 +773,13 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
                        c.lscores == nullptr && code.D <= kLaneMaxD && (long long)code.m * code.D <= kLaneMaxMD &&
                        !(s->family == kCC && c.input_errors && code.dg == nullptr)) ? 1 : 0;
     c.split_worklist = (s->ex.engine != GRAND_ENGINE_PLAIN) ? 1 : 0;
+    c.wl_parts = (s->ex.engine != GRAND_ENGINE_PLAIN) ? kWlParts : 1;
+    // the triage kernel reads the column words, offsets, signatures and
+    // degeneracy words of a small code from its LDS (lane search, s = H e)
+    {
+        const long long tsw = (long long)(code.D + 2) * code.m + (code.dg ? code.m : 0);
+        c.triage_stage_words = ((c.triage_search || c.input_errors) && tsw <= kCodeLdsMaxWords) ? (int)tsw : 0;
+    }
     // one claimed shot in the search kernel's LDS: id, signature, syndrome words,
     // then the packed errors (cc with errors) or the observed logicals (cln)
     c.stage_words = 2 + code.D + (c.input_errors ? 2 + code.M32 : 0) + (s->family == kCLN ? code.NL32 : 0);
@@ -807,7 +816,7 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     const DevCall *dcall = s->dcall.as<DevCall>();
     if (s->timing) HIPX(hipEventRecord(s->ev[0], st));
     const int block_words = std::max(c.input_errors ? code.M32 : code.D, s->family == kCLN ? code.NL32 : 0);
-    HIPX(grand_launch_triage(dcode, dcall, tgrid, block_words, st));
+    HIPX(grand_launch_triage(dcode, dcall, tgrid, block_words, c.triage_stage_words, st));
     if (s->timing) HIPX(hipEventRecord(s->ev[1], st));
     if (s->ex.engine == GRAND_ENGINE_PLAIN) {
         int rc = run_plain(s, st, num_shots);

@@ -1788,19 +1788,49 @@ __device__ __forceinline__ void decode_shot_wave(const DevCode &code, const DevC
     finalize_shot(code, call, shot, o, slot);
 }
 
+// First worklist slot of partition k and its number of slots (64 per group) for
+// G triage groups dealt over P partitions (group g to partition g % P).
+__device__ __forceinline__ unsigned wl_part_start(int k, int G, int P) {
+    const int q = G / P, r = G % P;
+    return 64u * (unsigned)(k * q + (k < r ? k : r));
+}
+__device__ __forceinline__ unsigned wl_part_slots(int k, int G, int P) {
+    return 64u * (unsigned)(G / P + (k < G % P ? 1 : 0));
+}
+
+// Worklist slot of position p: positions below `heavy` are the front entries of
+// all partitions in partition order, the others their back entries.  Lane j < P
+// holds the exclusive prefix sums wfx (fronts) and wbx (backs) of partition j;
+// the partition of p is the last whose prefix is <= p (empty partitions share
+// the next one's prefix).  Call with every lane active: it reads other lanes.
+__device__ __forceinline__ unsigned wl_slot(unsigned p, unsigned heavy, unsigned wfx, unsigned wbx, int P, int G) {
+    const bool fr = p < heavy;
+    const unsigned q = fr ? p : p - heavy;
+    int k = 0;
+    unsigned base = 0u;
+    for (int j = 1; j < P; ++j) {
+        const unsigned fj = (unsigned)__builtin_amdgcn_readlane((int)wfx, j);
+        const unsigned bj = (unsigned)__builtin_amdgcn_readlane((int)wbx, j);
+        const unsigned pj = fr ? fj : bj;
+        if (pj <= q) { k = j; base = pj; }
+    }
+    const unsigned r0 = wl_part_start(k, G, P);
+    return fr ? r0 + (q - base) : r0 + wl_part_slots(k, G, P) - 1u - (q - base);
+}
+
 // Stages the shots of one claim (worklist positions p0, p0 + 8, ...; nb of them)
 // in LDS: the worklist entries first, then every shot's words in one round of
 // loads, so the shots decode without waiting on global memory between them.
 __device__ void stage_claim(const DevCode &code, const DevCall &call, unsigned p0, int nb, unsigned heavy,
-                            const WaveLds &L) {
+                            unsigned wfx, unsigned wbx, const WaveLds &L) {
     const int lane = lane_id();
     const int SW = call.stage_words;
     const int per = SW - 1;                        // loaded words per shot: signature, syndrome, errors or logicals
     int myshot = 0;
-    if (lane < nb) {
-        // positions below `heavy` are the front segment of the worklist, the rest the back segment
+    {
         const unsigned p = p0 + (unsigned)kClaimWords * (unsigned)lane;
-        myshot = call.worklist[p < heavy ? p : (unsigned)call.num_shots - 1u - (p - heavy)];
+        const unsigned slot = wl_slot(p, heavy, wfx, wbx, call.wl_parts, (call.num_shots + 63) / 64);
+        if (lane < nb) myshot = call.worklist[slot];
     }
     if (lane < nb) L.stage[lane * SW] = (uint32_t)myshot;
     const int E = call.input_errors ? 2 + code.M32 : 0;
@@ -1864,24 +1894,39 @@ __device__ __forceinline__ unsigned lane_ncr(int n, int k) {
     return r;
 }
 
-template <int D>
-__device__ __forceinline__ bool lane_exact(const DevCode &code, const uint32_t (&s)[kMaxWords], int a, int b, int c,
+// The code tables the triage kernel reads by column: column words (word-major,
+// hcw[d * m + c]), score offsets (kNegScore marks an empty column), signatures
+// and degeneracy words (has_dg).  They sit in the wave's LDS when the code is
+// small enough to stage there, else they are the code's global arrays.
+template <class U, class I>
+struct TriTables {
+    U hcw;
+    I off;
+    U sig;
+    U dg;
+    bool has_dg;
+};
+using TriLds = TriTables<const uint32_t LDS *, const int LDS *>;
+using TriGlobal = TriTables<const uint32_t *, const int *>;
+
+template <int D, class T>
+__device__ __forceinline__ bool lane_exact(const T &tb, int m, const uint32_t (&s)[kMaxWords], int a, int b, int c,
                                            int w) {
     uint32_t bad = 0u;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        uint32_t x = s[d] ^ code.Hcm[(size_t)a * D + d];
-        if (w > 1) x ^= code.Hcm[(size_t)b * D + d];
-        if (w > 2) x ^= code.Hcm[(size_t)c * D + d];
+        uint32_t x = s[d] ^ tb.hcw[d * m + a];
+        if (w > 1) x ^= tb.hcw[d * m + b];
+        if (w > 2) x ^= tb.hcw[d * m + c];
         bad |= x;
     }
     return bad == 0u;
 }
 
-template <int D>
-__device__ bool lane_search(const DevCode &code, const DevCall &call, int shot, const uint32_t (&s)[kMaxWords],
-                            uint32_t sg, int ow, uint32_t dgv, int LDS *LA, uint32_t LDS *LS,
-                            unsigned long long t0) {
+template <int D, class T>
+__device__ bool lane_search(const DevCode &code, const DevCall &call, const T &tb, int shot,
+                            const uint32_t (&s)[kMaxWords], uint32_t sg, int ow, uint32_t dgv, int LDS *LA,
+                            uint32_t LDS *LS, unsigned long long t0) {
     const int lane = lane_id();
     const int m = code.m;
     int tam = 0;
@@ -1890,24 +1935,24 @@ __device__ bool lane_search(const DevCode &code, const DevCall &call, int shot, 
     for (; cb + 8 <= m; cb += 8) {
         uint32_t out[8];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) out[j] = (code.off[cb + j] == kNegScore) ? 1u : 0u;
+        for (int j = 0; j < 8; ++j) out[j] = (tb.off[cb + j] == kNegScore) ? 1u : 0u;
 #pragma unroll
         for (int d = 0; d < D; ++d)
 #pragma unroll
-            for (int j = 0; j < 8; ++j) out[j] |= code.Hcw[(size_t)d * m + cb + j] & ~s[d];
+            for (int j = 0; j < 8; ++j) out[j] |= tb.hcw[d * m + cb + j] & ~s[d];
 #pragma unroll
         for (int j = 0; j < 8; ++j)
             if (out[j] == 0u) {
-                if (tam < kLaneMax) { LA[tam * 64 + lane] = cb + j; LS[tam * 64 + lane] = code.sig[cb + j]; }
+                if (tam < kLaneMax) { LA[tam * 64 + lane] = cb + j; LS[tam * 64 + lane] = tb.sig[cb + j]; }
                 ++tam;
             }
     }
     for (int c = cb; c < m; ++c) {
-        uint32_t out = (code.off[c] == kNegScore) ? 1u : 0u;
+        uint32_t out = (tb.off[c] == kNegScore) ? 1u : 0u;
 #pragma unroll
-        for (int d = 0; d < D; ++d) out |= code.Hcw[(size_t)d * m + c] & ~s[d];
+        for (int d = 0; d < D; ++d) out |= tb.hcw[d * m + c] & ~s[d];
         if (out == 0u) {
-            if (tam < kLaneMax) { LA[tam * 64 + lane] = c; LS[tam * 64 + lane] = code.sig[c]; }
+            if (tam < kLaneMax) { LA[tam * 64 + lane] = c; LS[tam * 64 + lane] = tb.sig[c]; }
             ++tam;
         }
     }
@@ -1941,7 +1986,7 @@ __device__ bool lane_search(const DevCode &code, const DevCall &call, int shot, 
         unsigned r = 0;
         if (w == 1) {
             for (int a = 0; a < tam && hw == 0; ++a, ++r)
-                if (LS[a * 64 + lane] == sg && lane_exact<D>(code, s, LA[a * 64 + lane], 0, 0, 1)) {
+                if (LS[a * 64 + lane] == sg && lane_exact<D>(tb, m, s, LA[a * 64 + lane], 0, 0, 1)) {
                     hw = 1; ha = a; hidx = cum + r;
                 }
         } else if (w == 2) {
@@ -1949,7 +1994,7 @@ __device__ bool lane_search(const DevCode &code, const DevCall &call, int shot, 
                 const uint32_t ta = sg ^ LS[a * 64 + lane];
                 for (int b = a + 1; b < tam && hw == 0; ++b, ++r)
                     if (LS[b * 64 + lane] == ta &&
-                        lane_exact<D>(code, s, LA[a * 64 + lane], LA[b * 64 + lane], 0, 2)) {
+                        lane_exact<D>(tb, m, s, LA[a * 64 + lane], LA[b * 64 + lane], 0, 2)) {
                         hw = 2; ha = a; hb = b; hidx = cum + r;
                     }
             }
@@ -1957,10 +2002,10 @@ __device__ bool lane_search(const DevCode &code, const DevCall &call, int shot, 
             for (int a = 0; a + 2 < tam && hw == 0; ++a) {
                 const uint32_t ta = sg ^ LS[a * 64 + lane];
                 for (int b = a + 1; b + 1 < tam && hw == 0; ++b) {
-                    const uint32_t tb = ta ^ LS[b * 64 + lane];
+                    const uint32_t tsb = ta ^ LS[b * 64 + lane];
                     for (int c = b + 1; c < tam && hw == 0; ++c, ++r)
-                        if (LS[c * 64 + lane] == tb &&
-                            lane_exact<D>(code, s, LA[a * 64 + lane], LA[b * 64 + lane], LA[c * 64 + lane], 3)) {
+                        if (LS[c * 64 + lane] == tsb &&
+                            lane_exact<D>(tb, m, s, LA[a * 64 + lane], LA[b * 64 + lane], LA[c * 64 + lane], 3)) {
                             hw = 3; ha = a; hb = b; hc = c; hidx = cum + r;
                         }
                 }
@@ -1990,9 +2035,9 @@ __device__ bool lane_search(const DevCode &code, const DevCall &call, int shot, 
         if (hw > 1) all = all && ((e[c1 >> 5] >> (c1 & 31)) & 1u);
         if (hw > 2) all = all && ((e[c2 >> 5] >> (c2 & 31)) & 1u);
         eq = all ? 1 : 0;
-        uint32_t gh = code.dg[c0];
-        if (hw > 1) gh ^= code.dg[c1];
-        if (hw > 2) gh ^= code.dg[c2];
+        uint32_t gh = tb.dg[c0];
+        if (hw > 1) gh ^= tb.dg[c1];
+        if (hw > 2) gh ^= tb.dg[c2];
         deg = (gh == dgv) ? 1 : 0;
     } else if (call.family == kCLN) {
         const int NL32 = code.NL32;
@@ -2022,16 +2067,17 @@ __device__ bool lane_search(const DevCode &code, const DevCall &call, int shot, 
     return true;
 }
 
-__device__ __forceinline__ bool lane_search_any(const DevCode &code, const DevCall &call, int shot,
+template <class T>
+__device__ __forceinline__ bool lane_search_any(const DevCode &code, const DevCall &call, const T &tb, int shot,
                                              const uint32_t (&s)[kMaxWords], uint32_t sg, int ow, uint32_t dgv,
                                              int LDS *LA, uint32_t LDS *LS, unsigned long long t0) {
     switch (code.D) {
-        case 1: return lane_search<1>(code, call, shot, s, sg, ow, dgv, LA, LS, t0);
-        case 2: return lane_search<2>(code, call, shot, s, sg, ow, dgv, LA, LS, t0);
-        case 3: return lane_search<3>(code, call, shot, s, sg, ow, dgv, LA, LS, t0);
-        case 4: return lane_search<4>(code, call, shot, s, sg, ow, dgv, LA, LS, t0);
-        case 5: return lane_search<5>(code, call, shot, s, sg, ow, dgv, LA, LS, t0);
-        case 6: return lane_search<6>(code, call, shot, s, sg, ow, dgv, LA, LS, t0);
+        case 1: return lane_search<1>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, t0);
+        case 2: return lane_search<2>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, t0);
+        case 3: return lane_search<3>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, t0);
+        case 4: return lane_search<4>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, t0);
+        case 5: return lane_search<5>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, t0);
+        case 6: return lane_search<6>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, t0);
         default: return false;
     }
 }
@@ -2111,18 +2157,14 @@ __device__ __forceinline__ void pack_rows(const int *rows, int ncols, int base, 
 
 // One lane per shot, 64 shots per wave: packs the input rows (errors or
 // syndromes, and logicals), forms s = H e and sig(s) as XORs over the set error
-// bits (both maps are GF(2)-linear), writes the result of zero-syndrome shots
-// and appends the others to the worklist with one atomic per 64 shots.
-// LDS: the lane search's [kLaneMax][64] positions and signatures, then the
-// packed block, 64 rows of the larger of the input row's words (M32 for errors,
-// D for syndromes) and, for the DEM family, the NL32 words of the logicals.
-__global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ codep, const DevCall *__restrict__ callp) {
-    const DevCode &code = *codep;
-    const DevCall &call = *callp;
+// bits (both maps are GF(2)-linear), searches small first iterations by lane,
+// writes the result of zero-syndrome and lane-solved shots and appends the
+// others to the worklist with one atomic per 64 shots.
+template <class T>
+__device__ __forceinline__ void triage_groups(const DevCode &code, const DevCall &call, const T &tb, int LDS *LA,
+                                              uint32_t LDS *LS, uint32_t LDS *bits) {
     const int lane = lane_id();
-    int LDS *LA = (int LDS *)lds_base();
-    uint32_t LDS *LS = lds_base() + kLaneMax * 64;
-    uint32_t LDS *bits = lds_base() + 2 * kLaneMax * 64;
+    const int m = code.m;
     const int groups = (call.num_shots + 63) / 64;
     unsigned bad = 0;
     for (int g = blockIdx.x; g < groups; g += gridDim.x) {
@@ -2137,7 +2179,7 @@ __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ 
         int ow = 0;
         if (call.input_errors) {
             const int M32 = code.M32;
-            pack_rows(call.input, code.m, base, cnt, M32, bits, bad);
+            pack_rows(call.input, m, base, cnt, M32, bits, bad);
             // the block's packed rows are consecutive in e_packed
             for (int i = lane; i < cnt * M32; i += 64) call.e_packed[(size_t)base * M32 + i] = bits[i];
             for (int wd = 0; wd < M32; ++wd) {
@@ -2146,11 +2188,11 @@ __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ 
                 while (b) {
                     const int c = 32 * wd + __ffs(b) - 1;
                     b &= b - 1u;
-                    sg ^= code.sig[c];
-                    if (code.dg) dgv ^= code.dg[c];
+                    sg ^= tb.sig[c];
+                    if (tb.has_dg) dgv ^= tb.dg[c];
 #pragma unroll
                     for (int d = 0; d < kMaxWords; ++d)
-                        if (d < code.D) s[d] ^= code.Hcm[(size_t)c * code.D + d];
+                        if (d < code.D) s[d] ^= tb.hcw[d * m + c];
                 }
             }
         } else {
@@ -2180,7 +2222,7 @@ __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ 
         bool solved = false;
         if (call.triage_search && ballot(nz)) {
             const unsigned long long t0 = now_ticks();
-            if (nz) solved = lane_search_any(code, call, shot, s, sg, ow, dgv, LA, LS, t0);
+            if (nz) solved = lane_search_any(code, call, tb, shot, s, sg, ow, dgv, LA, LS, t0);
         }
         const bool queued = nz && !solved;
         if (valid) {
@@ -2210,22 +2252,62 @@ __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ 
         int sw = 0;
 #pragma unroll
         for (int d = 0; d < kMaxWords; ++d) sw += __popc(s[d]);
+        // The group's partition (g % wl_parts) spreads the counter atomics over
+        // wl_parts cache lines.
         const bool front = queued && (!call.split_worklist || sw > 2 * code.max_soma);
         const uint64_t mf = ballot(front), mb = ballot(queued && !front);
+        const int part = g % call.wl_parts;
+        unsigned *pc = call.counters + kWlBase + kWlStride * part;
+        const unsigned r0 = wl_part_start(part, groups, call.wl_parts);
         if (mf) {
             unsigned pos = 0;
-            if (lane == 0) pos = atomicAdd(&call.counters[0], (unsigned)__popcll(mf));
+            if (lane == 0) pos = atomicAdd(&pc[0], (unsigned)__popcll(mf));
             pos = uni(pos);
-            if (front) call.worklist[pos + lanes_below(mf)] = shot;
+            if (front) call.worklist[r0 + pos + lanes_below(mf)] = shot;
         }
         if (mb) {
             unsigned pos = 0;
-            if (lane == 0) pos = atomicAdd(&call.counters[1], (unsigned)__popcll(mb));
+            if (lane == 0) pos = atomicAdd(&pc[1], (unsigned)__popcll(mb));
             pos = uni(pos);
-            if (queued && !front) call.worklist[call.num_shots - 1 - (int)(pos + lanes_below(mb))] = shot;
+            const unsigned r1 = r0 + wl_part_slots(part, groups, call.wl_parts) - 1u;
+            if (queued && !front) call.worklist[r1 - (pos + lanes_below(mb))] = shot;
         }
     }
     if (ballot(bad != 0u) && lane == 0) atomicOr(&call.counters[2], kErrNonBinary);
+}
+
+// LDS: the lane search's [kLaneMax][64] positions and signatures; the code
+// tables when call.triage_stage_words > 0 (column words [D][m], offsets [m],
+// signatures [m], degeneracy words [m] when the code has them); then the packed
+// block, 64 rows of the larger of the input row's words (M32 for errors, D for
+// syndromes) and, for the DEM family, the NL32 words of the logicals.
+__global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ codep, const DevCall *__restrict__ callp) {
+    const DevCode &code = *codep;
+    const DevCall &call = *callp;
+    const int lane = lane_id();
+    int LDS *LA = (int LDS *)lds_base();
+    uint32_t LDS *LS = lds_base() + kLaneMax * 64;
+    uint32_t LDS *tab = lds_base() + 2 * kLaneMax * 64;
+    const int tsw = call.triage_stage_words;
+    if (tsw > 0) {
+        const int m = code.m, nw = code.D * m;
+#pragma unroll 8
+        for (int i = lane; i < nw; i += 64) tab[i] = code.Hcw[i];
+#pragma unroll 4
+        for (int i = lane; i < m; i += 64) {
+            tab[nw + i] = (uint32_t)code.off[i];
+            tab[nw + m + i] = code.sig[i];
+        }
+        const bool has_dg = code.dg != nullptr;
+        if (has_dg)
+#pragma unroll 4
+            for (int i = lane; i < m; i += 64) tab[nw + 2 * m + i] = code.dg[i];
+        const TriLds tb{tab, (const int LDS *)(tab + nw), tab + nw + m, tab + nw + 2 * m, has_dg};
+        triage_groups(code, call, tb, LA, LS, tab + tsw);   // its packing starts with a barrier
+    } else {
+        const TriGlobal tb{code.Hcw, code.off, code.sig, code.dg, code.dg != nullptr};
+        triage_groups(code, call, tb, LA, LS, tab);
+    }
 }
 
 // Persistent search kernel: waves claim worklist entries in batches until none remain.
@@ -2257,10 +2339,24 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
             for (int i = lane_id(); i < nr; i += 64) post[code.m + i] = code.R[i];
         }
     }
-    // worklist: shots likely to search deep at the front (counters[0] of them),
-    // the others at the back of the buffer (counters[1]), claimed in that order
-    const unsigned heavy = call.counters[0];
-    const unsigned total = heavy + call.counters[1];
+    // worklist: shots likely to search deep first (the front entries of every
+    // partition, `heavy` of them), then the others (the partitions' back
+    // entries); lane k < wl_parts gets the exclusive prefix sums of partition k
+    unsigned wf = 0u, wb = 0u;
+    if (lane_id() < call.wl_parts) {
+        wf = call.counters[kWlBase + kWlStride * lane_id()];
+        wb = call.counters[kWlBase + kWlStride * lane_id() + 1];
+    }
+    unsigned wfx = wf, wbx = wb;
+#pragma unroll
+    for (int o = 1; o < kWlParts; o <<= 1) {
+        const unsigned a = (unsigned)__shfl_up((int)wfx, o, 64), b = (unsigned)__shfl_up((int)wbx, o, 64);
+        if (lane_id() >= o) { wfx += a; wbx += b; }
+    }
+    const unsigned heavy = uni((unsigned)__shfl((int)wfx, call.wl_parts - 1, 64));
+    const unsigned total = heavy + uni((unsigned)__shfl((int)wbx, call.wl_parts - 1, 64));
+    wfx -= wf;
+    wbx -= wb;
     // shots per claim: the tuned batch when the worklist is long, fewer when
     // there are not several claims per wave (small batches finish sooner)
     unsigned B = total / (4u * gridDim.x);
@@ -2317,7 +2413,7 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
             if (got < total && p0 < total) {
                 const unsigned left = (total - 1u - p0) / kClaimWords + 1u;   // positions p0, p0 + 8, ... below total
                 const int nb = (int)(left < B ? left : B);
-                stage_claim(code, call, p0, nb, heavy, L);
+                stage_claim(code, call, p0, nb, heavy, wfx, wbx, L);
                 for (int j = 0; j < nb; ++j) decode_shot_wave(code, call, L.stage + j * call.stage_words, L);
                 ctask = -1;
                 continue;
@@ -2572,6 +2668,7 @@ __global__ void prep_kernel(const DevCall c, DevCall *dst, unsigned *counters) {
     if (threadIdx.x < 16) counters[threadIdx.x] = 0u;
     unsigned long long *claim_words = reinterpret_cast<unsigned long long *>(counters) + kClaimBase;
     if (threadIdx.x < kClaimWords) claim_words[threadIdx.x * kClaimStride] = 0ull;
+    if (threadIdx.x < 2 * kWlParts) counters[kWlBase + kWlStride * (threadIdx.x >> 1) + (threadIdx.x & 1)] = 0u;
 }
 
 __global__ void fill_u32_kernel(unsigned *p, unsigned v, int n) {
@@ -2606,11 +2703,13 @@ GRAND_INTERNAL hipError_t grand_launch_fill(unsigned *p, unsigned v, int n, hipS
 
 // `block_words`: words per shot of the triage kernel's packed block, the larger
 // of the input row (M32 words of errors or D words of syndrome) and, for the DEM
-// family, the NL32 words of observed logicals.
+// family, the NL32 words of observed logicals; `stage_words`: the code tables it
+// stages in LDS (DevCall::triage_stage_words).
 GRAND_INTERNAL hipError_t grand_launch_triage(const DevCode *code, const DevCall *call, int grid, int block_words,
-                                              hipStream_t st) {
+                                              int stage_words, hipStream_t st) {
     hipLaunchKernelGGL(grand::triage_kernel, dim3(grid), dim3(64),
-                       sizeof(uint32_t) * (2 * grand::kLaneMax * 64 + 64 * (size_t)block_words), st, code, call);
+                       sizeof(uint32_t) * (2 * grand::kLaneMax * 64 + (size_t)stage_words + 64 * (size_t)block_words),
+                       st, code, call);
     return hipGetLastError();
 }
 

@@ -76,7 +76,17 @@ constexpr int kLaneMaxMD = 2048;         // ... and m * D at most this (the lane
 constexpr int kClaimWords = 8;
 constexpr int kClaimBase = 16;        // in 64-bit words: byte offset 128
 constexpr int kClaimStride = 16;      // in 64-bit words: 128 bytes
-constexpr int kCounterBytes = 8 * (kClaimBase + kClaimWords * kClaimStride);
+// Worklist partitions: group g of the triage kernel (shots 64g .. 64g + 63)
+// appends the shots it queues to partition g % wl_parts, whose front and back
+// counters (32-bit counter indices kWlBase + kWlStride * k and one after) sit on
+// a 128-byte line of their own after the claim words.  Partition k owns the
+// worklist slots of its groups' count of 64-slot rows, starting at
+// 64 * (k * (G / P) + min(k, G % P)) for G groups and P partitions; fronts fill
+// it from the start, backs from the end.
+constexpr int kWlParts = 16;
+constexpr int kWlBase = 2 * (kClaimBase + kClaimWords * kClaimStride);
+constexpr int kWlStride = 32;
+constexpr int kCounterBytes = 4 * (kWlBase + kWlParts * kWlStride);
 constexpr int kTaskClasses = 6;          // weight classes one shared task can hold
 struct TaskSlot {
     unsigned long long ctl;           // generation | nchunks | next chunk to claim
@@ -111,7 +121,10 @@ struct DevCall {
     int claim_batch;                  // shots taken per worklist claim (<= kMaxBatch)
     int triage_search;                // 1: the triage kernel searches small first iterations one shot per lane
     int split_worklist;               // 1 (search kernel): worklist in two segments, likely-deep shots first
+    int wl_parts;                     // worklist partitions (1 for the plain engine, else kWlParts)
     int pair_cap;                     // search kernel: pairs the LDS pair table holds (0 = no pair table)
+    int triage_stage_words;           // triage kernel: words of code tables staged in LDS (column words,
+                                      // offsets, signatures, degeneracy words), 0 = read from global memory
     int stage_words;                  // search kernel: LDS words per claimed shot: id, signature, syndrome,
                                       // then original weight, degeneracy word and error words (cc with
                                       // errors) or the observed logicals (cln)
@@ -125,9 +138,9 @@ struct DevCall {
     uint32_t *e_aux;                  // [shots][2] original weight and XOR of dg over supp(e) (cc with errors)
     uint32_t *lg_packed;              // [shots][NL32] (cln)
     uint32_t *s_sig;                  // [shots]
-    int *worklist;                    // nonzero-syndrome shots
-    unsigned *counters;               // [0] worklist front length, [1] back length, [2] error bits,
-                                      // [4..6] task counters, claim words
+    int *worklist;                    // nonzero-syndrome shots, 64 slots per triage group
+    unsigned *counters;               // [2] error bits, [4..6] task counters, claim words, then the
+                                      // front and back lengths of each worklist partition
     GrandDecoderResult *results;      // [shots]
     uint32_t *dec;                    // [shots][M32] decoded pattern
     uint32_t *declog;                 // [shots][NL32] decoded logicals (cln)
