@@ -347,6 +347,21 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
     for (int r = 0; r < rank; ++r) pivrow[piv[r]] = r;
     std::vector<uint32_t> dg;
     if (family == kCC) (void)degeneracy_words(Hrow, R, piv, n, m, dg);
+    // Columns grouped by their lowest check (empty columns in no group): a
+    // column whose support lies inside supp(s) is in the group of one of s's
+    // checks, so the triage kernel's lane search tests only those groups.
+    std::vector<int> lgs(n + 1, 0), lgc;
+    {
+        std::vector<int> low(m, -1);
+        for (int c = 0; c < m; ++c)
+            for (int d = 0; d < D && low[c] < 0; ++d)
+                if (Hcm[(size_t)c * D + d]) low[c] = d * 32 + __builtin_ctz(Hcm[(size_t)c * D + d]);
+        for (int c = 0; c < m; ++c) if (low[c] >= 0) ++lgs[low[c] + 1];
+        for (int r = 0; r < n; ++r) lgs[r + 1] += lgs[r];
+        lgc.assign(lgs[n], 0);
+        std::vector<int> fill(lgs.begin(), lgs.end() - 1);
+        for (int c = 0; c < m; ++c) if (low[c] >= 0) lgc[fill[low[c]]++] = c;
+    }
     std::vector<uint32_t> Lp;
     if (family == kCLN) {
         Lp.assign((size_t)m * NL32, 0);
@@ -369,6 +384,7 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
                  iRs = add(rowsig.data(), rowsig.size() * 4), iOff = add(off.data(), off.size() * 4),
                  iR = add(R.data(), R.size() * 4), iPiv = add(pivrow.data(), pivrow.size() * 4),
                  iL = add(Lp.data(), Lp.size() * 4), iDg = add(dg.data(), dg.size() * 4),
+                 iGs = add(lgs.data(), lgs.size() * 4), iGc = add(lgc.data(), lgc.size() * 4),
                  iLlr = add(llr ? llr->data() : nullptr, llr ? llr->size() * 4 : 0);
     if (!ce->mem.ensure(total)) return nullptr;
     char *base = ce->mem.as<char>();
@@ -390,6 +406,8 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
     dc.pivrow = (const int *)(base + segs[iPiv].off);
     dc.Lp = (const uint32_t *)(base + segs[iL].off);
     dc.dg = dg.empty() ? nullptr : (const uint32_t *)(base + segs[iDg].off);
+    dc.lgs = (const int *)(base + segs[iGs].off);
+    dc.lgc = (const int *)(base + segs[iGc].off);
     dc.llr = (const float *)(base + segs[iLlr].off);
     dc.nCr = rt().nCr.as<unsigned long long>();
     dc.lds_words = 0;
@@ -777,7 +795,7 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     // the triage kernel reads the column words, offsets, signatures and
     // degeneracy words of a small code from its LDS (lane search, s = H e)
     {
-        const long long tsw = (long long)(code.D + 2) * code.m + (code.dg ? code.m : 0);
+        const long long tsw = (long long)(code.D + 3) * code.m + (code.dg ? code.m : 0) + code.n + 1;
         c.triage_stage_words = ((c.triage_search || c.input_errors) && tsw <= kCodeLdsMaxWords) ? (int)tsw : 0;
     }
     // one claimed shot in the search kernel's LDS: id, signature, syndrome words,
@@ -815,7 +833,9 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     const DevCode *dcode = s->code->dcode.as<DevCode>();
     const DevCall *dcall = s->dcall.as<DevCall>();
     if (s->timing) HIPX(hipEventRecord(s->ev[0], st));
-    const int block_words = std::max(c.input_errors ? code.M32 : code.D, s->family == kCLN ? code.NL32 : 0);
+    // (the lane search reuses the block as its M32-word active bitmask per lane)
+    const int block_words = std::max(std::max(c.input_errors ? code.M32 : code.D, s->family == kCLN ? code.NL32 : 0),
+                                     c.triage_search ? code.M32 : 0);
     HIPX(grand_launch_triage(dcode, dcall, tgrid, block_words, c.triage_stage_words, st));
     if (s->timing) HIPX(hipEventRecord(s->ev[1], st));
     if (s->ex.engine == GRAND_ENGINE_PLAIN) {

@@ -1887,23 +1887,46 @@ __device__ __forceinline__ void write_zero_result(GrandDecoderResult *dst, int o
 // false, having written nothing, when the shot needs the search kernel (list
 // too long, a class too large, a cap of the DEM family, or no match in the
 // first iteration).
-__device__ __forceinline__ unsigned lane_ncr(int n, int k) {
-    if (k < 0 || k > n) return 0u;
-    unsigned r = 1u;
-    for (int i = 1; i <= k; ++i) r = r * (unsigned)(n - k + i) / (unsigned)i;
-    return r;
+// Weight list of the first iteration for an active list of `tam` positions:
+// the weights of decode_shot_wave's list in order, 4 bits each, first in the
+// low bits, without the slots whose class is empty (weight above tam); 0 ends
+// the list (at most 15 slots: three phases of five).
+__device__ uint64_t lane_weight_list(const DevCall &call, int tam) {
+    if (call.kind != kUniform && call.n_masks <= 0) return 0ull;
+    int mi = call.n_masks - 1;
+    if (call.kind != kUniform)
+        for (int i = 0; i < call.n_masks; ++i) if (tam <= call.mask_tam_upper[i]) { mi = i; break; }
+    const int nslots = (call.kind == kUniform) ? call.W : call.num_phases * 5;
+    uint64_t wl = 0ull;
+    int n = 0;
+    for (int q = 0; q < nslots; ++q) {
+        int w;
+        if (call.kind == kUniform) {
+            w = q + 1;
+        } else {
+            const int ph = call.mask_order[mi][q / 5], j = q % 5;
+            w = (ph == 0) ? j + 1 : (ph == 1 && j == 0) ? 6 : 0;
+        }
+        if (w == 0 || w > tam) continue;
+        wl |= (uint64_t)w << (4 * n);
+        ++n;
+    }
+    return wl;
 }
 
 // The code tables the triage kernel reads by column: column words (word-major,
-// hcw[d * m + c]), score offsets (kNegScore marks an empty column), signatures
-// and degeneracy words (has_dg).  They sit in the wave's LDS when the code is
-// small enough to stage there, else they are the code's global arrays.
+// hcw[d * m + c]), score offsets (kNegScore marks an empty column), signatures,
+// degeneracy words (has_dg) and the columns grouped by their lowest check (gc,
+// group of check r at gs[r] .. gs[r + 1]).  They sit in the wave's LDS when the
+// code is small enough to stage there, else they are the code's global arrays.
 template <class U, class I>
 struct TriTables {
     U hcw;
     I off;
     U sig;
     U dg;
+    I gs;
+    I gc;
     bool has_dg;
 };
 using TriLds = TriTables<const uint32_t LDS *, const int LDS *>;
@@ -1924,35 +1947,42 @@ __device__ __forceinline__ bool lane_exact(const T &tb, int m, const uint32_t (&
 }
 
 template <int D, class T>
-__device__ bool lane_search(const DevCode &code, const DevCall &call, const T &tb, int shot,
+__device__ __forceinline__ bool lane_search(const DevCode &code, const DevCall &call, const T &tb, int shot,
                             const uint32_t (&s)[kMaxWords], uint32_t sg, int ow, uint32_t dgv, int LDS *LA,
-                            uint32_t LDS *LS, unsigned long long t0) {
+                            uint32_t LDS *LS, const unsigned long long LDS *WL, uint32_t LDS *AB,
+                            unsigned long long t0) {
     const int lane = lane_id();
     const int m = code.m;
-    int tam = 0;
-    // eight columns per step (wave-uniform loads, issued together)
-    int cb = 0;
-    for (; cb + 8 <= m; cb += 8) {
-        uint32_t out[8];
+    // Active columns: nonempty, support inside supp(s).  Such a column holds
+    // its lowest check, so only the groups of s's checks are tested; hits set
+    // bits of the lane's bitmask AB[wd * 64 + lane], read back in column order
+    // (the order of the wave path's active list), at most kLaneMax + 1 of them.
+    const int M32 = code.M32;
+    for (int wd = 0; wd < M32; ++wd) AB[wd * 64 + lane] = 0u;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) out[j] = (tb.off[cb + j] == kNegScore) ? 1u : 0u;
+    for (int d = 0; d < D; ++d) {
+        uint32_t bs = s[d];
+        while (bs) {
+            const int r = 32 * d + __ffs(bs) - 1;
+            bs &= bs - 1u;
+            const int k1 = tb.gs[r + 1];
+            for (int k = tb.gs[r]; k < k1; ++k) {
+                const int c = tb.gc[k];
+                uint32_t out = 0u;
 #pragma unroll
-        for (int d = 0; d < D; ++d)
-#pragma unroll
-            for (int j = 0; j < 8; ++j) out[j] |= tb.hcw[d * m + cb + j] & ~s[d];
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-            if (out[j] == 0u) {
-                if (tam < kLaneMax) { LA[tam * 64 + lane] = cb + j; LS[tam * 64 + lane] = tb.sig[cb + j]; }
-                ++tam;
+                for (int e = 0; e < D; ++e) out |= tb.hcw[e * m + c] & ~s[e];
+                if (out == 0u)
+                    __hip_atomic_fetch_or(&AB[(c >> 5) * 64 + lane], 1u << (c & 31), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP);
             }
+        }
     }
-    for (int c = cb; c < m; ++c) {
-        uint32_t out = (tb.off[c] == kNegScore) ? 1u : 0u;
-#pragma unroll
-        for (int d = 0; d < D; ++d) out |= tb.hcw[d * m + c] & ~s[d];
-        if (out == 0u) {
-            if (tam < kLaneMax) { LA[tam * 64 + lane] = c; LS[tam * 64 + lane] = tb.sig[c]; }
+    int tam = 0;
+    for (int wd = 0; wd < M32 && tam <= kLaneMax; ++wd) {
+        uint32_t w = AB[wd * 64 + lane];
+        while (w && tam <= kLaneMax) {
+            if (tam < kLaneMax) LA[tam * 64 + lane] = 32 * wd + __ffs(w) - 1;
+            w &= w - 1u;
             ++tam;
         }
     }
@@ -1960,58 +1990,130 @@ __device__ bool lane_search(const DevCode &code, const DevCall &call, const T &t
     if (call.family == kCLN && tam > call.max_size) return false;   // tam falls back to 0: second iteration
     const int br = branch_of(call, tam);
     if (br == 0) return false;
-    // the weight list of decode_shot_wave, walked class by class
-    if (call.kind != kUniform && call.n_masks <= 0) return false;
-    int mi = call.n_masks - 1;
-    if (call.kind != kUniform)
-        for (int i = 0; i < call.n_masks; ++i) if (tam <= call.mask_tam_upper[i]) { mi = i; break; }
-    const int nslots = (call.kind == kUniform) ? call.W : call.num_phases * 5;
+    // the weight list of decode_shot_wave for this active-list size
+    const uint64_t wl = WL[tam];
+    auto nc2 = [](int n) { return n * (n - 1) / 2; };
+    auto nc3 = [](int n) { return n * (n - 1) * (n - 2) / 6; };
+    // the weights the walk reaches before it would give up (a class heavier
+    // than 3, or the budget exceeded)
+    unsigned need = 0u;
+    {
+        unsigned cum = 0;
+        for (uint64_t l = wl; l; l >>= 4) {
+            const int w = (int)(l & 15u);
+            if (w > 3) break;
+            const unsigned N = (unsigned)((w == 1) ? tam : (w == 2) ? nc2(tam) : nc3(tam));
+            if (cum + N > (unsigned)kLaneBudget) break;
+            need |= 1u << w;
+            cum += N;
+        }
+    }
+    // First signature match of each needed class, lexicographic order, from the
+    // signatures in registers: the loops run backwards and the last match
+    // written is the first in order.  Packed as a | b << 4 | c << 8, -1 = none.
+    // The signatures of the list, looked up together, in registers and in LS.
+    // tmax: the longest list among the lanes searching (bounds the loops).
+    uint32_t S[kLaneMax];
+#pragma unroll
+    for (int i = 0; i < kLaneMax; ++i) S[i] = (i < tam) ? tb.sig[LA[i * 64 + lane]] : 0u;
+#pragma unroll
+    for (int i = 0; i < kLaneMax; ++i)
+        if (i < tam) LS[i * 64 + lane] = S[i];
+    int tmax = 1;
+    for (int t = kLaneMax; t > 1; --t)
+        if (ballot(tam >= t)) { tmax = t; break; }
+    int f1 = -1, f2 = -1, f3 = -1;
+    if (ballot(need & 2u)) {
+#pragma unroll
+        for (int x = kLaneMax - 1; x >= 0; --x)
+            if (x < tmax && x < tam && S[x] == sg) f1 = x;
+    }
+    // (the outer loops read S[x] from LDS: x varies at run time)
+    if (ballot(need & 4u)) {
+#pragma unroll 1
+        for (int x = tmax - 2; x >= 0; --x) {
+            const uint32_t t = sg ^ LS[x * 64 + lane];
+#pragma unroll
+            for (int y = kLaneMax - 1; y >= 1; --y)
+                if (y > x && y < tmax && y < tam && S[y] == t) f2 = x | (y << 4);
+        }
+    }
+    if (ballot(need & 8u)) {
+#pragma unroll 1
+        for (int x = tmax - 3; x >= 0; --x) {
+            const uint32_t tx = sg ^ LS[x * 64 + lane];
+#pragma unroll
+            for (int y = kLaneMax - 2; y >= 1; --y) {
+                if (y <= x || y >= tmax - 1) continue;
+                const uint32_t t = tx ^ S[y];
+#pragma unroll
+                for (int z = kLaneMax - 1; z > y; --z)
+                    if (z < tmax && z < tam && S[z] == t) f3 = x | (y << 4) | (z << 8);
+            }
+        }
+    }
+    // walk the classes: the first class with a signature match holds the
+    // candidate; its rank within the class is in closed form
     unsigned cum = 0, wsum = 0;
     int hw = 0, ha = 0, hb = 0, hc = 0;
     unsigned hidx = 0;
-    bool giveup = false;
-    for (int q = 0; q < nslots && hw == 0 && !giveup; ++q) {
-        int w;
-        if (call.kind == kUniform) {
-            w = q + 1;
-        } else {
-            const int ph = call.mask_order[mi][q / 5], j = q % 5;
-            w = (ph == 0) ? ((j < 5 && j + 1 <= tam && j + 1 <= kMaxW) ? j + 1 : 0)
-                          : (ph == 1 && j == 0 && tam >= 6) ? 6 : 0;
+    for (uint64_t l = wl; l; l >>= 4) {
+        const int w = (int)(l & 15u);
+        if (w > 3) break;
+        const unsigned N = (unsigned)((w == 1) ? tam : (w == 2) ? nc2(tam) : nc3(tam));
+        if (cum + N > (unsigned)kLaneBudget) break;
+        const int f = (w == 1) ? f1 : (w == 2) ? f2 : f3;
+        if (f >= 0) {
+            const int x = f & 15, y = (f >> 4) & 15, z = f >> 8;
+            const int rank = (w == 1) ? x
+                           : (w == 2) ? nc2(tam) - nc2(tam - x) + (y - x - 1)
+                                      : nc3(tam) - nc3(tam - x) + nc2(tam - x - 1) - nc2(tam - y) + (z - y - 1);
+            hw = w; ha = x; hb = y; hc = z; hidx = cum + (unsigned)rank;
+            break;
         }
-        if (w == 0) continue;
-        const unsigned N = lane_ncr(tam, w);
-        if (N == 0u) continue;
-        if (w > 3 || cum + N > (unsigned)kLaneBudget) { giveup = true; break; }
-        unsigned r = 0;
-        if (w == 1) {
-            for (int a = 0; a < tam && hw == 0; ++a, ++r)
-                if (LS[a * 64 + lane] == sg && lane_exact<D>(tb, m, s, LA[a * 64 + lane], 0, 0, 1)) {
-                    hw = 1; ha = a; hidx = cum + r;
-                }
-        } else if (w == 2) {
-            for (int a = 0; a + 1 < tam && hw == 0; ++a) {
-                const uint32_t ta = sg ^ LS[a * 64 + lane];
-                for (int b = a + 1; b < tam && hw == 0; ++b, ++r)
-                    if (LS[b * 64 + lane] == ta &&
-                        lane_exact<D>(tb, m, s, LA[a * 64 + lane], LA[b * 64 + lane], 0, 2)) {
-                        hw = 2; ha = a; hb = b; hidx = cum + r;
+        cum += N;
+        wsum += N * (unsigned)(w + 1);
+    }
+    // The candidate is confirmed exactly.  A signature collision (an earlier
+    // pattern matching the signature but not the syndrome) redoes the walk
+    // pattern by pattern with the exact check on every signature match.
+    if (hw != 0 && !lane_exact<D>(tb, m, s, LA[ha * 64 + lane], LA[hb * 64 + lane], LA[hc * 64 + lane], hw)) {
+        cum = 0; wsum = 0; hw = 0;
+        for (uint64_t l = wl; l && hw == 0; l >>= 4) {
+            const int w = (int)(l & 15u);
+            if (w > 3) break;
+            const unsigned N = (unsigned)((w == 1) ? tam : (w == 2) ? nc2(tam) : nc3(tam));
+            if (cum + N > (unsigned)kLaneBudget) break;
+            unsigned r = 0;
+            if (w == 1) {
+                for (int x = 0; x < tam && hw == 0; ++x, ++r)
+                    if (LS[x * 64 + lane] == sg && lane_exact<D>(tb, m, s, LA[x * 64 + lane], 0, 0, 1)) {
+                        hw = 1; ha = x; hidx = cum + r;
                     }
-            }
-        } else {
-            for (int a = 0; a + 2 < tam && hw == 0; ++a) {
-                const uint32_t ta = sg ^ LS[a * 64 + lane];
-                for (int b = a + 1; b + 1 < tam && hw == 0; ++b) {
-                    const uint32_t tsb = ta ^ LS[b * 64 + lane];
-                    for (int c = b + 1; c < tam && hw == 0; ++c, ++r)
-                        if (LS[c * 64 + lane] == tsb &&
-                            lane_exact<D>(tb, m, s, LA[a * 64 + lane], LA[b * 64 + lane], LA[c * 64 + lane], 3)) {
-                            hw = 3; ha = a; hb = b; hc = c; hidx = cum + r;
+            } else if (w == 2) {
+                for (int x = 0; x + 1 < tam && hw == 0; ++x) {
+                    const uint32_t tx = sg ^ LS[x * 64 + lane];
+                    for (int y = x + 1; y < tam && hw == 0; ++y, ++r)
+                        if (LS[y * 64 + lane] == tx &&
+                            lane_exact<D>(tb, m, s, LA[x * 64 + lane], LA[y * 64 + lane], 0, 2)) {
+                            hw = 2; ha = x; hb = y; hidx = cum + r;
                         }
                 }
+            } else {
+                for (int x = 0; x + 2 < tam && hw == 0; ++x) {
+                    const uint32_t tx = sg ^ LS[x * 64 + lane];
+                    for (int y = x + 1; y + 1 < tam && hw == 0; ++y) {
+                        const uint32_t txy = tx ^ LS[y * 64 + lane];
+                        for (int z = y + 1; z < tam && hw == 0; ++z, ++r)
+                            if (LS[z * 64 + lane] == txy &&
+                                lane_exact<D>(tb, m, s, LA[x * 64 + lane], LA[y * 64 + lane], LA[z * 64 + lane], 3)) {
+                                hw = 3; ha = x; hb = y; hc = z; hidx = cum + r;
+                            }
+                    }
+                }
             }
+            if (hw == 0) { cum += N; wsum += N * (unsigned)(w + 1); }
         }
-        if (hw == 0) { cum += N; wsum += N * (unsigned)(w + 1); }
     }
     if (hw == 0) return false;
     const unsigned long long t1 = now_ticks();
@@ -2020,7 +2122,6 @@ __device__ bool lane_search(const DevCode &code, const DevCall &call, const T &t
     const int c2 = hw > 2 ? LA[hc * 64 + lane] : c0;
     const unsigned count = hidx + 1u;
     wsum += (hidx + 1u - cum) * (unsigned)(hw + 1);
-    const int M32 = code.M32;
     for (int wd = 0; wd < M32; ++wd) {
         uint32_t v = 0u;
         if ((c0 >> 5) == wd) v |= 1u << (c0 & 31);
@@ -2070,14 +2171,15 @@ __device__ bool lane_search(const DevCode &code, const DevCall &call, const T &t
 template <class T>
 __device__ __forceinline__ bool lane_search_any(const DevCode &code, const DevCall &call, const T &tb, int shot,
                                              const uint32_t (&s)[kMaxWords], uint32_t sg, int ow, uint32_t dgv,
-                                             int LDS *LA, uint32_t LDS *LS, unsigned long long t0) {
+                                             int LDS *LA, uint32_t LDS *LS, const unsigned long long LDS *WL,
+                                             uint32_t LDS *AB, unsigned long long t0) {
     switch (code.D) {
-        case 1: return lane_search<1>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, t0);
-        case 2: return lane_search<2>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, t0);
-        case 3: return lane_search<3>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, t0);
-        case 4: return lane_search<4>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, t0);
-        case 5: return lane_search<5>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, t0);
-        case 6: return lane_search<6>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, t0);
+        case 1: return lane_search<1>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, AB, t0);
+        case 2: return lane_search<2>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, AB, t0);
+        case 3: return lane_search<3>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, AB, t0);
+        case 4: return lane_search<4>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, AB, t0);
+        case 5: return lane_search<5>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, AB, t0);
+        case 6: return lane_search<6>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, AB, t0);
         default: return false;
     }
 }
@@ -2162,7 +2264,8 @@ __device__ __forceinline__ void pack_rows(const int *rows, int ncols, int base, 
 // others to the worklist with one atomic per 64 shots.
 template <class T>
 __device__ __forceinline__ void triage_groups(const DevCode &code, const DevCall &call, const T &tb, int LDS *LA,
-                                              uint32_t LDS *LS, uint32_t LDS *bits) {
+                                              uint32_t LDS *LS, const unsigned long long LDS *WL,
+                                              uint32_t LDS *bits) {
     const int lane = lane_id();
     const int m = code.m;
     const int groups = (call.num_shots + 63) / 64;
@@ -2222,7 +2325,8 @@ __device__ __forceinline__ void triage_groups(const DevCode &code, const DevCall
         bool solved = false;
         if (call.triage_search && ballot(nz)) {
             const unsigned long long t0 = now_ticks();
-            if (nz) solved = lane_search_any(code, call, tb, shot, s, sg, ow, dgv, LA, LS, t0);
+            // the packed block is free again: its words hold the lanes' active bitmasks
+            if (nz) solved = lane_search_any(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, bits, t0);
         }
         const bool queued = nz && !solved;
         if (valid) {
@@ -2276,9 +2380,11 @@ __device__ __forceinline__ void triage_groups(const DevCode &code, const DevCall
     if (ballot(bad != 0u) && lane == 0) atomicOr(&call.counters[2], kErrNonBinary);
 }
 
-// LDS: the lane search's [kLaneMax][64] positions and signatures; the code
-// tables when call.triage_stage_words > 0 (column words [D][m], offsets [m],
-// signatures [m], degeneracy words [m] when the code has them); then the packed
+// LDS: the lane search's [kLaneMax + 1][64] positions and signatures and its weight
+// lists by active-list size [kTriageWlWords / 2]; the code tables when
+// call.triage_stage_words > 0 (column words [D][m], offsets [m],
+// signatures [m], degeneracy words [m] when the code has them, lowest-check
+// group starts [n + 1] and grouped columns [at most m]); then the packed
 // block, 64 rows of the larger of the input row's words (M32 for errors, D for
 // syndromes) and, for the DEM family, the NL32 words of the logicals.
 __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ codep, const DevCall *__restrict__ callp) {
@@ -2286,8 +2392,10 @@ __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ 
     const DevCall &call = *callp;
     const int lane = lane_id();
     int LDS *LA = (int LDS *)lds_base();
-    uint32_t LDS *LS = lds_base() + kLaneMax * 64;
-    uint32_t LDS *tab = lds_base() + 2 * kLaneMax * 64;
+    uint32_t LDS *LS = lds_base() + (kLaneMax + 1) * 64;
+    unsigned long long LDS *WL = (unsigned long long LDS *)(lds_base() + 2 * (kLaneMax + 1) * 64);
+    uint32_t LDS *tab = lds_base() + 2 * (kLaneMax + 1) * 64 + kTriageWlWords;
+    if (lane <= kLaneMax) WL[lane] = (lane == 0) ? 0ull : lane_weight_list(call, lane);
     const int tsw = call.triage_stage_words;
     if (tsw > 0) {
         const int m = code.m, nw = code.D * m;
@@ -2302,11 +2410,19 @@ __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ 
         if (has_dg)
 #pragma unroll 4
             for (int i = lane; i < m; i += 64) tab[nw + 2 * m + i] = code.dg[i];
-        const TriLds tb{tab, (const int LDS *)(tab + nw), tab + nw + m, tab + nw + 2 * m, has_dg};
-        triage_groups(code, call, tb, LA, LS, tab + tsw);   // its packing starts with a barrier
+        uint32_t LDS *gs = tab + nw + (has_dg ? 3 : 2) * m;
+        uint32_t LDS *gc = gs + code.n + 1;
+        const int ng = code.lgs[code.n];
+#pragma unroll 4
+        for (int i = lane; i <= code.n; i += 64) gs[i] = (uint32_t)code.lgs[i];
+#pragma unroll 4
+        for (int i = lane; i < ng; i += 64) gc[i] = (uint32_t)code.lgc[i];
+        const TriLds tb{tab, (const int LDS *)(tab + nw), tab + nw + m, tab + nw + 2 * m, (const int LDS *)gs,
+                        (const int LDS *)gc, has_dg};
+        triage_groups(code, call, tb, LA, LS, WL, tab + tsw);   // its packing starts with a barrier
     } else {
-        const TriGlobal tb{code.Hcw, code.off, code.sig, code.dg, code.dg != nullptr};
-        triage_groups(code, call, tb, LA, LS, tab);
+        const TriGlobal tb{code.Hcw, code.off, code.sig, code.dg, code.lgs, code.lgc, code.dg != nullptr};
+        triage_groups(code, call, tb, LA, LS, WL, tab);
     }
 }
 
@@ -2708,7 +2824,8 @@ GRAND_INTERNAL hipError_t grand_launch_fill(unsigned *p, unsigned v, int n, hipS
 GRAND_INTERNAL hipError_t grand_launch_triage(const DevCode *code, const DevCall *call, int grid, int block_words,
                                               int stage_words, hipStream_t st) {
     hipLaunchKernelGGL(grand::triage_kernel, dim3(grid), dim3(64),
-                       sizeof(uint32_t) * (2 * grand::kLaneMax * 64 + (size_t)stage_words + 64 * (size_t)block_words),
+                       sizeof(uint32_t) * (2 * (grand::kLaneMax + 1) * 64 + grand::kTriageWlWords + (size_t)stage_words +
+                                           64 * (size_t)block_words),
                        st, code, call);
     return hipGetLastError();
 }

@@ -42,6 +42,8 @@ struct DevCode {
     const int *pivrow;                // [m] RREF row whose pivot is this column, -1 for other columns
     const uint32_t *Lp;               // [m][NL32] logical flips per mechanism (cln)
     const uint32_t *dg;               // [m] degeneracy words (cc), or null when more than 32 are needed
+    const int *lgs;                   // [n + 1] start of each check's group in lgc
+    const int *lgc;                   // columns grouped by their lowest check (empty columns left out)
     const float *llr;                 // [m] (ML)
     const unsigned long long *nCr;    // [(kMaxTam + 1) * kNcrStride]
     // Small codes are staged in the search kernel's LDS.  The search kernel gets
@@ -70,6 +72,7 @@ constexpr int kLaneMax = 12;             // active-list entries a lane keeps
 constexpr int kLaneBudget = 300;         // patterns a lane may test
 constexpr int kLaneMaxD = 6;             // codes with at most 6 syndrome words ...
 constexpr int kLaneMaxMD = 2048;         // ... and m * D at most this (the lane scans every column)
+constexpr int kTriageWlWords = 32;       // LDS words of the lane search's weight lists (one u64 per list size)
 
 // Claim words of the persistent search kernel: kClaimWords 64-bit words, one
 // per 128-byte line, starting at 32-bit counter index 2 * kClaimBase.
@@ -124,7 +127,8 @@ struct DevCall {
     int wl_parts;                     // worklist partitions (1 for the plain engine, else kWlParts)
     int pair_cap;                     // search kernel: pairs the LDS pair table holds (0 = no pair table)
     int triage_stage_words;           // triage kernel: words of code tables staged in LDS (column words,
-                                      // offsets, signatures, degeneracy words), 0 = read from global memory
+                                      // offsets, signatures, degeneracy words, lowest-check groups),
+                                      // 0 = read from global memory
     int stage_words;                  // search kernel: LDS words per claimed shot: id, signature, syndrome,
                                       // then original weight, degeneracy word and error words (cc with
                                       // errors) or the observed logicals (cln)
