@@ -101,9 +101,14 @@ int grand_session_reserve(GrandSession *s, int num_shots);
 
 /* HIP-event timing of the kernels of each decode, recorded on the decode's
  * stream: triage (pack + syndrome + worklist) and search (the persistent
- * kernel, or the whole plain loop).  last_timing waits for the last decode. */
+ * kernel, or the whole plain loop).  The session keeps the events of its last
+ * 64 timed decodes, so timed decodes still run back to back; last_timing waits
+ * for the last decode, timing_totals for all of them and returns the sums of
+ * the times since the last timing_reset and the number of decodes summed. */
 int grand_session_enable_timing(GrandSession *s, int on);
 int grand_session_last_timing(GrandSession *s, float *triage_ms, float *search_ms);
+int grand_session_timing_totals(GrandSession *s, double *triage_ms, double *search_ms, long long *decodes);
+int grand_session_timing_reset(GrandSession *s);
 
 /* Device wall-clock period in milliseconds per tick (for d_stats). */
 double grand_session_ms_per_tick(GrandSession *s);

@@ -347,27 +347,48 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
     for (int r = 0; r < rank; ++r) pivrow[piv[r]] = r;
     std::vector<uint32_t> dg;
     if (family == kCC) (void)degeneracy_words(Hrow, R, piv, n, m, dg);
-    // Columns grouped by their lowest check (empty columns in no group): a
-    // column whose support lies inside supp(s) is in the group of one of s's
-    // checks, so the triage kernel's lane search tests only those groups.
-    std::vector<int> lgs(n + 1, 0), lgc;
-    {
-        std::vector<int> low(m, -1);
-        for (int c = 0; c < m; ++c)
-            for (int d = 0; d < D && low[c] < 0; ++d)
-                if (Hcm[(size_t)c * D + d]) low[c] = d * 32 + __builtin_ctz(Hcm[(size_t)c * D + d]);
-        for (int c = 0; c < m; ++c) if (low[c] >= 0) ++lgs[low[c] + 1];
-        for (int r = 0; r < n; ++r) lgs[r + 1] += lgs[r];
-        lgc.assign(lgs[n], 0);
-        std::vector<int> fill(lgs.begin(), lgs.end() - 1);
-        for (int c = 0; c < m; ++c) if (low[c] >= 0) lgc[fill[low[c]]++] = c;
-    }
     std::vector<uint32_t> Lp;
     if (family == kCLN) {
         Lp.assign((size_t)m * NL32, 0);
         for (int c = 0; c < m; ++c)
             for (int q = 0; q < NL; ++q)
                 if (L[(size_t)c * NL + q]) Lp[(size_t)c * NL32 + (q >> 5)] |= 1u << (q & 31);
+    }
+
+    // The triage kernel's tables as one image (staged in its LDS when small):
+    // column words [D][m], signatures [m], degeneracy words [m] (when there are
+    // any), the start of each check's group [n + 1], then one record per group
+    // member, 16-byte aligned: the column and its D words, padded to 4 words
+    // (D <= 3) or 8.  A group holds the nonempty columns whose lowest check is
+    // that check: a column whose support lies inside supp(s) is in the group of
+    // one of s's checks, so the lane search tests only those groups.
+    std::vector<uint32_t> timg;
+    int t_sig, t_dg, t_gs, t_rec, t_recw;
+    {
+        std::vector<int> low(m, -1), gs(n + 1, 0);
+        for (int c = 0; c < m; ++c)
+            for (int d = 0; d < D && low[c] < 0; ++d)
+                if (Hcm[(size_t)c * D + d]) low[c] = d * 32 + __builtin_ctz(Hcm[(size_t)c * D + d]);
+        for (int c = 0; c < m; ++c) if (low[c] >= 0) ++gs[low[c] + 1];
+        for (int r = 0; r < n; ++r) gs[r + 1] += gs[r];
+        t_recw = (D + 1 <= 4) ? 4 : 8 * ((D + 8) / 8);
+        timg.assign(Hcw.begin(), Hcw.end());
+        t_sig = (int)timg.size();
+        timg.insert(timg.end(), sig.begin(), sig.end());
+        t_dg = (int)timg.size();
+        timg.insert(timg.end(), dg.begin(), dg.end());
+        t_gs = (int)timg.size();
+        timg.insert(timg.end(), gs.begin(), gs.end());
+        timg.resize((timg.size() + 3) & ~(size_t)3, 0u);
+        t_rec = (int)timg.size();
+        timg.resize(timg.size() + (size_t)gs[n] * t_recw, 0u);
+        std::vector<int> fill(gs.begin(), gs.end() - 1);
+        for (int c = 0; c < m; ++c) {
+            if (low[c] < 0) continue;
+            uint32_t *rec = timg.data() + t_rec + (size_t)fill[low[c]]++ * t_recw;
+            rec[0] = (uint32_t)c;
+            for (int d = 0; d < D; ++d) rec[1 + d] = Hcm[(size_t)c * D + d];
+        }
     }
 
     // One allocation, 256-byte aligned segments.
@@ -384,7 +405,8 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
                  iRs = add(rowsig.data(), rowsig.size() * 4), iOff = add(off.data(), off.size() * 4),
                  iR = add(R.data(), R.size() * 4), iPiv = add(pivrow.data(), pivrow.size() * 4),
                  iL = add(Lp.data(), Lp.size() * 4), iDg = add(dg.data(), dg.size() * 4),
-                 iGs = add(lgs.data(), lgs.size() * 4), iGc = add(lgc.data(), lgc.size() * 4),
+                 iTimg = add(timg.data(), timg.size() * 4),
+
                  iLlr = add(llr ? llr->data() : nullptr, llr ? llr->size() * 4 : 0);
     if (!ce->mem.ensure(total)) return nullptr;
     char *base = ce->mem.as<char>();
@@ -406,8 +428,10 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
     dc.pivrow = (const int *)(base + segs[iPiv].off);
     dc.Lp = (const uint32_t *)(base + segs[iL].off);
     dc.dg = dg.empty() ? nullptr : (const uint32_t *)(base + segs[iDg].off);
-    dc.lgs = (const int *)(base + segs[iGs].off);
-    dc.lgc = (const int *)(base + segs[iGc].off);
+    dc.timg = (const uint32_t *)(base + segs[iTimg].off);
+    dc.timg_words = (int)timg.size();
+    dc.t_sig = t_sig; dc.t_dg = t_dg; dc.t_gs = t_gs; dc.t_rec = t_rec; dc.t_recw = t_recw;
+
     dc.llr = (const float *)(base + segs[iLlr].off);
     dc.nCr = rt().nCr.as<unsigned long long>();
     dc.lds_words = 0;
@@ -493,10 +517,33 @@ struct GrandSession {
     grand::DevMem plain_scratch, plain_state, plain_records, plain_out;
     grand::DevMem dcall;              // device copy of `call`, refreshed before every batch
     grand::DevMem dbg;                // chunk trace buffer (GRAND_DEBUG_TASKS)
+    // HIP-event timing: a ring of kTimingSlots event triples (before triage,
+    // after triage, after search), one per decode.  A slot's times are read
+    // (waiting for its events) when the slot is reused or the times are asked
+    // for, so decodes with timing on still run back to back.
+    static constexpr int kTimingSlots = 64;
     bool timing = false;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};   // before triage, after triage, after search
+    std::vector<hipEvent_t> ev;       // [kTimingSlots * 3]
+    int tslot = -1;                   // slot of the last timed decode
+    unsigned long long tpending = 0;  // bit per slot whose times are not read yet
+    float slot_ms[kTimingSlots][2] = {};
+    double sum_ms[2] = {0.0, 0.0};
+    long long sum_n = 0;
     ~GrandSession() {
         for (hipEvent_t e : ev) if (e) (void)hipEventDestroy(e);
+    }
+    // reads slot k's times into slot_ms and the sums
+    bool harvest(int k) {
+        if (!((tpending >> k) & 1ull)) return true;
+        if (hipEventSynchronize(ev[3 * k + 2]) != hipSuccess) return false;
+        float a = 0.0f, b = 0.0f;
+        if (hipEventElapsedTime(&a, ev[3 * k], ev[3 * k + 1]) != hipSuccess ||
+            hipEventElapsedTime(&b, ev[3 * k + 1], ev[3 * k + 2]) != hipSuccess)
+            return false;
+        slot_ms[k][0] = a; slot_ms[k][1] = b;
+        sum_ms[0] += a; sum_ms[1] += b; ++sum_n;
+        tpending &= ~(1ull << k);
+        return true;
     }
 };
 
@@ -795,7 +842,7 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     // the triage kernel reads the column words, offsets, signatures and
     // degeneracy words of a small code from its LDS (lane search, s = H e)
     {
-        const long long tsw = (long long)(code.D + 3) * code.m + (code.dg ? code.m : 0) + code.n + 1;
+        const long long tsw = code.timg_words;
         c.triage_stage_words = ((c.triage_search || c.input_errors) && tsw <= kCodeLdsMaxWords) ? (int)tsw : 0;
     }
     // one claimed shot in the search kernel's LDS: id, signature, syndrome words,
@@ -832,21 +879,28 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     HIPX(grand_launch_prep(&c, s->dcall.as<DevCall>(), c.counters, st));
     const DevCode *dcode = s->code->dcode.as<DevCode>();
     const DevCall *dcall = s->dcall.as<DevCall>();
-    if (s->timing) HIPX(hipEventRecord(s->ev[0], st));
+    hipEvent_t *tev = nullptr;
+    if (s->timing) {
+        const int k = (s->tslot + 1) % GrandSession::kTimingSlots;
+        if (!s->harvest(k)) return -1;
+        s->tslot = k;
+        tev = s->ev.data() + 3 * k;
+        HIPX(hipEventRecord(tev[0], st));
+    }
     // (the lane search reuses the block as its M32-word active bitmask per lane)
     const int block_words = std::max(std::max(c.input_errors ? code.M32 : code.D, s->family == kCLN ? code.NL32 : 0),
                                      c.triage_search ? code.M32 : 0);
     HIPX(grand_launch_triage(dcode, dcall, tgrid, block_words, c.triage_stage_words, st));
-    if (s->timing) HIPX(hipEventRecord(s->ev[1], st));
+    if (tev) HIPX(hipEventRecord(tev[1], st));
     if (s->ex.engine == GRAND_ENGINE_PLAIN) {
         int rc = run_plain(s, st, num_shots);
         if (rc == -2) { (void)grand_launch_fill(c.counters + 2, kErrTamOverflow, 1, st); rc = 0; }
-        if (s->timing) HIPX(hipEventRecord(s->ev[2], st));
+        if (tev) { HIPX(hipEventRecord(tev[2], st)); s->tpending |= 1ull << s->tslot; }
         return rc;
     }
     const int grid = std::max(1, std::min(R.cu_count * wpc, num_shots));
     HIPX(grand_launch_fused(s->code->dcode_search.as<DevCode>(), dcall, &c, s->code->search_lds_words, grid, st));
-    if (s->timing) HIPX(hipEventRecord(s->ev[2], st));
+    if (tev) { HIPX(hipEventRecord(tev[2], st)); s->tpending |= 1ull << s->tslot; }
     return 0;
 }
 
@@ -1142,21 +1196,40 @@ int grand_session_reserve(GrandSession *s, int num_shots) {
 
 int grand_session_enable_timing(GrandSession *s, int on) {
     if (!s) return -1;
-    if (on && !s->ev[0])
+    if (on && s->ev.empty()) {
+        s->ev.assign(3 * GrandSession::kTimingSlots, nullptr);
         for (hipEvent_t &e : s->ev)
             if (hipEventCreate(&e) != hipSuccess) return -1;
+    }
     s->timing = on != 0;
     return 0;
 }
 
 int grand_session_last_timing(GrandSession *s, float *triage_ms, float *search_ms) {
-    if (!s || !s->timing) return -1;
-    if (hipEventSynchronize(s->ev[2]) != hipSuccess) return -1;
-    float a = 0.0f, b = 0.0f;
-    if (hipEventElapsedTime(&a, s->ev[0], s->ev[1]) != hipSuccess) return -1;
-    if (hipEventElapsedTime(&b, s->ev[1], s->ev[2]) != hipSuccess) return -1;
-    if (triage_ms) *triage_ms = a;
-    if (search_ms) *search_ms = b;
+    if (!s || !s->timing || s->tslot < 0) return -1;
+    if (!s->harvest(s->tslot)) return -1;
+    if (triage_ms) *triage_ms = s->slot_ms[s->tslot][0];
+    if (search_ms) *search_ms = s->slot_ms[s->tslot][1];
+    return 0;
+}
+
+int grand_session_timing_totals(GrandSession *s, double *triage_ms, double *search_ms, long long *decodes) {
+    if (!s || s->ev.empty()) return -1;
+    for (int k = 0; k < GrandSession::kTimingSlots; ++k)
+        if (!s->harvest(k)) return -1;
+    if (triage_ms) *triage_ms = s->sum_ms[0];
+    if (search_ms) *search_ms = s->sum_ms[1];
+    if (decodes) *decodes = s->sum_n;
+    return 0;
+}
+
+int grand_session_timing_reset(GrandSession *s) {
+    if (!s) return -1;
+    if (!s->ev.empty())
+        for (int k = 0; k < GrandSession::kTimingSlots; ++k)
+            if (!s->harvest(k)) return -1;
+    s->sum_ms[0] = s->sum_ms[1] = 0.0;
+    s->sum_n = 0;
     return 0;
 }
 

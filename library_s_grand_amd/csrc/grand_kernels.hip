@@ -1914,23 +1914,27 @@ __device__ uint64_t lane_weight_list(const DevCall &call, int tam) {
     return wl;
 }
 
-// The code tables the triage kernel reads by column: column words (word-major,
-// hcw[d * m + c]), score offsets (kNegScore marks an empty column), signatures,
-// degeneracy words (has_dg) and the columns grouped by their lowest check (gc,
-// group of check r at gs[r] .. gs[r + 1]).  They sit in the wave's LDS when the
-// code is small enough to stage there, else they are the code's global arrays.
-template <class U, class I>
+// The triage kernel's code tables (the image DevCode::timg): column words
+// (word-major, hcw[d * m + c]), signatures, degeneracy words (has_dg), the
+// start of each check's group (gs, n + 1 words) and the group-member records
+// (rec, a column and its D words per record).  In the wave's LDS when the image
+// is small enough to stage there, else read from global memory.
+template <class U>
 struct TriTables {
     U hcw;
-    I off;
     U sig;
     U dg;
-    I gs;
-    I gc;
+    U gs;
+    U rec;
     bool has_dg;
 };
-using TriLds = TriTables<const uint32_t LDS *, const int LDS *>;
-using TriGlobal = TriTables<const uint32_t *, const int *>;
+using TriLds = TriTables<const uint32_t LDS *>;
+using TriGlobal = TriTables<const uint32_t *>;
+template <class U>
+__device__ __forceinline__ TriTables<U> tri_tables(const DevCode &code, U img) {
+    return TriTables<U>{img, img + code.t_sig, img + code.t_dg, img + code.t_gs, img + code.t_rec,
+                        code.dg != nullptr};
+}
 
 template <int D, class T>
 __device__ __forceinline__ bool lane_exact(const T &tb, int m, const uint32_t (&s)[kMaxWords], int a, int b, int c,
@@ -1965,12 +1969,14 @@ __device__ __forceinline__ bool lane_search(const DevCode &code, const DevCall &
         while (bs) {
             const int r = 32 * d + __ffs(bs) - 1;
             bs &= bs - 1u;
-            const int k1 = tb.gs[r + 1];
-            for (int k = tb.gs[r]; k < k1; ++k) {
-                const int c = tb.gc[k];
+            const int k1 = (int)tb.gs[r + 1];
+            for (int k = (int)tb.gs[r]; k < k1; ++k) {
+                // the record: the column, then its words (read together)
+                constexpr int RW = (D + 1 <= 4) ? 4 : 8;
+                const int c = (int)tb.rec[k * RW];
                 uint32_t out = 0u;
 #pragma unroll
-                for (int e = 0; e < D; ++e) out |= tb.hcw[e * m + c] & ~s[e];
+                for (int e = 0; e < D; ++e) out |= tb.rec[k * RW + 1 + e] & ~s[e];
                 if (out == 0u)
                     __hip_atomic_fetch_or(&AB[(c >> 5) * 64 + lane], 1u << (c & 31), __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -2257,6 +2263,21 @@ __device__ __forceinline__ void pack_rows(const int *rows, int ncols, int base, 
     __syncthreads();
 }
 
+// Writes the table image held in registers (sv_vecs 16-byte vectors, vector
+// lane + 64 i in sv[i]) to the wave's LDS.
+constexpr int kTriageStageVec = 8;
+__device__ __forceinline__ void flush_stage(const uint4 (&sv)[kTriageStageVec], int sv_vecs, uint32_t LDS *tab) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int i = 0; i < kTriageStageVec; ++i) {
+        const int q = lane + 64 * i;
+        if (q < sv_vecs) {
+            tab[4 * q] = sv[i].x; tab[4 * q + 1] = sv[i].y; tab[4 * q + 2] = sv[i].z; tab[4 * q + 3] = sv[i].w;
+        }
+    }
+    __syncthreads();
+}
+
 // One lane per shot, 64 shots per wave: packs the input rows (errors or
 // syndromes, and logicals), forms s = H e and sig(s) as XORs over the set error
 // bits (both maps are GF(2)-linear), searches small first iterations by lane,
@@ -2265,7 +2286,8 @@ __device__ __forceinline__ void pack_rows(const int *rows, int ncols, int base, 
 template <class T>
 __device__ __forceinline__ void triage_groups(const DevCode &code, const DevCall &call, const T &tb, int LDS *LA,
                                               uint32_t LDS *LS, const unsigned long long LDS *WL,
-                                              uint32_t LDS *bits) {
+                                              uint32_t LDS *bits, const uint4 (&sv)[kTriageStageVec], int sv_vecs,
+                                              uint32_t LDS *tab) {
     const int lane = lane_id();
     const int m = code.m;
     const int groups = (call.num_shots + 63) / 64;
@@ -2283,6 +2305,7 @@ __device__ __forceinline__ void triage_groups(const DevCode &code, const DevCall
         if (call.input_errors) {
             const int M32 = code.M32;
             pack_rows(call.input, m, base, cnt, M32, bits, bad);
+            if (g == (int)blockIdx.x) flush_stage(sv, sv_vecs, tab);
             // the block's packed rows are consecutive in e_packed
             for (int i = lane; i < cnt * M32; i += 64) call.e_packed[(size_t)base * M32 + i] = bits[i];
             for (int wd = 0; wd < M32; ++wd) {
@@ -2301,6 +2324,7 @@ __device__ __forceinline__ void triage_groups(const DevCode &code, const DevCall
         } else {
             const int D = code.D;
             pack_rows(call.input, code.n, base, cnt, D, bits, bad);
+            if (g == (int)blockIdx.x) flush_stage(sv, sv_vecs, tab);
 #pragma unroll
             for (int d = 0; d < kMaxWords; ++d) {
                 if (d < D) {
@@ -2382,9 +2406,7 @@ __device__ __forceinline__ void triage_groups(const DevCode &code, const DevCall
 
 // LDS: the lane search's [kLaneMax + 1][64] positions and signatures and its weight
 // lists by active-list size [kTriageWlWords / 2]; the code tables when
-// call.triage_stage_words > 0 (column words [D][m], offsets [m],
-// signatures [m], degeneracy words [m] when the code has them, lowest-check
-// group starts [n + 1] and grouped columns [at most m]); then the packed
+// call.triage_stage_words > 0 (the image DevCode::timg); then the packed
 // block, 64 rows of the larger of the input row's words (M32 for errors, D for
 // syndromes) and, for the DEM family, the NL32 words of the logicals.
 __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ codep, const DevCall *__restrict__ callp) {
@@ -2396,33 +2418,30 @@ __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ 
     unsigned long long LDS *WL = (unsigned long long LDS *)(lds_base() + 2 * (kLaneMax + 1) * 64);
     uint32_t LDS *tab = lds_base() + 2 * (kLaneMax + 1) * 64 + kTriageWlWords;
     if (lane <= kLaneMax) WL[lane] = (lane == 0) ? 0ull : lane_weight_list(call, lane);
-    const int tsw = call.triage_stage_words;
+    const int tsw = call.triage_stage_words;          // a multiple of 4
+    uint4 sv[kTriageStageVec];
     if (tsw > 0) {
-        const int m = code.m, nw = code.D * m;
-#pragma unroll 8
-        for (int i = lane; i < nw; i += 64) tab[i] = code.Hcw[i];
-#pragma unroll 4
-        for (int i = lane; i < m; i += 64) {
-            tab[nw + i] = (uint32_t)code.off[i];
-            tab[nw + m + i] = code.sig[i];
+        // An image of up to 256 * kTriageStageVec words is loaded here and
+        // written to LDS after the first block is packed (its loads overlap the
+        // block's); a larger one is staged now.
+        const uint4 *src = reinterpret_cast<const uint4 *>(code.timg);
+        const int nv = tsw >> 2;
+        if (nv <= 64 * kTriageStageVec) {
+#pragma unroll
+            for (int i = 0; i < kTriageStageVec; ++i) {
+                const int q = lane + 64 * i;
+                sv[i] = (q < nv) ? src[q] : make_uint4(0u, 0u, 0u, 0u);
+            }
+        } else {
+            for (int q = lane; q < nv; q += 64) {
+                const uint4 v = src[q];
+                tab[4 * q] = v.x; tab[4 * q + 1] = v.y; tab[4 * q + 2] = v.z; tab[4 * q + 3] = v.w;
+            }
         }
-        const bool has_dg = code.dg != nullptr;
-        if (has_dg)
-#pragma unroll 4
-            for (int i = lane; i < m; i += 64) tab[nw + 2 * m + i] = code.dg[i];
-        uint32_t LDS *gs = tab + nw + (has_dg ? 3 : 2) * m;
-        uint32_t LDS *gc = gs + code.n + 1;
-        const int ng = code.lgs[code.n];
-#pragma unroll 4
-        for (int i = lane; i <= code.n; i += 64) gs[i] = (uint32_t)code.lgs[i];
-#pragma unroll 4
-        for (int i = lane; i < ng; i += 64) gc[i] = (uint32_t)code.lgc[i];
-        const TriLds tb{tab, (const int LDS *)(tab + nw), tab + nw + m, tab + nw + 2 * m, (const int LDS *)gs,
-                        (const int LDS *)gc, has_dg};
-        triage_groups(code, call, tb, LA, LS, WL, tab + tsw);   // its packing starts with a barrier
+        triage_groups(code, call, tri_tables(code, (const uint32_t LDS *)tab), LA, LS, WL, tab + tsw, sv,
+                      nv <= 64 * kTriageStageVec ? nv : 0, tab);
     } else {
-        const TriGlobal tb{code.Hcw, code.off, code.sig, code.dg, code.lgs, code.lgc, code.dg != nullptr};
-        triage_groups(code, call, tb, LA, LS, WL, tab);
+        triage_groups(code, call, tri_tables(code, code.timg), LA, LS, WL, tab, sv, 0, tab);
     }
 }
 

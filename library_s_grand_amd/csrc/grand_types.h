@@ -42,8 +42,13 @@ struct DevCode {
     const int *pivrow;                // [m] RREF row whose pivot is this column, -1 for other columns
     const uint32_t *Lp;               // [m][NL32] logical flips per mechanism (cln)
     const uint32_t *dg;               // [m] degeneracy words (cc), or null when more than 32 are needed
-    const int *lgs;                   // [n + 1] start of each check's group in lgc
-    const int *lgc;                   // columns grouped by their lowest check (empty columns left out)
+    // the triage kernel's tables as one image of timg_words words: column words
+    // [D][m] at 0, signatures [m] at t_sig, degeneracy words [m] at t_dg (when
+    // dg is set), group starts [n + 1] at t_gs, group-member records of t_recw
+    // words (column, then its D words) at t_rec; groups: nonempty columns by
+    // their lowest check
+    const uint32_t *timg;
+    int timg_words, t_sig, t_dg, t_gs, t_rec, t_recw;
     const float *llr;                 // [m] (ML)
     const unsigned long long *nCr;    // [(kMaxTam + 1) * kNcrStride]
     // Small codes are staged in the search kernel's LDS.  The search kernel gets
@@ -126,9 +131,8 @@ struct DevCall {
     int split_worklist;               // 1 (search kernel): worklist in two segments, likely-deep shots first
     int wl_parts;                     // worklist partitions (1 for the plain engine, else kWlParts)
     int pair_cap;                     // search kernel: pairs the LDS pair table holds (0 = no pair table)
-    int triage_stage_words;           // triage kernel: words of code tables staged in LDS (column words,
-                                      // offsets, signatures, degeneracy words, lowest-check groups),
-                                      // 0 = read from global memory
+    int triage_stage_words;           // triage kernel: words of its table image staged in LDS
+                                      // (DevCode::timg), 0 = read from global memory
     int stage_words;                  // search kernel: LDS words per claimed shot: id, signature, syndrome,
                                       // then original weight, degeneracy word and error words (cc with
                                       // errors) or the observed logicals (cln)

@@ -34,6 +34,9 @@ def bind_session(lib):
     lib.grand_session_error_bits.argtypes = [_VP, _VP]
     lib.grand_session_enable_timing.argtypes = [_VP, ctypes.c_int]
     lib.grand_session_last_timing.argtypes = [_VP, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+    lib.grand_session_timing_totals.argtypes = [_VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                                ctypes.POINTER(ctypes.c_longlong)]
+    lib.grand_session_timing_reset.argtypes = [_VP]
     lib.grand_session_ms_per_tick.argtypes = [_VP]
     lib.grand_session_ms_per_tick.restype = ctypes.c_double
     lib.grand_session_pattern_words.argtypes = [_VP]
@@ -127,6 +130,18 @@ class Session:
         if self.lib.grand_session_last_timing(self.h, ctypes.byref(a), ctypes.byref(b)) != 0:
             raise RuntimeError("timing not enabled")
         return a.value, b.value
+
+    def timing_totals(self):
+        """(triage ms, search ms, decodes) summed over the timed decodes since the
+        last timing_reset; waits for all of them."""
+        a, b, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        if self.lib.grand_session_timing_totals(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)) != 0:
+            raise RuntimeError("timing not enabled")
+        return a.value, b.value, n.value
+
+    def timing_reset(self):
+        if self.lib.grand_session_timing_reset(self.h) != 0:
+            raise RuntimeError("timing reset failed")
 
     def footprint(self, shots):
         return self.lib.grand_session_footprint(self.h, shots)
