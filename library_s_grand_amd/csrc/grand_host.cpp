@@ -855,7 +855,7 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     if (s->ex.engine != GRAND_ENGINE_PLAIN && s->search != kML) {
         const size_t base = grand_fused_lds_bytes(c.cap, c.lscores != nullptr) +
                             4 * ((size_t)((s->code->search_lds_words + 3) & ~3) + (size_t)c.claim_batch * c.stage_words) +
-                            4 * ((size_t)kPairBitmapWords + kPairBuckets);
+                            4 * ((size_t)kPairBitmapWords + kPairHeadWords);
         const size_t budget = std::min<size_t>(65536, kLdsPerCu / (size_t)std::max(wpc, 1));
         const long long all = (long long)c.cap * (c.cap - 1) / 2;
         if (budget > base) {
@@ -898,7 +898,10 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
         if (tev) { HIPX(hipEventRecord(tev[2], st)); s->tpending |= 1ull << s->tslot; }
         return rc;
     }
-    const int grid = std::max(1, std::min(R.cu_count * wpc, num_shots));
+    // a deep shot of a small batch shares its classes with the other waves, so
+    // with shared tasks on the grid keeps at least one wave per CU
+    const int want = c.split_threshold ? std::max(num_shots, R.cu_count) : num_shots;
+    const int grid = std::max(1, std::min(R.cu_count * wpc, want));
     HIPX(grand_launch_fused(s->code->dcode_search.as<DevCode>(), dcall, &c, s->code->search_lds_words, grid, st));
     if (tev) { HIPX(hipEventRecord(tev[2], st)); s->tpending |= 1ull << s->tslot; }
     return 0;

@@ -229,7 +229,7 @@ __device__ __forceinline__ WaveLds carve_lds(int cap, bool ml, int code_words = 
     }
     if (pair_cap > 0) {
         L.pbm = p; p += kPairBitmapWords;
-        L.phead = p; p += kPairBuckets;
+        L.phead = p; p += kPairHeadWords;
         L.pkey = p; p += pair_cap;
         L.pval = p; p += pair_cap;
     } else {
@@ -671,17 +671,17 @@ __device__ __attribute__((noinline)) void build_pairs_lds(uint32_t LDS *phead, u
             const bool ok = k + u < k1;
             key[u] = sg[ok ? yy[u] : 0] ^ sg[ok ? xx[u] : 1];
         }
+        // Unconditional atomics (a conditional one becomes a branch that waits for
+        // its result): past the block's end a lane ORs 0 into the filter and
+        // exchanges on the spare head kPairBuckets, which no lookup reads.
         uint32_t nxt[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const bool ok = k + u < k1;
-            if (ok)
-                __hip_atomic_fetch_or(&pbm[(key[u] & 0x7FFFu) >> 5], 1u << (key[u] & 31u), __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_WORKGROUP);
-            // past the block's end the exchange puts back what it reads (no change)
-            uint32_t LDS *hp = &phead[ok ? (key[u] >> 16) & (unsigned)(kPairBuckets - 1) : 0u];
-            nxt[u] = ok ? __hip_atomic_exchange(hp, (uint32_t)(k + u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
-                        : 0u;
+            __hip_atomic_fetch_or(&pbm[(key[u] & 0x7FFFu) >> 5], ok ? 1u << (key[u] & 31u) : 0u, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+            uint32_t LDS *hp = &phead[ok ? (key[u] >> 16) & (unsigned)(kPairBuckets - 1) : (unsigned)kPairBuckets];
+            nxt[u] = __hip_atomic_exchange(hp, (uint32_t)(k + u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -876,14 +876,27 @@ __device__ __forceinline__ float key_to_float(uint32_t key) {
 }
 
 // Claims the next chunk of slot T for this wave; returns it or ~0u.  Lane 0
-// issues the fetch-and-add; every exit depends only on wave-uniform values.
-__device__ __forceinline__ unsigned claim_chunk(TaskSlot *T, unsigned *open_count, unsigned &gen) {
+// issues the fetch-and-add and, behind it in the same round trip, the loads of
+// the task's launch-unique id and of best_q (it only decreases: a stale value
+// at worst searches a chunk past a known hit).  Every exit depends only on
+// wave-uniform values.
+__device__ __forceinline__ unsigned claim_chunk(TaskSlot *T, unsigned *open_count, unsigned &gen,
+                                                unsigned long long &bq, unsigned &uid) {
     gen = 0;
+    bq = ~0ull;
+    uid = 0;
     const unsigned long long snap = uni64(ld_relaxed(&T->ctl));
     if (ctl_next(snap) >= ctl_nch(snap)) return ~0u;
-    unsigned long long old = 0;
-    if (lane_id() == 0) old = __hip_atomic_fetch_add(&T->ctl, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long old = 0, b = ~0ull;
+    unsigned u = 0;
+    if (lane_id() == 0) {
+        old = __hip_atomic_fetch_add(&T->ctl, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        b = ld_relaxed(&T->best_q);
+        u = ld_relaxed(&T->gen);
+    }
     old = uni64(old);
+    bq = uni64(b);
+    uid = uni(u);
     if (ctl_next(old) >= ctl_nch(old)) return ~0u;
     const unsigned ch = ctl_next(old);
     if (lane_id() == 0 && ch + 1u == ctl_nch(old)) atomicSub(open_count, 1u);
@@ -915,35 +928,37 @@ __device__ __forceinline__ int class_chunk_r(const DevCall &call, uint64_t units
 // its unit index (so that chunks past a known hit are skipped).  ML (one
 // class): atomic-min of (cost key << 32 | rank).  Returns the chunk's result
 // (~0 = none).
+// The task's descriptor comes in registers: nclass, wpack, ppack, and per lane
+// cch (lane i: first chunk of class i, lane nclass: the chunk count) and cR
+// (lane i: units per lane per chunk of class i); bq is a recent best_q.
 __device__ unsigned long long task_chunk(const DevCode &code, TaskSlot *T, const uint32_t LDS *s, uint32_t t0,
-                                        int tam, bool ml, unsigned ch, const int LDS *act, const uint32_t LDS *sg,
-                                        const float LDS *llr, const unsigned long long LDS *ht, unsigned hmask,
+                                        int tam, bool ml, unsigned ch, unsigned long long bq, int nclass,
+                                        unsigned wpack, unsigned ppack, unsigned cch, int cR, const int LDS *act,
+                                        const uint32_t LDS *sg, const float LDS *llr,
+                                        const unsigned long long LDS *ht, unsigned hmask,
                                         const uint32_t LDS *pkey, const uint32_t LDS *pval,
                                         const uint32_t LDS *phead, const uint32_t LDS *pbm) {
     const int lane = lane_id();
-    const int nclass = (int)uni((unsigned)ld_relaxed(&T->nclass));
-    const unsigned wpack = uni((unsigned)ld_relaxed(&T->wpack)), ppack = uni((unsigned)ld_relaxed(&T->ppack));
-    int w = 0, R = 1, k = 0;
+    int w = 0, R = 1;
     bool pairs = false;
     uint64_t ubase = 0, pbase = 0, NP = 0;
     unsigned ch0 = 0;
     for (int i = 0; i < nclass; ++i) {
         const int wi = (int)((wpack >> (4 * i)) & 15u);
         const bool pi = ((ppack >> i) & 1u) != 0;
-        const unsigned c1 = uni(ld_relaxed(&T->cch[i + 1]));
+        const unsigned c1 = (unsigned)__builtin_amdgcn_readlane((int)cch, i + 1);
         if (ch < c1) {
-            k = i; w = wi; pairs = pi;
-            ch0 = uni(ld_relaxed(&T->cch[i]));
-            R = (int)uni((unsigned)ld_relaxed(&T->cR[i]));
+            w = wi; pairs = pi;
+            ch0 = (unsigned)__builtin_amdgcn_readlane((int)cch, i);
+            R = __builtin_amdgcn_readlane(cR, i);
             NP = class_units(code, tam, wi, pi);
             break;
         }
         ubase += class_units(code, tam, wi, pi);
         pbase += ncr_w(code, tam, wi);
     }
-    (void)k;
     const uint64_t base = (uint64_t)(ch - ch0) * 64ull * (uint64_t)R;
-    if (!ml && ubase + base > uni64(ld_relaxed(&T->best_q))) return ~0ull - 1;
+    if (!ml && ubase + base > bq) return ~0ull - 1;
     const uint64_t q0 = base + (uint64_t)lane * (uint64_t)R;
     const int len = q0 >= NP ? 0 : (int)((NP - q0) < (uint64_t)R ? (NP - q0) : (uint64_t)R);
     const PrefHit h = ml ? scan_prefix_w<1>(w, code, s, t0, q0, len, tam, act, sg, llr, ht, hmask)
@@ -1054,12 +1069,14 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
         __hip_atomic_exchange(&T->ctl, make_ctl(gen, nch), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         atomicAdd(&call.task_counters[0], 1u);
     }
+    const unsigned cchv = (lane < nclass) ? myc : (lane == nclass ? nch : 0u);
     for (;;) {
-        unsigned g;
-        const unsigned ch = claim_chunk(T, &call.task_counters[0], g);
+        unsigned g, u;
+        unsigned long long bq;
+        const unsigned ch = claim_chunk(T, &call.task_counters[0], g, bq, u);
         if (ch == ~0u) break;
-        const unsigned long long res =
-            task_chunk(code, T, s, t0, tam, ml, ch, act, sg, llr, ht, hmask, pkey, pval, phead, pbm);
+        const unsigned long long res = task_chunk(code, T, s, t0, tam, ml, ch, bq, nclass, wpack, ppack, cchv, myR,
+                                                  act, sg, llr, ht, hmask, pkey, pval, phead, pbm);
         trace_chunk(call, 0, slot, uid, ch, (int)(wpack & 15u), tam, nclass, nch, t0, s, act, sg, res);
         wait_mem();                               // the chunk's result atomics complete before it counts as done
         if (lane == 0) atomicAdd(&T->done, 1u);
@@ -1074,8 +1091,19 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
 // Helper side: claim one chunk of any open task and process it.  The task's
 // arrays are copied into this wave's LDS and its tables (hash table and/or pair
 // table, as its classes need) built once per task.
-__device__ __forceinline__ bool help_task(const DevCode &code, const DevCall &call, const WaveLds &L, int &ctask,
-                                          unsigned &hmask) {
+// The task a helper wave has in its LDS (arrays, syndrome words, tables) and
+// its descriptor in registers; key = the task's launch-unique id (-1: none).
+struct HelpCache {
+    int key;
+    unsigned hmask;
+    int tam, ml, nclass;
+    unsigned wpack, ppack;
+    unsigned cch;                     // lane i: first chunk of class i; lane nclass: chunk count
+    int cR;                           // lane i: units per lane per chunk of class i
+    uint32_t t0;
+};
+
+__device__ __forceinline__ bool help_task(const DevCode &code, const DevCall &call, const WaveLds &L, HelpCache &hc) {
     const int lane = lane_id();
     if (__builtin_amdgcn_readfirstlane(ld_relaxed(&call.task_counters[0])) == 0u) return false;
     static_assert(kTaskSlots == 128, "the open-slot scan reads two 64-slot halves");
@@ -1089,39 +1117,54 @@ __device__ __forceinline__ bool help_task(const DevCode &code, const DevCall &ca
     else if (b1) slot = (int)uni((unsigned)(start + 64 + __ffsll((long long)b1) - 1) & (kTaskSlots - 1));
     if (slot < 0) return false;
     TaskSlot *T = &call.tasks[slot];
-    unsigned g;
-    const unsigned ch = claim_chunk(T, &call.task_counters[0], g);
+    unsigned g, uid;
+    unsigned long long bq;
+    const unsigned ch = claim_chunk(T, &call.task_counters[0], g, bq, uid);
     if (ch == ~0u) return true;
-    // the task was complete in memory before its ctl was published; device-coherent
-    // loads read it after the claim
-    const int tam = ld_relaxed(&T->tam);
-    const unsigned uid = uni(ld_relaxed(&T->gen));
-    const bool ml = ld_relaxed(&T->ml) != 0;
-    const int nclass = ld_relaxed(&T->nclass);
-    const unsigned wpack = (unsigned)ld_relaxed(&T->wpack), ppack = (unsigned)ld_relaxed(&T->ppack);
-    const uint32_t t0 = ld_relaxed(&T->t0);
-    if (lane < kMaxWords) L.sw[lane] = ld_relaxed(&T->s[lane]);
-    __syncthreads();
-    const uint32_t LDS *s = L.sw;
-    int LDS *act = ml ? L.P : L.A;
-    uint32_t LDS *sg = ml ? L.sigP : L.sigA;
-    if (ctask != (int)uid) {
-        uint32_t *arr = call.task_arrays + (size_t)slot * 3 * call.cap;
-        for (int i = lane; i < tam; i += 64) {
+    const int key = (int)(uid & 0x7FFFFFFFu);
+    if (hc.key != key) {
+        // the task was complete in memory before its ctl was published; device-coherent
+        // loads read it after the claim (all issued together)
+        hc.tam = ld_relaxed(&T->tam);
+        hc.ml = ld_relaxed(&T->ml);
+        hc.nclass = ld_relaxed(&T->nclass);
+        hc.wpack = (unsigned)ld_relaxed(&T->wpack);
+        hc.ppack = (unsigned)ld_relaxed(&T->ppack);
+        hc.t0 = ld_relaxed(&T->t0);
+        hc.cch = (lane <= kTaskClasses) ? ld_relaxed(&T->cch[lane]) : 0u;
+        hc.cR = (lane < kTaskClasses) ? ld_relaxed(&T->cR[lane]) : 1;
+        const uint32_t sword = (lane < kMaxWords) ? ld_relaxed(&T->s[lane]) : 0u;
+        hc.tam = uni((unsigned)hc.tam);
+        hc.ml = uni((unsigned)hc.ml);
+        hc.nclass = uni((unsigned)hc.nclass);
+        hc.wpack = uni(hc.wpack);
+        hc.ppack = uni(hc.ppack);
+        hc.t0 = uni(hc.t0);
+        if (lane < kMaxWords) L.sw[lane] = sword;
+        const bool ml = hc.ml != 0;
+        int LDS *act = ml ? L.P : L.A;
+        uint32_t LDS *sg = ml ? L.sigP : L.sigA;
+        const uint32_t *arr = call.task_arrays + (size_t)slot * 3 * call.cap;
+        for (int i = lane; i < hc.tam; i += 64) {
             act[i] = (int)ld_relaxed(&arr[i]);
             sg[i] = ld_relaxed(&arr[call.cap + i]);
             if (ml) L.llrP[i] = __uint_as_float(ld_relaxed(&arr[2 * call.cap + i]));
         }
         __syncthreads();
-        const unsigned all = (nclass >= 32) ? ~0u : ((1u << nclass) - 1u);
-        if ((ppack & all) != 0u) build_pairs(L, sg, tam);
-        if ((ppack & all) != all) hmask = build_ht(L.ht, sg, tam);
-        ctask = (int)uid;
+        const unsigned all = (hc.nclass >= 32) ? ~0u : ((1u << hc.nclass) - 1u);
+        if ((hc.ppack & all) != 0u) build_pairs(L, sg, hc.tam);
+        if ((hc.ppack & all) != all) hc.hmask = build_ht(L.ht, sg, hc.tam);
+        hc.key = key;
     }
-    const unsigned long long res = task_chunk(code, T, s, t0, tam, ml, ch, act, sg, ml ? L.llrP : nullptr, L.ht,
-                                              hmask, L.pkey, L.pval, L.phead, L.pbm);
-    trace_chunk(call, 1, slot, uid, ch, (int)(wpack & 15u), tam, nclass, ld_relaxed(&T->nchunks), t0, s, act, sg,
-                res);
+    const bool ml = hc.ml != 0;
+    const uint32_t LDS *s = L.sw;
+    const int LDS *act = ml ? L.P : L.A;
+    const uint32_t LDS *sg = ml ? L.sigP : L.sigA;
+    const unsigned long long res =
+        task_chunk(code, T, s, hc.t0, hc.tam, ml, ch, bq, hc.nclass, hc.wpack, hc.ppack, hc.cch, hc.cR, act, sg,
+                   ml ? L.llrP : nullptr, L.ht, hc.hmask, L.pkey, L.pval, L.phead, L.pbm);
+    trace_chunk(call, 1, slot, (unsigned)key, ch, (int)(hc.wpack & 15u), hc.tam, hc.nclass,
+                (unsigned)__builtin_amdgcn_readlane((int)hc.cch, hc.nclass), hc.t0, s, act, sg, res);
     wait_mem();                                   // the chunk's result atomics complete before it counts as done
     if (lane == 0) atomicAdd(&T->done, 1u);
     return true;
@@ -2496,8 +2539,9 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
     // there are not several claims per wave (small batches finish sooner)
     unsigned B = total / (4u * gridDim.x);
     B = B < 1u ? 1u : (B > (unsigned)call.claim_batch ? (unsigned)call.claim_batch : B);
-    int ctask = -1;        // task whose arrays and hash table this wave's LDS holds
-    unsigned hmask = 0;
+    HelpCache hc;          // task whose arrays and tables this wave's LDS holds
+    hc.key = -1;
+    hc.hmask = 0;
     {
         // 17 loads per lane, issued together
         constexpr int kN = kNcrLdsRows * kNcrStride;
@@ -2541,7 +2585,7 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
         // shared classes of deep shots first: they bound the batch's completion time
         bool helped = false;
         if (open != 0u)
-            while (help_task(code, call, L, ctask, hmask)) helped = true;
+            while (help_task(code, call, L, hc)) helped = true;
         if (!drained) {
             ++claims;
             const unsigned p0 = (unsigned)cur + kClaimWords * got;
@@ -2550,7 +2594,7 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
                 const int nb = (int)(left < B ? left : B);
                 stage_claim(code, call, p0, nb, heavy, wfx, wbx, L);
                 for (int j = 0; j < nb; ++j) decode_shot_wave(code, call, L.stage + j * call.stage_words, L);
-                ctask = -1;
+                hc.key = -1;
                 continue;
             }
             if (++dry < kClaimWords) { cur = (cur + 1) & (kClaimWords - 1); continue; }
@@ -2853,7 +2897,7 @@ GRAND_INTERNAL hipError_t grand_launch_fused(const DevCode *code, const DevCall 
                               int grid, hipStream_t st) {
     const size_t lds = grand_fused_lds_bytes(hcall->cap, hcall->kind == grand::kML || hcall->lscores != nullptr) +
                        sizeof(uint32_t) * ((size_t)((code_lds_words + 3) & ~3) + (size_t)hcall->claim_batch * hcall->stage_words +
-                                           (hcall->pair_cap > 0 ? (size_t)grand::kPairBitmapWords + grand::kPairBuckets +
+                                           (hcall->pair_cap > 0 ? (size_t)grand::kPairBitmapWords + grand::kPairHeadWords +
                                                                       2 * (size_t)hcall->pair_cap
                                                                 : 0));
     hipLaunchKernelGGL(grand::fused_kernel, dim3(grid), dim3(64), lds, st, code, call);

@@ -71,6 +71,7 @@ constexpr int kMaxBatch = 8;             // most shots per worklist claim
 constexpr int kPairBuckets = 2048;       // buckets, by signature bits 16..26
 constexpr int kPairBitmapWords = 1024;   // 2^15-bit filter, by signature bits 0..14
 constexpr unsigned kPairNone = 0xFFFu;   // end of a bucket chain; also the most pairs a table holds
+constexpr int kPairHeadWords = kPairBuckets + 4;   // bucket heads, a spare head (never looked up), padding
 constexpr size_t kLdsPerCu = 160 * 1024; // gfx950
 // Search of a shot by its own lane in the triage kernel (first iteration only)
 constexpr int kLaneMax = 12;             // active-list entries a lane keeps

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import library_s_grand_amd as G
+from library_s_grand_amd import codes
 import grandtest as T
 
 pytestmark = pytest.mark.gpu
@@ -219,3 +220,24 @@ def test_pair_table_deep_classes(oracle, product, tmp_path, wpc):
             p = runner(product, *args, ex=G.make_ex(waves_per_cu=wpc, split_threshold=split), **kw)
             T.assert_same(o, p, f"{what} wpc={wpc} split={split}")
         print(what, "wpc", wpc, "max count", int(o[1]["total_tested_combinations"].max()), flush=True)
+
+
+def test_deep_shots_in_small_batches(oracle, product, tmp_path):
+    """The deepest shots of the BB144 prob_cc bench workload (1.2e6 to 2.7e7 patterns each, three
+    iterations) decoded alone and as a batch of three: a batch smaller than the CU count still
+    launches helper waves for its shared tasks; results bit-exact with the oracle."""
+    hx, hz = codes.bb(144)
+    seed = 20240501
+    p = codes.log_uniform(np.random.default_rng(seed), 1e-3, 1e-2, hx.shape[1])
+    errors = codes.sample_errors(np.random.default_rng(seed + 1), 100_000, p)
+    pf = T.phase_file(tmp_path)
+    cfg = G.make_config(max_error_weight=4)
+    deep = [79335, 37877]
+    want = T.run_cc(oracle, G.KIND_PROB_CC, hx, hz, errors=np.ascontiguousarray(errors[deep]), cfg=cfg,
+                    prob_file=pf)
+    assert int(want[1]["total_tested_combinations"].max()) > 10_000_000
+    for sel in ([0], [1], [0, 1]):
+        got = T.run_cc(product, G.KIND_PROB_CC, hx, hz, errors=np.ascontiguousarray(errors[[deep[i] for i in sel]]),
+                       cfg=cfg, prob_file=pf)
+        sub = (want[0], want[1][sel], None if want[2] is None else want[2][sel], None)
+        T.assert_same(sub, got, f"deep shots {sel}")
