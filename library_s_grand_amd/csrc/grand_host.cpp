@@ -937,6 +937,7 @@ void print_shot(int family, bool with_errors, int shot, const GrandDecoderResult
 int report_error_bits(unsigned bits) {
     if (bits & kErrNonBinary) fprintf(stderr, "grand: input rows must hold only 0 and 1\n");
     if (bits & kErrTamOverflow) fprintf(stderr, "grand: an active set exceeds the supported %d positions\n", kMaxTam);
+    if (bits & kErrTaskWait) fprintf(stderr, "grand: a shared search task did not complete\n");
     return bits ? -1 : 0;
 }
 

@@ -294,7 +294,7 @@ __device__ __forceinline__ bool exact_match(const DevCode &code, const uint32_t 
 // Writes the first min(tam, cap) entries; returns the full tam.
 template <class PW, class PO, class PS>
 __device__ __forceinline__ int build_active_t(PW Hcw, PO off, PS sig, int D, int m, const uint32_t LDS *s, int thr,
-                                              const WaveLds &L, int cap) {
+                                              int LDS *A, uint32_t LDS *sigA, int cap) {
     const int lane = lane_id();
     int tam = 0;
     for (int c0 = 0; c0 < m; c0 += 64) {
@@ -314,7 +314,7 @@ __device__ __forceinline__ int build_active_t(PW Hcw, PO off, PS sig, int D, int
         const uint64_t b = ballot(on);
         if (on) {
             const int p = tam + lanes_below(b);
-            if (p < cap) { L.A[p] = c; L.sigA[p] = sig[c]; }
+            if (p < cap) { A[p] = c; sigA[p] = sig[c]; }
         }
         tam += __popcll(b);
     }
@@ -322,11 +322,17 @@ __device__ __forceinline__ int build_active_t(PW Hcw, PO off, PS sig, int D, int
     return tam;
 }
 
-__device__ int build_active(const DevCode &code, const uint32_t LDS *s, int thr, const WaveLds &L, int cap) {
+// Active list at threshold thr into A / sigA (the wave's LDS).
+__device__ int build_active_into(const DevCode &code, const uint32_t LDS *s, int thr, int LDS *A, uint32_t LDS *sigA,
+                                 int cap) {
     if (code.lds_words)
         return build_active_t(code_lds(), (const int LDS *)code_lds_off(code), code_lds_sig(code), code.D, code.m, s,
-                              thr, L, cap);
-    return build_active_t(code.Hcw, code.off, code.sig, code.D, code.m, s, thr, L, cap);
+                              thr, A, sigA, cap);
+    return build_active_t(code.Hcw, code.off, code.sig, code.D, code.m, s, thr, A, sigA, cap);
+}
+__device__ __forceinline__ int build_active(const DevCode &code, const uint32_t LDS *s, int thr, const WaveLds &L,
+                                            int cap) {
+    return build_active_into(code, s, thr, L.A, L.sigA, cap);
 }
 
 // Stable sort of A[0..tam) by llr (ties keep active-list order), the order of
@@ -1005,17 +1011,46 @@ __device__ void trace_chunk(const DevCall &call, int who, int slot, unsigned gen
     r[15] = (unsigned)now_ticks();
 }
 
-// Owner side: publish nclass consecutive weight classes of the list act/sg/llr
-// (this wave's LDS; weights in wpack, 4 bits each; bit i of ppack: class i by
-// stems) as one shared task, work on it, wait for every chunk and return the
-// result in best (~0 = no match): first-hit, the lowest matching pattern index
-// over the classes concatenated in order; ML (one class), the lowest (cost key,
-// rank).  Returns false when no slot is free (the caller searches alone).
-__device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, const uint32_t LDS *s, uint32_t t0,
-                           int tam, bool ml, int nclass, unsigned wpack, unsigned ppack, const int LDS *act,
-                           const uint32_t LDS *sg, const float LDS *llr, const unsigned long long LDS *ht, unsigned hmask,
-                           const uint32_t LDS *pkey, const uint32_t LDS *pval,
-                           const uint32_t LDS *phead, const uint32_t LDS *pbm, unsigned long long &best) {
+// Chunk ranges of a task's classes (lane i < nclass: first chunk and units per
+// lane per chunk of class i); returns the chunk count.
+__device__ __forceinline__ unsigned task_ranges(const DevCode &code, const DevCall &call, int tam, int nclass,
+                                                unsigned wpack, unsigned ppack, unsigned &myc, int &myR) {
+    const int lane = lane_id();
+    unsigned nch = 0;
+    myc = 0;
+    myR = 1;
+    for (int i = 0; i < nclass; ++i) {
+        const int wi = (int)((wpack >> (4 * i)) & 15u);
+        const bool pi = ((ppack >> i) & 1u) != 0;
+        const uint64_t u = class_units(code, tam, wi, pi);
+        const int Ri = class_chunk_r(call, u, pi);
+        if (lane == i) { myc = nch; myR = Ri; }
+        nch += (unsigned)((u + 64ull * (uint64_t)Ri - 1) / (64ull * (uint64_t)Ri));
+    }
+    return nch;
+}
+
+// Waits until every chunk of T is done (bounded: past the limit the kernel
+// raises kErrTaskWait instead of waiting on).
+constexpr unsigned kTaskWaitLimit = 1u << 22;
+__device__ __forceinline__ void wait_task_done(const DevCall &call, TaskSlot *T, unsigned nch) {
+    for (unsigned spin = 0; uni(ld_relaxed(&T->done)) < nch; ++spin) {
+        if (spin >= kTaskWaitLimit) {
+            if (lane_id() == 0) atomicOr(&call.counters[2], kErrTaskWait);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Owner side: publishes nclass consecutive weight classes of the list act/sg/llr
+// (weights in wpack, 4 bits each; bit i of ppack: class i by stems) as one
+// shared task and returns its slot, or -1 when no slot is free.  The task is
+// complete in memory before its ctl is published; helpers build their own
+// tables from the copied list.
+__device__ int publish_shared(const DevCode &code, const DevCall &call, int shot, const uint32_t LDS *s, uint32_t t0,
+                              int tam, bool ml, int nclass, unsigned wpack, unsigned ppack, const int LDS *act,
+                              const uint32_t LDS *sg, const float LDS *llr) {
     const int lane = lane_id();
     // find a free slot: 64 candidates per step, lane 0 takes the first free one
     int slot = -1;
@@ -1032,7 +1067,7 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
         }
         if (slot >= 0) break;
     }
-    if (slot < 0) return false;
+    if (slot < 0) return -1;
     TaskSlot *T = &call.tasks[slot];
     uint32_t *arr = call.task_arrays + (size_t)slot * 3 * call.cap;
     for (int i = lane; i < tam; i += 64) {
@@ -1040,17 +1075,9 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
         st_relaxed(&arr[call.cap + i], (uint32_t)sg[i]);
         if (ml) st_relaxed(&arr[2 * call.cap + i], __float_as_uint(llr[i]));
     }
-    // chunk ranges of the classes (lane i < nclass holds class i)
-    unsigned nch = 0, myc = 0;
-    int myR = 1;
-    for (int i = 0; i < nclass; ++i) {
-        const int wi = (int)((wpack >> (4 * i)) & 15u);
-        const bool pi = ((ppack >> i) & 1u) != 0;
-        const uint64_t u = class_units(code, tam, wi, pi);
-        const int Ri = class_chunk_r(call, u, pi);
-        if (lane == i) { myc = nch; myR = Ri; }
-        nch += (unsigned)((u + 64ull * (uint64_t)Ri - 1) / (64ull * (uint64_t)Ri));
-    }
+    unsigned myc;
+    int myR;
+    const unsigned nch = task_ranges(code, call, tam, nclass, wpack, ppack, myc, myR);
     const unsigned gen = (ctl_gen(ld_relaxed(&T->ctl)) + 1u) & 0xFFFu;
     unsigned uid = 0;
     if (lane == 0) uid = atomicAdd(&call.task_counters[2], 1u);
@@ -1069,6 +1096,24 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
         __hip_atomic_exchange(&T->ctl, make_ctl(gen, nch), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         atomicAdd(&call.task_counters[0], 1u);
     }
+    return slot;
+}
+
+// Owner side: works on its published task in `slot` (the list and tables in
+// this wave's LDS), waits for every chunk, frees the slot and returns the
+// result in best (~0 = no match): first-hit, the lowest matching pattern index
+// over the classes concatenated in order; ML (one class), the lowest (cost
+// key, rank).
+__device__ void work_shared(const DevCode &code, const DevCall &call, int slot, const uint32_t LDS *s, uint32_t t0,
+                            int tam, bool ml, int nclass, unsigned wpack, unsigned ppack, const int LDS *act,
+                            const uint32_t LDS *sg, const float LDS *llr, const unsigned long long LDS *ht,
+                            unsigned hmask, const uint32_t LDS *pkey, const uint32_t LDS *pval,
+                            const uint32_t LDS *phead, const uint32_t LDS *pbm, unsigned long long &best) {
+    const int lane = lane_id();
+    TaskSlot *T = &call.tasks[slot];
+    unsigned myc;
+    int myR;
+    const unsigned nch = task_ranges(code, call, tam, nclass, wpack, ppack, myc, myR);
     const unsigned cchv = (lane < nclass) ? myc : (lane == nclass ? nch : 0u);
     for (;;) {
         unsigned g, u;
@@ -1077,14 +1122,53 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
         if (ch == ~0u) break;
         const unsigned long long res = task_chunk(code, T, s, t0, tam, ml, ch, bq, nclass, wpack, ppack, cchv, myR,
                                                   act, sg, llr, ht, hmask, pkey, pval, phead, pbm);
-        trace_chunk(call, 0, slot, uid, ch, (int)(wpack & 15u), tam, nclass, nch, t0, s, act, sg, res);
+        trace_chunk(call, 0, slot, u, ch, (int)(wpack & 15u), tam, nclass, nch, t0, s, act, sg, res);
         wait_mem();                               // the chunk's result atomics complete before it counts as done
         if (lane == 0) atomicAdd(&T->done, 1u);
     }
-    while (uni(ld_relaxed(&T->done)) < nch) __builtin_amdgcn_s_sleep(1);
+    wait_task_done(call, T, nch);
     best = uni64(ld_relaxed(&T->best));
     wait_mem();
     if (lane == 0) st_relaxed(&T->busy, 0u);
+}
+
+// Owner side: withdraws its published task in `slot` (its result is no longer
+// needed).  One exchange on ctl ends the claims; the chunks nobody claimed
+// count as done (and the task leaves the open count, which its last claim
+// would otherwise have done); the claimed ones see best_q = 0 and skip their
+// search.  Waits for the chunks in flight and frees the slot.
+__device__ void cancel_shared(const DevCall &call, int slot) {
+    TaskSlot *T = &call.tasks[slot];
+    unsigned nch = 0;
+    if (lane_id() == 0) {
+        atomicMin(&T->best_q, 0ull);
+        const unsigned long long c = ld_relaxed(&T->ctl);
+        nch = ctl_nch(c);
+        const unsigned long long prev = __hip_atomic_exchange(&T->ctl, (c & ~0xFFFFFFFFull) | (unsigned long long)nch,
+                                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned nxt = ctl_next(prev);
+        if (nxt < nch) {
+            atomicAdd(&T->done, nch - nxt);
+            atomicSub(&call.task_counters[0], 1u);
+        }
+    }
+    nch = uni(nch);
+    wait_task_done(call, T, nch);
+    wait_mem();
+    if (lane_id() == 0) st_relaxed(&T->busy, 0u);
+}
+
+// Owner side: publish a task and work on it; false when no slot is free (the
+// caller searches alone).
+__device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, const uint32_t LDS *s, uint32_t t0,
+                           int tam, bool ml, int nclass, unsigned wpack, unsigned ppack, const int LDS *act,
+                           const uint32_t LDS *sg, const float LDS *llr, const unsigned long long LDS *ht, unsigned hmask,
+                           const uint32_t LDS *pkey, const uint32_t LDS *pval,
+                           const uint32_t LDS *phead, const uint32_t LDS *pbm, unsigned long long &best) {
+    const int slot = publish_shared(code, call, shot, s, t0, tam, ml, nclass, wpack, ppack, act, sg, llr);
+    if (slot < 0) return false;
+    work_shared(code, call, slot, s, t0, tam, ml, nclass, wpack, ppack, act, sg, llr, ht, hmask, pkey, pval, phead,
+                pbm, best);
     return true;
 }
 
@@ -1230,12 +1314,46 @@ __device__ __attribute__((noinline)) uint64_t rest_work(const DevCode &code, con
 }
 
 // ------------------------------------------------------------ first-hit search
+// The run of classes that first_hit_wave shares for a weight list over tam
+// positions, if it shares one (and reaches it without a hit): first_hit_wave's
+// walk over the class sizes, without the searches.  The list comes packed,
+// 4 bits per weight (first in the low bits, 0 ends it), without classes of
+// weight above tam (empty: the walk passes over them).  nc classes, their
+// weights in wpack, bit i of ppack set when class i is walked by stems.
+__device__ __forceinline__ bool find_split(const DevCode &code, const DevCall &call, int tam, uint64_t wl4,
+                                           int &nc, unsigned &wpack, unsigned &ppack) {
+    while (wl4) {
+        const int w = (int)(wl4 & 15u);
+        const uint64_t N = ncr_w(code, tam, w);
+        if (N <= kFlatMax) {                      // a run of small classes goes pattern by pattern
+            wl4 >>= 4;
+            continue;
+        }
+        if (w >= 2 && call.split_threshold && rest_work(code, call, wl4, tam) > call.split_threshold) {
+            nc = 0; wpack = 0u; ppack = 0u;
+            for (uint64_t r = wl4; r && nc < kTaskClasses && (r & 15u) >= 2u; r >>= 4) {
+                const int wr = (int)(r & 15u);
+                const bool pk = use_pairs(code, call, wr, tam);
+                wpack |= (unsigned)wr << (4 * nc);
+                ppack |= (pk ? 1u : 0u) << nc;
+                ++nc;
+            }
+            return true;
+        }
+        wl4 >>= 4;                                // searched alone
+    }
+    return false;
+}
+
 // Weight list wl[0..nw) searched in order over A[0..tam) (grand_decoder.cu:295-350,
 // :534-588); on a hit writes the decoded columns to win and returns true; count
 // gets the patterns tested up to and including the hit.
-__device__ bool first_hit_wave(const DevCode &code, const DevCall &call, int shot, const uint32_t LDS *s, uint32_t t0,
-                               int tam, const int *wl, int nw, const WaveLds &L, int *win, int &fw,
-                               unsigned long long &count, unsigned long long &wops) {
+// spec: slot of this list's shared run when it was published before the
+// search (-1: none).  Returns bit 0: hit; bit 1: spec was used or withdrawn.
+__device__ int first_hit_wave(const DevCode &code, const DevCall &call, int shot, const uint32_t LDS *s, uint32_t t0,
+                              int tam, const int *wl, int nw, const WaveLds &L, int *win, int &fw,
+                              unsigned long long &count, unsigned long long &wops, int spec) {
+    int spec_done = 0;
     const int lane = lane_id();
     unsigned hmask = 0;          // hash table of A, built before the first class that needs it
     bool have_table = false;
@@ -1318,8 +1436,29 @@ __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, int sho
                 if (need_pairs && !have_pairs) { build_pairs(L, L.sigA, tam); have_pairs = true; }
                 if (need_ht && !have_table) { hmask = build_ht(L.ht, L.sigA, tam); have_table = true; }
                 unsigned long long best;
-                if (run_shared(code, call, shot, s, t0, tam, false, nc, wpack, ppack, L.A, L.sigA, nullptr, L.ht, hmask,
-                               L.pkey, L.pval, L.phead, L.pbm, best)) {
+                bool shared = false;
+                if (spec >= 0) {
+                    // the run was published before this iteration's search; it is
+                    // used when it is this run (it always is: the same list and walk)
+                    TaskSlot *T = &call.tasks[spec];
+                    const bool same = uni((unsigned)ld_relaxed(&T->nclass)) == (unsigned)nc &&
+                                      uni((unsigned)ld_relaxed(&T->wpack)) == wpack &&
+                                      uni((unsigned)ld_relaxed(&T->ppack)) == ppack &&
+                                      uni((unsigned)ld_relaxed(&T->tam)) == (unsigned)tam;
+                    if (same) {
+                        work_shared(code, call, spec, s, t0, tam, false, nc, wpack, ppack, L.A, L.sigA, nullptr, L.ht,
+                                    hmask, L.pkey, L.pval, L.phead, L.pbm, best);
+                        shared = true;
+                    } else {
+                        cancel_shared(call, spec);
+                    }
+                    spec = -1;
+                    spec_done = 2;
+                }
+                if (!shared)
+                    shared = run_shared(code, call, shot, s, t0, tam, false, nc, wpack, ppack, L.A, L.sigA, nullptr, L.ht,
+                                        hmask, L.pkey, L.pval, L.phead, L.pbm, best);
+                if (shared) {
                     uint64_t span = 0;
                     for (int k = kc; k < kend; ++k) span += N[k];
                     if (best != ~0ull) {
@@ -1371,11 +1510,11 @@ __device__ bool first_hit_wave(const DevCode &code, const DevCall &call, int sho
         }
         count += hit_idx + 1u;
         wops += weighted_prefix(N, wl, nw, hit_idx + 1u);
-        return true;
+        return 1 | spec_done;
     }
     count += total;
     wops += weighted_prefix(N, wl, nw, total);
-    return false;
+    return spec_done;
 }
 
 // -------------------------------------------------------------------- ML search
@@ -1459,7 +1598,7 @@ __device__ bool ml_wave(const DevCode &code, const DevCall &call, int shot, cons
 // --------------------------------------------------------- per-iteration logic
 // Weight list of one iteration: 1..W (uniform) or the phase lists of the mask
 // chosen for tam (grand_decoder.cu:754-786, :1001-1003).
-__device__ int weight_list(const DevCall &call, int tam, int *wl) {
+__device__ __forceinline__ int weight_list(const DevCall &call, int tam, int *wl) {
     int nw = 0;
     if (call.kind == kUniform) {
         for (int w = 1; w <= call.W; ++w) wl[nw++] = w;
@@ -1777,6 +1916,30 @@ __device__ bool learned_pass(const DevCode &code, const DevCall &call, int shot,
 // One claimed shot, staged in LDS by stage_claim: slot = [shot, signature,
 // syndrome words (D), then original weight, degeneracy word and error words
 // (M32) for cc with errors, or the observed logicals (NL32) for cln].
+// Publishes the run of classes that the third iteration (threshold
+// max_soma - 2) would share, before the second iteration is searched: a shot
+// whose second list is long (kSpecTam positions or more) is likely to need the
+// third, whose run then proceeds on the other waves while this wave searches
+// the second list.  When the third iteration is searched it works on this task
+// instead of publishing one; when it is not, the task is withdrawn (a result of
+// an earlier iteration is never changed by it).  Returns the slot or -1.
+// Overwrites the wave's list (A, sigA).
+// Out of line: the wave's LDS layout is carved again from the call (the
+// search kernel's own layout).
+__device__ uint64_t lane_weight_list(const DevCall &call, int tam);
+constexpr int kSpecTam = 16;
+__device__ __attribute__((noinline)) int publish_iteration3(const DevCode &code, const DevCall &call, int shot,
+                                                             const uint32_t LDS *s, uint32_t t0) {
+    const WaveLds L = carve_lds(call.cap, call.kind == kML || call.lscores != nullptr, code.lds_words,
+                                call.claim_batch * call.stage_words, call.pair_cap);
+    const int tam3 = build_active_into(code, s, code.max_soma - 2, L.A, L.sigA, call.cap);
+    if (tam3 <= 0 || tam3 > call.cap || branch_of(call, tam3) == 0) return -1;
+    int nc;
+    unsigned wpack, ppack;
+    if (!find_split(code, call, tam3, lane_weight_list(call, tam3), nc, wpack, ppack)) return -1;
+    return publish_shared(code, call, shot, s, t0, tam3, false, nc, wpack, ppack, L.A, L.sigA, nullptr);
+}
+
 __device__ __forceinline__ void decode_shot_wave(const DevCode &code, const DevCall &call, const uint32_t LDS *slot,
                                                  const WaveLds &L) {
     ShotOutcome o;
@@ -1791,6 +1954,9 @@ __device__ __forceinline__ void decode_shot_wave(const DevCode &code, const DevC
 #pragma unroll
     for (int l = 0; l < kMaxWin; ++l) o.win[l] = -1;
     int it = 0, tam = 0, tam_old = 0;
+    int spec = -1;                                 // the third iteration's run, published early (slot)
+    const bool spec_ok = call.kind != kML && call.family == kCC && call.lscores == nullptr && call.split_threshold &&
+                         code.max_soma >= 3;
     const bool learned_hit = call.lscores && call.kind != kML && learned_pass(code, call, shot, s, t0, L, o);
     while (!learned_hit && it < code.max_soma && it < 3) {
         const int thr = code.max_soma - it;
@@ -1804,6 +1970,10 @@ __device__ __forceinline__ void decode_shot_wave(const DevCode &code, const DevC
         }
         const int br = branch_of(call, tam);
         if (br == 0) continue;
+        if (spec_ok && it == 2 && tam >= kSpecTam) {
+            spec = publish_iteration3(code, call, shot, s, t0);
+            if (spec >= 0) tam = build_active(code, s, thr, L, call.cap);   // the second list again
+        }
         o.hibrido = (br > 0) ? (o.hibrido == -1 ? 1 : 0) : -1;
         bool hit;
         if (call.kind == kML) {
@@ -1811,10 +1981,14 @@ __device__ __forceinline__ void decode_shot_wave(const DevCode &code, const DevC
         } else {
             int wl[2 * kMaxW];
             const int nw = weight_list(call, tam, wl);
-            hit = first_hit_wave(code, call, shot, s, t0, tam, wl, nw, L, o.win, o.fw, o.count, o.wops);
+            const int r = first_hit_wave(code, call, shot, s, t0, tam, wl, nw, L, o.win, o.fw, o.count, o.wops,
+                                         it == 3 ? spec : -1);
+            hit = (r & 1) != 0;
+            if (r & 2) spec = -1;
         }
         if (hit) { o.found = 1; break; }
     }
+    if (spec >= 0) cancel_shared(call, spec);       // the third iteration was not searched
     if (!learned_hit) {
         o.iteration = it;
         o.tam = tam;

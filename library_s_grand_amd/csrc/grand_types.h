@@ -23,7 +23,7 @@ enum Family : int { kCC = 0, kCLN = 1 };
 enum Kind : int { kUniform = 0, kPhase = 1, kML = 2 };
 
 // Error bits raised by kernels and turned into a -1 return by the host.
-enum ErrBits : unsigned { kErrNonBinary = 1u, kErrTamOverflow = 2u };
+enum ErrBits : unsigned { kErrNonBinary = 1u, kErrTamOverflow = 2u, kErrTaskWait = 4u };
 
 // Immutable per-code device data.  Column bits are kept twice: word-major
 // (Hcw[d * m + c]) for coalesced score computation, column-major

@@ -223,21 +223,26 @@ def test_pair_table_deep_classes(oracle, product, tmp_path, wpc):
 
 
 def test_deep_shots_in_small_batches(oracle, product, tmp_path):
-    """The deepest shots of the BB144 prob_cc bench workload (1.2e6 to 2.7e7 patterns each, three
-    iterations) decoded alone and as a batch of three: a batch smaller than the CU count still
-    launches helper waves for its shared tasks; results bit-exact with the oracle."""
+    """Shots of the BB144 prob_cc bench workload whose second active list has 16 or more columns (the
+    third iteration's shared run is then published before the second is searched: joined when the
+    third iteration is searched, withdrawn when an earlier one finds the pattern), plus the deep shot
+    37877 (1.2e6 patterns): as one small batch, the deepest (2.7e7 patterns) alone, and 37877 alone.
+    A batch smaller than the CU count still launches helper waves.  Bit-exact with the oracle."""
     hx, hz = codes.bb(144)
     seed = 20240501
     p = codes.log_uniform(np.random.default_rng(seed), 1e-3, 1e-2, hx.shape[1])
     errors = codes.sample_errors(np.random.default_rng(seed + 1), 100_000, p)
+    syn = (errors.astype(np.int64) @ hz.T) % 2
+    tam2 = ((syn @ hz) >= 2).sum(axis=1)          # second list: columns with two checks in supp(s)
+    sel = sorted(set(np.nonzero(tam2 >= 16)[0].tolist()) | {37877})
+    assert 79335 in sel and len(sel) >= 8
     pf = T.phase_file(tmp_path)
     cfg = G.make_config(max_error_weight=4)
-    deep = [79335, 37877]
-    want = T.run_cc(oracle, G.KIND_PROB_CC, hx, hz, errors=np.ascontiguousarray(errors[deep]), cfg=cfg,
-                    prob_file=pf)
-    assert int(want[1]["total_tested_combinations"].max()) > 10_000_000
-    for sel in ([0], [1], [0, 1]):
-        got = T.run_cc(product, G.KIND_PROB_CC, hx, hz, errors=np.ascontiguousarray(errors[[deep[i] for i in sel]]),
+    want = T.run_cc(oracle, G.KIND_PROB_CC, hx, hz, errors=np.ascontiguousarray(errors[sel]), cfg=cfg, prob_file=pf)
+    it = want[1]["iteration"]
+    assert (it == 3).sum() >= 3 and (it == 2).sum() >= 3, it
+    for sub in (list(range(len(sel))), [sel.index(79335)], [sel.index(37877)]):
+        got = T.run_cc(product, G.KIND_PROB_CC, hx, hz, errors=np.ascontiguousarray(errors[[sel[i] for i in sub]]),
                        cfg=cfg, prob_file=pf)
-        sub = (want[0], want[1][sel], None if want[2] is None else want[2][sel], None)
-        T.assert_same(sub, got, f"deep shots {sel}")
+        ref = (want[0], want[1][sub], None if want[2] is None else want[2][sub], None)
+        T.assert_same(ref, got, f"shots {[sel[i] for i in sub]}")
