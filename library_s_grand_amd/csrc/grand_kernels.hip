@@ -2591,15 +2591,16 @@ __device__ __forceinline__ void triage_groups(const DevCode &code, const DevCall
                 call.stats[3 * (size_t)shot + 2] = 0;
             }
         }
-        // search kernel: shots whose syndrome needs more than two columns go to
-        // the front of the worklist (their deep classes then overlap with the
-        // rest of the batch), the others to the back; the plain engine reads one list
+        // search kernel: shots whose syndrome weight is at least that of two
+        // disjoint columns go to the front of the worklist (their deep classes then
+        // overlap with the rest of the batch), the others to the back; the plain
+        // engine reads one list
         int sw = 0;
 #pragma unroll
         for (int d = 0; d < kMaxWords; ++d) sw += __popc(s[d]);
         // The group's partition (g % wl_parts) spreads the counter atomics over
         // wl_parts cache lines.
-        const bool front = queued && (!call.split_worklist || sw > 2 * code.max_soma);
+        const bool front = queued && (!call.split_worklist || sw >= 2 * code.max_soma);
         const uint64_t mf = ballot(front), mb = ballot(queued && !front);
         const int part = g % call.wl_parts;
         unsigned *pc = call.counters + kWlBase + kWlStride * part;
@@ -2756,9 +2757,11 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
         }
         open = uni(open);
         got = uni(got);
-        // shared classes of deep shots first: they bound the batch's completion time
+        // shared classes of deep shots first: they bound the batch's completion
+        // time; a claimed front shot (likely deep itself) is decoded before helping
         bool helped = false;
-        if (open != 0u)
+        const bool front_claim = !drained && got < total && (unsigned)cur + kClaimWords * got < heavy;
+        if (open != 0u && !front_claim)
             while (help_task(code, call, L, hc)) helped = true;
         if (!drained) {
             ++claims;
