@@ -2757,11 +2757,12 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
         }
         open = uni(open);
         got = uni(got);
-        // shared classes of deep shots first: they bound the batch's completion
-        // time; a claimed front shot (likely deep itself) is decoded before helping
+        // the claimed shots are decoded first; a wave serves the open classes of
+        // deep shots only while it holds no claim (claims held during helping
+        // left light shots to the end of the batch)
         bool helped = false;
-        const bool front_claim = !drained && got < total && (unsigned)cur + kClaimWords * got < heavy;
-        if (open != 0u && !front_claim)
+        const bool have_claim = !drained && got < total && (unsigned)cur + kClaimWords * got < total;
+        if (open != 0u && !have_claim)
             while (help_task(code, call, L, hc)) helped = true;
         if (!drained) {
             ++claims;
