@@ -257,7 +257,6 @@ def main():
 
     from library_s_grand_amd import autotune
     tuning = None
-    tuning_defaults = autotune.get(lib)
     if not a.no_tune:
 
         def tune_run():
@@ -383,13 +382,13 @@ def main():
     sess.close()
     if tuning is not None:
         # the tuned knobs belong to this workload: the other legs run with the built-in defaults
-        autotune.apply(lib, tuning_defaults)
+        autotune.reset(lib)
     if not a.no_legs:
         oracle_path = None if a.no_cpu else os.path.join(ROOT, "oracle", "libgrand_oracle.so")
         out["bb72_cc"] = leg_bb72_cc(lib, oracle_path, tune=not a.no_tune)
-        autotune.apply(lib, tuning_defaults)
+        autotune.reset(lib)
         out["dem_ml_cln"] = leg_dem_ml(lib, oracle_path, tune=not a.no_tune)
-        autotune.apply(lib, tuning_defaults)
+        autotune.reset(lib)
     if not a.no_serving:
         from library_s_grand_amd import serving
         out["serving"] = serving.measure(rate_hz=2000.0, n_requests=4000)

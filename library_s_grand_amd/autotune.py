@@ -42,6 +42,12 @@ def get(lib):
     return {k: x.value for k, x in zip(KEYS, v)}
 
 
+def reset(lib):
+    """Restores the library's built-in defaults (one set for the first-hit kinds, one for ML)."""
+    bind_session(lib)
+    lib.grand_set_tuning(-1, -1, -1, -1)
+
+
 def apply(lib, cfg):
     bind_session(lib)
     lib.grand_set_tuning(*[int(cfg[k]) for k in KEYS])

@@ -559,9 +559,19 @@ int kind_search(int kind) {
     }
 }
 
-// Autotuned defaults (see tune_* in the Python package); overridable per call.
-struct Tuning { int waves_per_cu = 4; int claim_batch = 2; int rmax = 32; int split_threshold = 4096; };
-Tuning g_tuning;
+// Tuning knobs: built-in defaults per search kind, replaced for every kind by
+// grand_set_tuning (the autotuner in the Python package), overridable per
+// session.  First-hit kinds: 2 waves per CU, so a wave's share of the CU's LDS
+// holds the pair table of lists up to 81 columns (BB144 prob_cc, 100k shots,
+// untuned: 292 M shots/s with the former single set 4, 2, 32, 4096, 577 M with
+// this one).  ML: 8 waves per CU (the DEM ML workload, tuned: 10.1 M shots/s,
+// against 5.2 M with the former set and 2.8 M with the first-hit set).
+struct Tuning { int waves_per_cu; int claim_batch; int rmax; int split_threshold; };
+constexpr Tuning kTuningFirstHit{2, 2, 64, 32768};
+constexpr Tuning kTuningMl{8, 2, 64, 32768};
+Tuning g_tuning = kTuningFirstHit;
+Tuning g_tuning_ml = kTuningMl;
+const Tuning &tuning_for(int search) { return search == kML ? g_tuning_ml : g_tuning; }
 
 // Validates exactly as the reference driver for `kind` would, then applies the
 // library's own limits, and creates a session (code upload cached by content).
@@ -826,11 +836,12 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
         const int st_ex = s->ex.split_threshold;
         c.split_threshold = st_ex < 0 ? 0ull
                           : st_ex > 0 ? (unsigned long long)st_ex
-                          : (unsigned long long)g_tuning.split_threshold;
+                          : (unsigned long long)tuning_for(s->search).split_threshold;
     }
     // tuning knobs are read at every decode, so the autotuner can vary them on a live session
-    c.rmax = g_tuning.rmax;
-    c.claim_batch = std::max(1, std::min((s->ex.claim_batch > 0) ? s->ex.claim_batch : g_tuning.claim_batch, kMaxBatch));
+    c.rmax = tuning_for(s->search).rmax;
+    c.claim_batch = std::max(1, std::min((s->ex.claim_batch > 0) ? s->ex.claim_batch : tuning_for(s->search).claim_batch,
+                                         kMaxBatch));
     // shots whose first iteration is small are searched one per lane by the
     // triage kernel (fused engine, first-hit kinds, small codes); with errors
     // as input the degeneracy words must exist
@@ -850,7 +861,7 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     c.stage_words = 2 + code.D + (c.input_errors ? 2 + code.M32 : 0) + (s->family == kCLN ? code.NL32 : 0);
     // search-kernel pair table (first-hit kinds): what is left of the wave's share
     // of the CU's LDS (at most 64 KB per workgroup) after everything else
-    const int wpc = (s->ex.waves_per_cu > 0) ? s->ex.waves_per_cu : g_tuning.waves_per_cu;
+    const int wpc = (s->ex.waves_per_cu > 0) ? s->ex.waves_per_cu : tuning_for(s->search).waves_per_cu;
     c.pair_cap = 0;
     if (s->ex.engine != GRAND_ENGINE_PLAIN && s->search != kML) {
         const size_t base = grand_fused_lds_bytes(c.cap, c.lscores != nullptr) +
@@ -1281,12 +1292,20 @@ int grand_device_info(int *cus, int *clock_khz, int *wall_clock_khz) {
     return 0;
 }
 
-// Tuning knobs (written by the autotuner).
+// Tuning knobs (written by the autotuner): set for every search kind; all four
+// negative restores the built-in defaults.
 void grand_set_tuning(int waves_per_cu, int claim_batch, int rmax, int split_threshold) {
-    if (waves_per_cu > 0) g_tuning.waves_per_cu = waves_per_cu;
-    if (claim_batch > 0) g_tuning.claim_batch = std::min(claim_batch, kMaxBatch);
-    if (rmax > 0) g_tuning.rmax = rmax;
-    if (split_threshold > 0) g_tuning.split_threshold = split_threshold;
+    if (waves_per_cu < 0 && claim_batch < 0 && rmax < 0 && split_threshold < 0) {
+        g_tuning = kTuningFirstHit;
+        g_tuning_ml = kTuningMl;
+        return;
+    }
+    for (Tuning *t : {&g_tuning, &g_tuning_ml}) {
+        if (waves_per_cu > 0) t->waves_per_cu = waves_per_cu;
+        if (claim_batch > 0) t->claim_batch = std::min(claim_batch, kMaxBatch);
+        if (rmax > 0) t->rmax = rmax;
+        if (split_threshold > 0) t->split_threshold = split_threshold;
+    }
 }
 
 void grand_get_tuning(int *waves_per_cu, int *claim_batch, int *rmax, int *split_threshold) {

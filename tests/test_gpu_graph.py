@@ -99,5 +99,5 @@ def test_autotuner_applies_best_and_keeps_results(tmp_path):
         sess.decode(d_err, True, d_res)
         _same(results_from_bytes(d_res.cpu().numpy()), first, "tuned")
     finally:
-        autotune.apply(lib, before)
+        autotune.reset(lib)                       # the built-in defaults of every search kind
         sess.close()
