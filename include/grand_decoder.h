@@ -9,7 +9,7 @@
  *
  * Unlike the reference header this one is valid C as well as C++.
  *
- * Layout contract (checked by tests/test_abi_layout.py):
+ * Layout contract (checked by tests/test_abi_cpu.py):
  *   GrandDecoderConfig  24 bytes, 6 x int32.
  *   GrandDecoderResult  72 bytes; total_tested_combinations at offset 24,
  *                       hibrido at 32, iteration at 36, time_sum_ms at 40.
@@ -52,10 +52,17 @@ typedef struct {
 
 /* Reference grand_decoder.h:53-68.  One record per shot.
  * total_tested_combinations counts patterns as the reference CPU search does:
- * up to and including the hit.  The four timing doubles are filled from device
- * timestamps: time_sum_ms = 0 (the column scores are fused into the search),
- * total_time_search_ms = device time of the shot's search, transfer_time_ms =
- * 0 (no per-shot transfers), total_to_find_ms = search time when found. */
+ * up to and including the hit.  The four timing doubles (not parity fields):
+ *   time_sum_ms          device time from the start of the shot's search to
+ *                        its first active list (column scores included);
+ *   total_time_search_ms device time of the shot's search;
+ *   transfer_time_ms     host<->device copies of the whole call (inputs in,
+ *                        results and decoded patterns out, from HIP events),
+ *                        an equal share per shot: the library copies all
+ *                        shots at once, not one shot at a time;
+ *   total_to_find_ms     the search time when found, else 0.
+ * Shots with an empty syndrome have 0 in every timing field except
+ * transfer_time_ms.  The _ex entry points fill the same fields. */
 typedef struct {
     int found;
     int found_weight;

@@ -346,37 +346,3 @@ def simulate_random_error(n, p_error):
 def error_positions_random(n, error_weight):
     """error_weight distinct positions in [0, n) (auxilary_functions.py:168-170)."""
     return np.random.choice(n, size=error_weight, replace=False)
-
-
-def build_quantum_circuit_H(H, error_positions, measure):
-    """The reference builds a Qiskit syndrome-extraction circuit (auxilary_functions.py:134-155):
-    X on the error qubits, a CNOT from data qubit c to ancilla r for every H[r][c] = 1, and
-    optional measurement of the ancillas.  Needs qiskit, which this package does not require."""
-    try:
-        from qiskit import QuantumCircuit
-    except ImportError as exc:
-        raise ImportError("build_quantum_circuit_H needs qiskit, which is not installed") from exc
-    m, n = np.asarray(H).shape
-    qc = QuantumCircuit(n + m, m)
-    if error_positions is not None:
-        for q in error_positions:
-            qc.x(int(q))
-    for row in range(m):
-        for col in range(n):
-            if H[row, col] == 1:
-                qc.cx(col, n + row)
-    if measure:
-        for row in range(m):
-            qc.measure(n + row, row)
-    return qc
-
-
-def extract_syndrome(qc, n_shots):
-    """Runs the circuit on Qiskit Aer and returns the first measured syndrome
-    (auxilary_functions.py:157-166); needs qiskit-aer."""
-    try:
-        from qiskit_aer import AerSimulator
-    except ImportError as exc:
-        raise ImportError("extract_syndrome needs qiskit-aer, which is not installed") from exc
-    counts = AerSimulator().run(qc, shots=n_shots).result().get_counts()
-    return [int(b) for b in next(iter(counts))[::-1]]

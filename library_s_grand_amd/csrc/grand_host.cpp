@@ -14,7 +14,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -282,6 +281,18 @@ struct CodeEntry {
     int n = 0, m = 0, NL = 0;
     int search_lds_words = 0;         // dcode_search's lds_words
     size_t bytes = 0;
+    // the uploaded content, compared on every cache hit (the key is a 64-bit
+    // hash): the searched matrix, the other CSS matrix (cc) or L (cln), LLRs
+    std::vector<int> src_h, src_other;
+    std::vector<float> src_llr;
+    bool same(int family_, int n_, int m_, int NL_, const int *H, const int *other, size_t other_len,
+              const std::vector<float> &llr) const {
+        if (family_ != family || n_ != n || m_ != m || NL_ != NL) return false;
+        if (src_other.size() != other_len || src_llr.size() != llr.size()) return false;
+        if (memcmp(src_h.data(), H, src_h.size() * sizeof(int)) != 0) return false;
+        if (other_len && memcmp(src_other.data(), other, other_len * sizeof(int)) != 0) return false;
+        return llr.empty() || memcmp(src_llr.data(), llr.data(), llr.size() * sizeof(float)) == 0;
+    }
 };
 
 // Packs H (n x m), the RREF of the other CSS matrix (cc) or L (cln), and LLRs.
@@ -461,9 +472,10 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
 // Small LRU of uploaded codes, keyed by content hash.
 std::list<std::shared_ptr<CodeEntry>> g_codes;
 
-std::shared_ptr<CodeEntry> find_code(uint64_t key) {
+std::shared_ptr<CodeEntry> find_code(uint64_t key, int family, int n, int m, int NL, const int *H, const int *other,
+                                     size_t other_len, const std::vector<float> &llr) {
     for (auto it = g_codes.begin(); it != g_codes.end(); ++it)
-        if ((*it)->key == key) {
+        if ((*it)->key == key && (*it)->same(family, n, m, NL, H, other, other_len, llr)) {
             auto e = *it;
             g_codes.erase(it);
             g_codes.push_front(e);
@@ -516,7 +528,6 @@ struct GrandSession {
     grand::Workspace ws;
     grand::DevMem plain_scratch, plain_state, plain_records, plain_out;
     grand::DevMem dcall;              // device copy of `call`, refreshed before every batch
-    grand::DevMem dbg;                // chunk trace buffer (GRAND_DEBUG_TASKS)
     // HIP-event timing: a ring of kTimingSlots event triples (before triage,
     // after triage, after search), one per decode.  A slot's times are read
     // (waiting for its events) when the slot is reused or the times are asked
@@ -638,10 +649,15 @@ GrandSession *create_session(int kind, const int *H, const int *H2, int n, int m
     if (!llr.empty()) { h.mix(0x4c4c52ull); h.ints(reinterpret_cast<const int *>(llr.data()), llr.size()); }
     if (!bin) { fprintf(stderr, "grand: check/logical matrices must hold only 0 and 1\n"); return nullptr; }
 
-    std::shared_ptr<CodeEntry> ce = find_code(h.h);
+    const int *second = (family == kCC) ? other : (NL > 0 ? H2 : nullptr);
+    const size_t second_len = (family == kCC) ? (size_t)n * m : (NL > 0 ? (size_t)m * NL : 0);
+    std::shared_ptr<CodeEntry> ce = find_code(h.h, family, n, m, NL, Hs, second, second_len, llr);
     if (!ce) {
         ce = upload_code(h.h, family, Hs, other, family == kCLN ? H2 : nullptr, n, m, NL, llr.empty() ? nullptr : &llr);
         if (!ce) return nullptr;
+        ce->src_h.assign(Hs, Hs + (size_t)n * m);
+        if (second) ce->src_other.assign(second, second + second_len);
+        ce->src_llr = llr;
         remember_code(ce);
     }
     auto *s = new GrandSession();
@@ -668,7 +684,7 @@ GrandSession *create_session(int kind, const int *H, const int *H2, int n, int m
 // Host loop over nonzero shots, one launch per (shot, iteration, weight class),
 // a synchronous read-back after each: the reference driver's structure
 // (grand_decoder.cu:1284-1343) with the deterministic CPU-order search.
-int run_plain(GrandSession *s, hipStream_t st, int num_shots) {
+int run_plain(GrandSession *s, hipStream_t st) {
     const DevCode &code = s->code->dev;
     DevCall &call = s->call;
     const DevCode *dcode = s->code->dcode.as<DevCode>();
@@ -793,7 +809,6 @@ int run_plain(GrandSession *s, hipStream_t st, int num_shots) {
         o.tam = tam;
         HIPX(grand_launch_plain_finalize(dcode, dcall, shot, &o, s->plain_scratch.p, hit_rank, st));
     }
-    (void)num_shots;
     return 0;
 }
 
@@ -820,15 +835,6 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     c.dec = d_dec ? d_dec : s->ws.dec.as<uint32_t>();
     c.declog = d_declog ? d_declog : s->ws.declog.as<uint32_t>();
     c.stats = d_stats;
-    c.dbg = nullptr;
-    c.dbg_cap = 0;
-    if (const char *dbg = getenv("GRAND_DEBUG_TASKS")) {
-        if (dbg[0] && s->dbg.ensure(4u << 20)) {
-            c.dbg = s->dbg.as<unsigned>();
-            c.dbg_cap = (4u << 20) / 4;
-            HIPX(grand_launch_fill(c.dbg, 0u, 1, st));
-        }
-    }
     c.tasks = s->ws.tasks.as<TaskSlot>();
     c.task_arrays = s->ws.task_arrays.as<uint32_t>();
     c.task_counters = s->ws.counters.as<unsigned>() + 4;
@@ -904,7 +910,7 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     HIPX(grand_launch_triage(dcode, dcall, tgrid, block_words, c.triage_stage_words, st));
     if (tev) HIPX(hipEventRecord(tev[1], st));
     if (s->ex.engine == GRAND_ENGINE_PLAIN) {
-        int rc = run_plain(s, st, num_shots);
+        int rc = run_plain(s, st);
         if (rc == -2) { (void)grand_launch_fill(c.counters + 2, kErrTamOverflow, 1, st); rc = 0; }
         if (tev) { HIPX(hipEventRecord(tev[2], st)); s->tpending |= 1ull << s->tslot; }
         return rc;
@@ -920,7 +926,14 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
 
 // ----------------------------------------------------------------- ABI path
 struct AbiBuffers {
-    DevMem input, logicals, results, times, dec_int, declog_int, counters_zero;
+    DevMem input, logicals, results, dec_int, declog_int;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // H2D start/end, D2H start/end
+    bool events() {
+        if (ev[0]) return true;
+        for (auto &e : ev)
+            if (hipEventCreate(&e) != hipSuccess) { e = nullptr; return false; }
+        return true;
+    }
 };
 AbiBuffers g_abi;
 
@@ -974,15 +987,15 @@ int run_abi(int kind, const int *H, const int *H2, int n, int m, int NL, char ty
     if (!B.input.ensure(S * cols * 4) || !B.results.ensure(S * sizeof(GrandDecoderResult)) ||
         (family == kCLN && !B.logicals.ensure(S * std::max(NL, 1) * 4)))
         return -1;
+    const bool timed = B.events();
+    if (timed) HIPX(hipEventRecord(B.ev[0], st));
     HIPX(hipMemcpyAsync(B.input.p, input, S * cols * 4, hipMemcpyHostToDevice, st));
     if (family == kCLN && NL > 0) HIPX(hipMemcpyAsync(B.logicals.p, logicals, S * NL * 4, hipMemcpyHostToDevice, st));
-    const bool plain = s->ex.engine == GRAND_ENGINE_PLAIN;
-    std::vector<double> host_ms;
-    auto t0 = std::chrono::steady_clock::now();
+    if (timed) HIPX(hipEventRecord(B.ev[1], st));
     if (decode_device(s, B.input.as<int>(), input_errors, B.logicals.as<int>(), num_shots,
                       B.results.as<GrandDecoderResult>(), nullptr, nullptr, nullptr, st) != 0)
         return -1;
-    (void)t0;
+    if (timed) HIPX(hipEventRecord(B.ev[2], st));
     HIPX(hipMemcpyAsync(results, B.results.p, S * sizeof(GrandDecoderResult), hipMemcpyDeviceToHost, st));
     if (decoded) {
         if (!B.dec_int.ensure(S * m * 4)) return -1;
@@ -996,22 +1009,15 @@ int run_abi(int kind, const int *H, const int *H2, int n, int m, int NL, char ty
     }
     unsigned bits = 0;
     HIPX(hipMemcpyAsync(&bits, s->call.counters + 2, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    if (B.ev[0]) HIPX(hipEventRecord(B.ev[3], st));
     HIPX(hipStreamSynchronize(st));
-    (void)plain;
-    if (const char *dbg = getenv("GRAND_DEBUG_TASKS")) {
-        if (dbg[0] && s->call.dbg) {
-            std::vector<unsigned> h(s->call.dbg_cap);
-            if (hipMemcpy(h.data(), s->call.dbg, h.size() * 4, hipMemcpyDeviceToHost) == hipSuccess) {
-                FILE *f = fopen(dbg, "w");
-                if (f) {
-                    const unsigned nrec = std::min<unsigned>(h[0], s->call.dbg_cap / 16 - 2);
-                    for (unsigned i = 0; i < nrec; ++i) {
-                        for (int k = 0; k < 16; ++k) fprintf(f, "%u ", h[16 * (i + 1) + k]);
-                        fprintf(f, "\n");
-                    }
-                    fclose(f);
-                }
-            }
+    if (B.ev[0]) {
+        // transfer_time_ms: the call's host<->device copies, an equal share per shot
+        float h2d = 0.0f, d2h = 0.0f;
+        if (hipEventElapsedTime(&h2d, B.ev[0], B.ev[1]) == hipSuccess &&
+            hipEventElapsedTime(&d2h, B.ev[2], B.ev[3]) == hipSuccess) {
+            const double share = ((double)h2d + (double)d2h) / (double)num_shots;
+            for (int i = 0; i < num_shots; ++i) results[i].transfer_time_ms = share;
         }
     }
     if (report_error_bits(bits)) return -1;

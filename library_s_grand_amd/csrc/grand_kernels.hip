@@ -850,19 +850,33 @@ __device__ PrefHit weight_one(const DevCode &code, const uint32_t LDS *s, uint32
 }
 
 // ---------------------------------------------------------------- shared tasks
-// ctl word: generation (12 bits) | nchunks (20 bits) | next chunk (32 bits).
+// ctl word: generation (20 bits) | nchunks (20 bits) | next chunk (24 bits).
 // Chunks are claimed with one fetch-and-add on ctl: a returned `next` below
 // `nchunks` is a claim of that chunk of generation `gen`; anything else claims
-// nothing (the counter may run past nchunks by at most one step per wave, far
-// from carrying into the nchunks field).
+// nothing.  A wave adds only after it read next < nchunks, so the counter runs
+// past nchunks by at most one step per resident wave (< 2^12), far from
+// carrying into the nchunks field.  The generation counts the publications of
+// the slot (it wraps after 2^20 of them).  (slot, generation), as returned by
+// the claiming fetch-and-add itself, identifies the task a chunk belongs to.
 constexpr unsigned kMaxChunks = (1u << 20) - 1;
 constexpr int kStemCost = 4;    // prefix steps one stem step is counted as (split decision, chunk size)
 constexpr int kMinStemR = 4;
-__device__ __forceinline__ unsigned ctl_next(unsigned long long c) { return (unsigned)c; }
-__device__ __forceinline__ unsigned ctl_nch(unsigned long long c) { return (unsigned)((c >> 32) & 0xFFFFFu); }
-__device__ __forceinline__ unsigned ctl_gen(unsigned long long c) { return (unsigned)(c >> 52); }
+__device__ __forceinline__ unsigned ctl_next(unsigned long long c) { return (unsigned)(c & 0xFFFFFFull); }
+__device__ __forceinline__ unsigned ctl_nch(unsigned long long c) { return (unsigned)((c >> 24) & 0xFFFFFull); }
+__device__ __forceinline__ unsigned ctl_gen(unsigned long long c) { return (unsigned)(c >> 44); }
 __device__ __forceinline__ unsigned long long make_ctl(unsigned gen, unsigned nch) {
-    return ((unsigned long long)(gen & 0xFFFu) << 52) | ((unsigned long long)nch << 32);
+    return ((unsigned long long)(gen & 0xFFFFFu) << 44) | ((unsigned long long)nch << 24);
+}
+// best_q holds the task's generation in its top 20 bits and the unit index of
+// the lowest hit so far in the low 44.  Every unit index of a task is below
+// 2^44: its classes have weights 2..6 over at most 1023 positions, and the sum
+// over w = 2..6 of C(1023, w - 1) prefixes is 9.3e12 < 2^44.  A claim reads it
+// in the same round trip as the claiming fetch-and-add, so the value may
+// predate the task's publication; one with another generation is ignored.
+constexpr int kQBits = 44;
+constexpr unsigned long long kQMask = (1ull << kQBits) - 1ull;
+__device__ __forceinline__ unsigned long long make_bq(unsigned gen, unsigned long long q) {
+    return ((unsigned long long)(gen & 0xFFFFFu) << kQBits) | (q & kQMask);
 }
 
 template <class T>
@@ -874,39 +888,46 @@ __device__ __forceinline__ void st_relaxed(T *p, T v) { __hip_atomic_store(p, v,
 // a writer only waits for its own stores and atomics to complete before the
 // store or atomic that publishes them.  (An agent-scope acquire or release
 // fence would invalidate or write back the whole L2 of the XCD each time.)
-__device__ __forceinline__ void wait_mem() { __builtin_amdgcn_s_waitcnt(0); }
+// s_waitcnt 0 waits for every vector memory operation of the wave, stores
+// included (gfx9 counts them in vmcnt).  The intrinsic alone is not a memory
+// operation for the compiler, which could move memory accesses across it; the
+// signal fences on both sides keep every access of the wave on its side.
+__device__ __forceinline__ void wait_mem() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 __device__ __forceinline__ float key_to_float(uint32_t key) {
     const uint32_t b = (key & 0x80000000u) ? (key & 0x7FFFFFFFu) : ~key;
     return __uint_as_float(b);
 }
 
-// Claims the next chunk of slot T for this wave; returns it or ~0u.  Lane 0
-// issues the fetch-and-add and, behind it in the same round trip, the loads of
-// the task's launch-unique id and of best_q (it only decreases: a stale value
-// at worst searches a chunk past a known hit).  Every exit depends only on
-// wave-uniform values.
+// Claims the next chunk of slot T for this wave; returns it or ~0u, with the
+// task's generation in gen.  Lane 0 issues the fetch-and-add and, behind it in
+// the same round trip, the load of best_q; bq is its unit index when it
+// carries the claimed task's generation, else ~0 (an older task's value, read
+// before the claim was performed).  best_q only decreases during a task, so a
+// value of the right generation at worst searches a chunk past a known hit.
+// Every exit depends only on wave-uniform values.
 __device__ __forceinline__ unsigned claim_chunk(TaskSlot *T, unsigned *open_count, unsigned &gen,
-                                                unsigned long long &bq, unsigned &uid) {
+                                                unsigned long long &bq) {
     gen = 0;
     bq = ~0ull;
-    uid = 0;
     const unsigned long long snap = uni64(ld_relaxed(&T->ctl));
     if (ctl_next(snap) >= ctl_nch(snap)) return ~0u;
     unsigned long long old = 0, b = ~0ull;
-    unsigned u = 0;
     if (lane_id() == 0) {
         old = __hip_atomic_fetch_add(&T->ctl, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         b = ld_relaxed(&T->best_q);
-        u = ld_relaxed(&T->gen);
     }
     old = uni64(old);
-    bq = uni64(b);
-    uid = uni(u);
+    b = uni64(b);
     if (ctl_next(old) >= ctl_nch(old)) return ~0u;
     const unsigned ch = ctl_next(old);
     if (lane_id() == 0 && ch + 1u == ctl_nch(old)) atomicSub(open_count, 1u);
     gen = ctl_gen(old);
+    bq = ((unsigned)(b >> kQBits) == gen) ? (b & kQMask) : ~0ull;
     return ch;
 }
 
@@ -936,9 +957,10 @@ __device__ __forceinline__ int class_chunk_r(const DevCall &call, uint64_t units
 // (~0 = none).
 // The task's descriptor comes in registers: nclass, wpack, ppack, and per lane
 // cch (lane i: first chunk of class i, lane nclass: the chunk count) and cR
-// (lane i: units per lane per chunk of class i); bq is a recent best_q.
+// (lane i: units per lane per chunk of class i); bq is a recent best_q unit
+// index of generation gen (~0: none known).
 __device__ unsigned long long task_chunk(const DevCode &code, TaskSlot *T, const uint32_t LDS *s, uint32_t t0,
-                                        int tam, bool ml, unsigned ch, unsigned long long bq, int nclass,
+                                        int tam, bool ml, unsigned ch, unsigned gen, unsigned long long bq, int nclass,
                                         unsigned wpack, unsigned ppack, unsigned cch, int cR, const int LDS *act,
                                         const uint32_t LDS *sg, const float LDS *llr,
                                         const unsigned long long LDS *ht, unsigned hmask,
@@ -978,7 +1000,7 @@ __device__ unsigned long long task_chunk(const DevCode &code, TaskSlot *T, const
             const uint64_t r = pbase + (uint64_t)__shfl((long long)h.rank, hl, 64);
             const uint64_t q = ubase + q0 + (uint64_t)(h.j < 0 ? 0 : h.j);
             const uint64_t qh = (uint64_t)__shfl((long long)q, hl, 64);
-            if (lane == 0) { atomicMin(&T->best, (unsigned long long)r); atomicMin(&T->best_q, (unsigned long long)qh); }
+            if (lane == 0) { atomicMin(&T->best, (unsigned long long)r); atomicMin(&T->best_q, make_bq(gen, qh)); }
             out = r;
         }
     } else {
@@ -994,21 +1016,6 @@ __device__ unsigned long long task_chunk(const DevCode &code, TaskSlot *T, const
         }
     }
     return out;
-}
-
-// Debug trace of shared chunks (enabled by the host only for diagnosis).
-__device__ void trace_chunk(const DevCall &call, int who, int slot, unsigned gen, unsigned ch, int w, int tam, int R,
-                            uint64_t NP, uint32_t t0, const uint32_t LDS *s, const int LDS *act,
-                            const uint32_t LDS *sg, unsigned long long res) {
-    const unsigned active = (unsigned)__popcll(ballot(true));
-    if (!call.dbg || lane_id() != 0) return;
-    const unsigned i = atomicAdd(&call.dbg[0], 1u);
-    if (16u * (i + 1u) + 16u > call.dbg_cap) return;
-    unsigned *r = call.dbg + 16u * (i + 1u);
-    r[0] = (unsigned)who; r[1] = (unsigned)slot; r[2] = gen; r[3] = ch; r[4] = (unsigned)w; r[5] = (unsigned)tam;
-    r[6] = (unsigned)R; r[7] = (unsigned)NP; r[8] = t0; r[9] = s[0]; r[10] = (unsigned)act[0]; r[11] = sg[0];
-    r[12] = (unsigned)res; r[13] = (unsigned)(res >> 32); r[14] = blockIdx.x | (active << 24);
-    r[15] = (unsigned)now_ticks();
 }
 
 // Chunk ranges of a task's classes (lane i < nclass: first chunk and units per
@@ -1078,16 +1085,13 @@ __device__ int publish_shared(const DevCode &code, const DevCall &call, int shot
     unsigned myc;
     int myR;
     const unsigned nch = task_ranges(code, call, tam, nclass, wpack, ppack, myc, myR);
-    const unsigned gen = (ctl_gen(ld_relaxed(&T->ctl)) + 1u) & 0xFFFu;
-    unsigned uid = 0;
-    if (lane == 0) uid = atomicAdd(&call.task_counters[2], 1u);
-    uid = uni(uid);
+    const unsigned gen = (uni((unsigned)ctl_gen(ld_relaxed(&T->ctl))) + 1u) & 0xFFFFFu;
     if (lane == 0) {
-        st_relaxed(&T->gen, uid); st_relaxed(&T->nchunks, nch); st_relaxed(&T->shot, shot);
+        st_relaxed(&T->nchunks, nch); st_relaxed(&T->shot, shot);
         st_relaxed(&T->tam, tam); st_relaxed(&T->ml, ml ? 1 : 0); st_relaxed(&T->t0, t0);
         st_relaxed(&T->nclass, nclass); st_relaxed(&T->wpack, (int)wpack); st_relaxed(&T->ppack, (int)ppack);
         st_relaxed(&T->cch[nclass], nch);
-        st_relaxed(&T->done, 0u); st_relaxed(&T->best, ~0ull); st_relaxed(&T->best_q, ~0ull);
+        st_relaxed(&T->done, 0u); st_relaxed(&T->best, ~0ull); st_relaxed(&T->best_q, make_bq(gen, kQMask));
     }
     if (lane < nclass) { st_relaxed(&T->cch[lane], myc); st_relaxed(&T->cR[lane], myR); }
     if (lane < kMaxWords) st_relaxed(&T->s[lane], (uint32_t)s[lane]);
@@ -1116,13 +1120,12 @@ __device__ void work_shared(const DevCode &code, const DevCall &call, int slot, 
     const unsigned nch = task_ranges(code, call, tam, nclass, wpack, ppack, myc, myR);
     const unsigned cchv = (lane < nclass) ? myc : (lane == nclass ? nch : 0u);
     for (;;) {
-        unsigned g, u;
+        unsigned g;
         unsigned long long bq;
-        const unsigned ch = claim_chunk(T, &call.task_counters[0], g, bq, u);
+        const unsigned ch = claim_chunk(T, &call.task_counters[0], g, bq);
         if (ch == ~0u) break;
-        const unsigned long long res = task_chunk(code, T, s, t0, tam, ml, ch, bq, nclass, wpack, ppack, cchv, myR,
-                                                  act, sg, llr, ht, hmask, pkey, pval, phead, pbm);
-        trace_chunk(call, 0, slot, u, ch, (int)(wpack & 15u), tam, nclass, nch, t0, s, act, sg, res);
+        (void)task_chunk(code, T, s, t0, tam, ml, ch, g, bq, nclass, wpack, ppack, cchv, myR, act, sg, llr, ht, hmask,
+                         pkey, pval, phead, pbm);
         wait_mem();                               // the chunk's result atomics complete before it counts as done
         if (lane == 0) atomicAdd(&T->done, 1u);
     }
@@ -1136,15 +1139,17 @@ __device__ void work_shared(const DevCode &code, const DevCall &call, int slot, 
 // needed).  One exchange on ctl ends the claims; the chunks nobody claimed
 // count as done (and the task leaves the open count, which its last claim
 // would otherwise have done); the claimed ones see best_q = 0 and skip their
-// search.  Waits for the chunks in flight and frees the slot.
+// search.  Waits for the chunks in flight and frees the slot.  (Only the owner
+// changes the slot's generation and chunk count, so its own read of ctl holds
+// them.)
 __device__ void cancel_shared(const DevCall &call, int slot) {
     TaskSlot *T = &call.tasks[slot];
     unsigned nch = 0;
     if (lane_id() == 0) {
-        atomicMin(&T->best_q, 0ull);
         const unsigned long long c = ld_relaxed(&T->ctl);
         nch = ctl_nch(c);
-        const unsigned long long prev = __hip_atomic_exchange(&T->ctl, (c & ~0xFFFFFFFFull) | (unsigned long long)nch,
+        atomicMin(&T->best_q, make_bq(ctl_gen(c), 0ull));
+        const unsigned long long prev = __hip_atomic_exchange(&T->ctl, (c & ~0xFFFFFFull) | (unsigned long long)nch,
                                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned nxt = ctl_next(prev);
         if (nxt < nch) {
@@ -1176,7 +1181,8 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
 // arrays are copied into this wave's LDS and its tables (hash table and/or pair
 // table, as its classes need) built once per task.
 // The task a helper wave has in its LDS (arrays, syndrome words, tables) and
-// its descriptor in registers; key = the task's launch-unique id (-1: none).
+// its descriptor in registers; key = slot | generation << 7 of that task, as
+// returned by the claim that loaded it (-1: none).
 struct HelpCache {
     int key;
     unsigned hmask;
@@ -1201,14 +1207,18 @@ __device__ __forceinline__ bool help_task(const DevCode &code, const DevCall &ca
     else if (b1) slot = (int)uni((unsigned)(start + 64 + __ffsll((long long)b1) - 1) & (kTaskSlots - 1));
     if (slot < 0) return false;
     TaskSlot *T = &call.tasks[slot];
-    unsigned g, uid;
+    unsigned g;
     unsigned long long bq;
-    const unsigned ch = claim_chunk(T, &call.task_counters[0], g, bq, uid);
+    const unsigned ch = claim_chunk(T, &call.task_counters[0], g, bq);
     if (ch == ~0u) return true;
-    const int key = (int)(uid & 0x7FFFFFFFu);
+    static_assert(kTaskSlots <= 128, "the cache key holds the slot in 7 bits");
+    const int key = (int)((unsigned)slot | (g << 7));
     if (hc.key != key) {
-        // the task was complete in memory before its ctl was published; device-coherent
-        // loads read it after the claim (all issued together)
+        // The task was complete in memory before its ctl was published.  These
+        // device-coherent loads are issued after the claim returned (the key
+        // comes from its result), so they read the published task; the fences
+        // keep the compiler from moving them above that branch.
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
         hc.tam = ld_relaxed(&T->tam);
         hc.ml = ld_relaxed(&T->ml);
         hc.nclass = ld_relaxed(&T->nclass);
@@ -1244,11 +1254,8 @@ __device__ __forceinline__ bool help_task(const DevCode &code, const DevCall &ca
     const uint32_t LDS *s = L.sw;
     const int LDS *act = ml ? L.P : L.A;
     const uint32_t LDS *sg = ml ? L.sigP : L.sigA;
-    const unsigned long long res =
-        task_chunk(code, T, s, hc.t0, hc.tam, ml, ch, bq, hc.nclass, hc.wpack, hc.ppack, hc.cch, hc.cR, act, sg,
-                   ml ? L.llrP : nullptr, L.ht, hc.hmask, L.pkey, L.pval, L.phead, L.pbm);
-    trace_chunk(call, 1, slot, (unsigned)key, ch, (int)(hc.wpack & 15u), hc.tam, hc.nclass,
-                (unsigned)__builtin_amdgcn_readlane((int)hc.cch, hc.nclass), hc.t0, s, act, sg, res);
+    (void)task_chunk(code, T, s, hc.t0, hc.tam, ml, ch, g, bq, hc.nclass, hc.wpack, hc.ppack, hc.cch, hc.cR, act, sg,
+                     ml ? L.llrP : nullptr, L.ht, hc.hmask, L.pkey, L.pval, L.phead, L.pbm);
     wait_mem();                                   // the chunk's result atomics complete before it counts as done
     if (lane == 0) atomicAdd(&T->done, 1u);
     return true;
@@ -1625,9 +1632,10 @@ __device__ __forceinline__ int branch_of(const DevCall &call, int tam) {
 // ----------------------------------------------------------------- result write
 // The 72-byte record is written as 18 dwords by 18 lanes.
 __device__ void write_result(GrandDecoderResult *dst, int found, int fw, int ow, int deg, int eq, int tam,
-                             uint64_t count, int hib, int it, double t_search, double t_find) {
+                             uint64_t count, int hib, int it, double t_sum, double t_search, double t_find) {
     const int lane = lane_id();
     uint32_t v = 0;
+    const uint64_t tu = (uint64_t)__double_as_longlong(t_sum);
     const uint64_t ts = (uint64_t)__double_as_longlong(t_search);
     const uint64_t tf = (uint64_t)__double_as_longlong(t_find);
     switch (lane) {
@@ -1641,6 +1649,8 @@ __device__ void write_result(GrandDecoderResult *dst, int found, int fw, int ow,
         case 7: v = (uint32_t)(count >> 32); break;
         case 8: v = (uint32_t)hib; break;
         case 9: v = (uint32_t)it; break;
+        case 10: v = (uint32_t)tu; break;
+        case 11: v = (uint32_t)(tu >> 32); break;
         case 12: v = (uint32_t)ts; break;
         case 13: v = (uint32_t)(ts >> 32); break;
         case 16: v = (uint32_t)tf; break;
@@ -1693,12 +1703,12 @@ __device__ __attribute__((noinline)) void finalize_shot_v(const DevCode &code, c
                                                           int w2, int w3, int w4, int w5, int w6, int w7,
                                                           unsigned long long count, unsigned long long wops,
                                                           unsigned long long t0, unsigned long long t1,
-                                                          const uint32_t LDS *slot) {
+                                                          unsigned long long ta, const uint32_t LDS *slot) {
     ShotOutcome o;
     o.found = found; o.fw = fw; o.iteration = iteration; o.tam = tam; o.hibrido = hibrido;
     o.win[0] = w0; o.win[1] = w1; o.win[2] = w2; o.win[3] = w3; o.win[4] = w4; o.win[5] = w5; o.win[6] = w6;
     o.win[7] = w7;
-    o.count = count; o.wops = wops; o.t0 = t0; o.t1 = t1;
+    o.count = count; o.wops = wops; o.t0 = t0; o.t1 = t1; o.ta = ta;
     const int lane = lane_id();
     const int M32 = code.M32;
     uint32_t dw[2] = {0u, 0u};   // decoded pattern words lane + 64 q, q < 2 (m <= 4096)
@@ -1779,9 +1789,11 @@ __device__ __attribute__((noinline)) void finalize_shot_v(const DevCode &code, c
         }
         if (o.found) deg = ballot(!same) == 0ull;
     }
+    // time_sum_ms: the scores and the first active list; total_time_search_ms: the whole search
+    const double t_sum = o.ta ? (double)(o.ta - o.t0) * call.ms_per_tick : 0.0;
     const double t_search = (double)(o.t1 - o.t0) * call.ms_per_tick;
     write_result(&call.results[shot], o.found, o.found ? o.fw : 0, ow, deg, eq, o.tam, o.count, o.hibrido,
-                 o.iteration, t_search, o.found ? t_search : 0.0);
+                 o.iteration, t_sum, t_search, o.found ? t_search : 0.0);
     if (call.stats && lane == 0) {
         call.stats[3 * (size_t)shot] = o.t0;
         call.stats[3 * (size_t)shot + 1] = o.t1;
@@ -1795,7 +1807,7 @@ static_assert(kMaxWin == 8, "finalize_shot_v takes eight decoded columns");
 __device__ __forceinline__ void finalize_shot(const DevCode &code, const DevCall &call, int shot, const ShotOutcome &o,
                                               const uint32_t LDS *slot) {
     finalize_shot_v(code, call, shot, o.found, o.fw, o.iteration, o.tam, o.hibrido, o.win[0], o.win[1], o.win[2],
-                    o.win[3], o.win[4], o.win[5], o.win[6], o.win[7], o.count, o.wops, o.t0, o.t1, slot);
+                    o.win[3], o.win[4], o.win[5], o.win[6], o.win[7], o.count, o.wops, o.t0, o.t1, o.ta, slot);
 }
 
 // ------------------------------------------------------------ learned pass
@@ -1851,6 +1863,7 @@ __device__ bool learned_pass(const DevCode &code, const DevCall &call, int shot,
     const int lane = lane_id();
     const int thr = code.max_soma;
     int tam = build_active(code, s, thr, L, call.cap);
+    o.ta = now_ticks();
     if (call.family == kCLN && tam > call.max_size) tam = 0;      // the first iteration has no list
     if (tam <= 0 || tam > call.cap || (!call.lper_column && tam > call.lstride) || branch_of(call, tam) == 0)
         return false;
@@ -1895,7 +1908,7 @@ __device__ bool learned_pass(const DevCode &code, const DevCall &call, int shot,
 #pragma unroll
                 for (int l = 0; l < kMaxWin; ++l) o.win[l] = (l < wp) ? L.P[c[l]] : ((l == wp) ? ck : -1);
                 o.count += G + r + 1;
-                o.wops += (G + r + 1) * (uint64_t)(wp + 2);
+                o.wops += (r + 1) * (uint64_t)(wp + 2);   // the earlier classes' share is already in wops
                 o.found = 1;
                 o.iteration = 1;
                 o.tam = tam;
@@ -1950,7 +1963,7 @@ __device__ __forceinline__ void decode_shot_wave(const DevCode &code, const DevC
     const uint32_t LDS *s = L.sw;
     const uint32_t t0 = uni(slot[1]);
 
-    o.found = 0; o.fw = 0; o.count = 0; o.wops = 0; o.hibrido = -2;
+    o.found = 0; o.fw = 0; o.count = 0; o.wops = 0; o.hibrido = -2; o.ta = 0;
 #pragma unroll
     for (int l = 0; l < kMaxWin; ++l) o.win[l] = -1;
     int it = 0, tam = 0, tam_old = 0;
@@ -1962,6 +1975,7 @@ __device__ __forceinline__ void decode_shot_wave(const DevCode &code, const DevC
         const int thr = code.max_soma - it;
         ++it;
         tam = build_active(code, s, thr, L, call.cap);
+        if (it == 1 && o.ta == 0) o.ta = now_ticks();
         if (call.family == kCLN) { if (tam > call.max_size) tam = tam_old; else tam_old = tam; }
         if (tam <= 0) continue;
         if (tam > call.cap) {                      // beyond the supported active-set size
@@ -1971,8 +1985,11 @@ __device__ __forceinline__ void decode_shot_wave(const DevCode &code, const DevC
         const int br = branch_of(call, tam);
         if (br == 0) continue;
         if (spec_ok && it == 2 && tam >= kSpecTam) {
+            // publish_iteration3 builds the third list in A / sigA before it
+            // decides whether to publish, so the second list is rebuilt whatever
+            // it returned
             spec = publish_iteration3(code, call, shot, s, t0);
-            if (spec >= 0) tam = build_active(code, s, thr, L, call.cap);   // the second list again
+            tam = build_active(code, s, thr, L, call.cap);
         }
         o.hibrido = (br > 0) ? (o.hibrido == -1 ? 1 : 0) : -1;
         bool hit;
@@ -2209,6 +2226,7 @@ __device__ __forceinline__ bool lane_search(const DevCode &code, const DevCall &
             ++tam;
         }
     }
+    const unsigned long long ta = now_ticks();     // the active list is built (time_sum_ms)
     if (tam == 0 || tam > kLaneMax || tam > call.cap) return false;
     if (call.family == kCLN && tam > call.max_size) return false;   // tam falls back to 0: second iteration
     const int br = branch_of(call, tam);
@@ -2378,10 +2396,12 @@ __device__ __forceinline__ bool lane_search(const DevCode &code, const DevCall &
     }
     const double ts = (double)(t1 - t0) * call.ms_per_tick;
     const uint64_t tsb = (uint64_t)__double_as_longlong(ts);
+    const uint64_t tub = (uint64_t)__double_as_longlong((double)(ta - t0) * call.ms_per_tick);
     uint32_t *p = reinterpret_cast<uint32_t *>(&call.results[shot]);
     p[0] = 1u; p[1] = (uint32_t)hw; p[2] = (uint32_t)ow; p[3] = (uint32_t)deg; p[4] = (uint32_t)eq;
     p[5] = (uint32_t)tam; p[6] = count; p[7] = 0u; p[8] = (uint32_t)((br > 0) ? 0 : -1); p[9] = 1u;
-    p[10] = 0u; p[11] = 0u; p[12] = (uint32_t)tsb; p[13] = (uint32_t)(tsb >> 32); p[14] = 0u; p[15] = 0u;
+    p[10] = (uint32_t)tub; p[11] = (uint32_t)(tub >> 32); p[12] = (uint32_t)tsb; p[13] = (uint32_t)(tsb >> 32);
+    p[14] = 0u; p[15] = 0u;
     p[16] = (uint32_t)tsb; p[17] = (uint32_t)(tsb >> 32);
     if (call.stats) {
         call.stats[3 * (size_t)shot] = t0;

@@ -1,6 +1,7 @@
 // grand_types.h -- device-resident code/call descriptors shared by the host
 // driver (grand_host.cpp) and the gfx950 kernels (grand_kernels.hip).
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 #include "../../include/grand_decoder.h"
 
@@ -63,7 +64,7 @@ struct DevCode {
 
 // A weight class too large for one wave, shared through the task table.  The
 // owner wave publishes it; any wave claims chunks of 64 * R prefixes with one
-// fetch-and-add on ctl = (generation << 52) | (nchunks << 32) | next_chunk.
+// fetch-and-add on ctl = (generation << 44) | (nchunks << 24) | next_chunk.
 constexpr int kTaskSlots = 128;
 constexpr int kCodeLdsMaxWords = 4096;   // codes up to 16 KB of staged words (DevCode::lds_words) go to LDS
 constexpr int kMaxBatch = 8;             // most shots per worklist claim
@@ -101,11 +102,10 @@ struct TaskSlot {
     unsigned long long ctl;           // generation | nchunks | next chunk to claim
     unsigned long long best;          // first-hit: lowest matching pattern index over the task's classes
                                       // (classes concatenated in list order); ML: (cost key << 32) | rank
-    unsigned long long best_q;        // first-hit: unit index of that hit over the task's classes (later
-                                      // chunks are skipped)
+    unsigned long long best_q;        // first-hit: generation << 44 | unit index of that hit over the task's
+                                      // classes (later chunks are skipped)
     unsigned done;                    // chunks finished
     unsigned busy;                    // 0 free, 1 owned
-    unsigned gen;                     // launch-unique task id (helpers' LDS cache key)
     unsigned nchunks;
     int shot, tam, ml;
     int nclass;                       // weight classes of the task, consecutive in the list (ML: 1)
@@ -116,6 +116,11 @@ struct TaskSlot {
     uint32_t t0;                      // signature of the syndrome
     uint32_t s[kMaxWords];            // syndrome words
 };
+
+// Byte offsets the compiled-code check of the publication order relies on.
+static_assert(offsetof(TaskSlot, ctl) == 0 && offsetof(TaskSlot, best_q) == 16 && offsetof(TaskSlot, done) == 24 &&
+                  offsetof(TaskSlot, busy) == 28,
+              "TaskSlot layout");
 
 // Per-call parameters and per-shot buffers.
 struct DevCall {
@@ -156,10 +161,8 @@ struct DevCall {
     unsigned long long *stats;        // [shots][3] claim tick, completion tick, weighted pattern count; or null
     TaskSlot *tasks;                  // [kTaskSlots]
     uint32_t *task_arrays;            // [kTaskSlots][3][cap]: active list, signatures, sorted LLRs
-    unsigned *task_counters;          // [0] open tasks, [1] waves finished with their own shots, [2] task ids
+    unsigned *task_counters;          // [0] open tasks, [1] waves finished with their own shots
     unsigned long long split_threshold;  // classes larger than this are shared (0 = never)
-    unsigned *dbg;                    // optional chunk trace (debug builds of a run), [0] = records
-    unsigned dbg_cap;
     // learned search order, tried before the plain search (lscores == null: off)
     const float *lscores;             // per shot: [lstride] scores, by slot of the first active list or by column
     int lstride;
@@ -174,7 +177,8 @@ struct ShotOutcome {
     int win[kMaxWin];                 // decoded columns (ascending within the pattern order)
     unsigned long long count;
     unsigned long long wops;          // sum over tested patterns of (weight + 1)
-    unsigned long long t0, t1;        // device timestamps
+    unsigned long long t0, t1;        // device timestamps: search start and end
+    unsigned long long ta;            // device timestamp when the first active list was built (0: none)
 };
 
 }  // namespace grand

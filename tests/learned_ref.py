@@ -27,8 +27,21 @@ def first_active(H, s):
     return [c for c in range(H.shape[1]) if score[c] >= max_soma]
 
 
-def learned_pass(H, s, scores, per_column, maxw, budget, max_size=None):
-    """Returns (hit columns or None, patterns tried, tam)."""
+def _pack(bits):
+    """0/1 rows (..., n) -> uint64 words (..., ceil(n / 64)), bit r of the row in word r // 64."""
+    bits = np.asarray(bits, dtype=np.uint64)
+    n = bits.shape[-1]
+    nw = (n + 63) // 64
+    out = np.zeros(bits.shape[:-1] + (nw,), dtype=np.uint64)
+    for r in range(n):
+        out[..., r // 64] |= bits[..., r] << np.uint64(r % 64)
+    return out
+
+
+def learned_pass(H, s, scores, per_column, maxw, budget, max_size=None, block=4096):
+    """Returns (hit columns or None, patterns tried, tam).  The patterns of one
+    (stage, weight) are tested in blocks of lexicographic order (numpy XOR of
+    the packed columns); the first matching one in order is the hit."""
     A = first_active(H, s)
     tam = len(A)
     if max_size is not None and tam > max_size:
@@ -38,21 +51,27 @@ def learned_pass(H, s, scores, per_column, maxw, budget, max_size=None):
     sc = np.array([scores[c] if per_column else scores[i] for i, c in enumerate(A)], dtype=np.float32)
     key = np.where(np.isnan(sc), np.inf, -sc).astype(np.float32)
     P = [A[i] for i in np.argsort(key, kind="stable")]
-    # columns and syndrome as Python integers (bit r = row r): a pattern
-    # matches when the XOR of its columns equals the syndrome
-    colint = [int("".join("1" if H[r, c] else "0" for r in reversed(range(H.shape[0]))), 2) for c in P]
-    sint = int("".join("1" if s[r] else "0" for r in reversed(range(H.shape[0]))), 2)
+    cols = _pack(np.asarray(H)[:, P].T)              # [tam][words]
+    sv = _pack(np.asarray(s))
     tried = 0
     for k in range(tam):
-        tk = sint ^ colint[k]
+        tk = sv ^ cols[k]
         for wp in range(min(maxw, k + 1)):
-            for S in itertools.combinations(range(k), wp):
-                if tried >= budget:
+            combos = itertools.combinations(range(k), wp)
+            while True:
+                left = budget - tried
+                if left <= 0:
                     return None, tried, tam
-                tried += 1
-                x = tk
-                for j in S:
-                    x ^= colint[j]
-                if x == 0:
-                    return sorted([P[j] for j in S] + [P[k]]), tried, tam
+                chunk = list(itertools.islice(combos, min(block, left)))
+                if not chunk:
+                    break
+                idx = np.array(chunk, dtype=np.int64).reshape(len(chunk), wp)
+                x = np.repeat(tk[None, :], len(chunk), axis=0)
+                for j in range(wp):
+                    x ^= cols[idx[:, j]]
+                z = np.nonzero(~x.any(axis=1))[0]
+                if z.size:
+                    i = int(z[0])
+                    return sorted([P[j] for j in idx[i]] + [P[k]]), tried + i + 1, tam
+                tried += len(chunk)
     return None, tried, tam

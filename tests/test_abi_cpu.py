@@ -239,5 +239,3 @@ def test_reference_file_and_simulation_helpers(tmp_path):
     assert e.any() and e.shape == (72,)
     pos = G.error_positions_random(72, 5)
     assert len(set(pos.tolist())) == 5 and pos.max() < 72
-    with pytest.raises(ImportError):
-        G.build_quantum_circuit_H(hz, [1], True)

@@ -111,36 +111,74 @@ def _small_dem_task():
     return learned.DemTask(seed=5, n_det=216, n_mech=800, max_size=64, W=3)
 
 
+def _check_learned(task, sess, det, lg, o, scores, per_column, budget, maxw):
+    """The learned decode of every shot against learned_ref, and after the
+    budget against the oracle's plain result (its count plus the pass's)."""
+    r, bits, _ = task.decode(sess, det, lg, scores=torch.from_numpy(scores).cuda(), per_column=per_column,
+                             budget=budget, max_weight=maxw)
+    bits = bits.cpu().numpy()
+    det_np = det.cpu().numpy()
+    hits = 0
+    for i in range(det.shape[0]):
+        s = det_np[i]
+        if not s.any():
+            assert r["found"][i] == 0 and r["total_tested_combinations"][i] == 0
+            continue
+        hit, tried, tam = learned_ref.learned_pass(task.H, s, scores[i], per_column, maxw, budget, task.max_size)
+        if hit is not None:
+            hits += 1
+            assert r["found"][i] == 1 and r["iteration"][i] == 1 and r["tam"][i] == tam, i
+            assert r["total_tested_combinations"][i] == tried, i
+            assert sorted(np.nonzero(bits[i])[0].tolist()) == hit, i
+        else:
+            for f in ("found", "found_weight", "degenerate", "tam", "iteration"):
+                assert r[f][i] == o[f][i], (i, f)
+            assert r["total_tested_combinations"][i] == o["total_tested_combinations"][i] + tried, i
+    return hits
+
+
 def test_learned_pass_matches_cpu_statement(oracle):
     task = _small_dem_task()
     det, lg = task.sample(300, seed=4)
     rng = np.random.default_rng(2)
     sess = task.session()
-    cfg = task.cfg
-    det_np, lg_np = det.cpu().numpy(), lg.cpu().numpy()
-    rc, o = T.run_cln(oracle, G.KIND_PROB_CLN, task.H, task.L, det_np, lg_np, cfg=cfg, file=task.phase_file)[:2]
+    rc, o = T.run_cln(oracle, G.KIND_PROB_CLN, task.H, task.L, det.cpu().numpy(), lg.cpu().numpy(), cfg=task.cfg,
+                      file=task.phase_file)[:2]
     assert rc == 0
     for per_column, budget, maxw in ((True, 50, 4), (True, 20_000, 4), (False, 2000, 4), (True, 3000, 8)):
         width = task.m if per_column else task.max_size
         scores = rng.standard_normal((det.shape[0], width)).astype(np.float32)
         scores[0, :] = 0.0                                          # ties keep list order
-        r, bits, _ = task.decode(sess, det, lg, scores=torch.from_numpy(scores).cuda(), per_column=per_column,
-                                 budget=budget, max_weight=maxw)
-        bits = bits.cpu().numpy()
-        for i in range(det.shape[0]):
-            s = det_np[i]
-            if not s.any():
-                assert r["found"][i] == 0 and r["total_tested_combinations"][i] == 0
-                continue
-            hit, tried, tam = learned_ref.learned_pass(task.H, s, scores[i], per_column, maxw, budget, task.max_size)
-            if hit is not None:
-                assert r["found"][i] == 1 and r["iteration"][i] == 1 and r["tam"][i] == tam, i
-                assert r["total_tested_combinations"][i] == tried, i
-                assert sorted(np.nonzero(bits[i])[0].tolist()) == hit, i
-            else:
-                for f in ("found", "found_weight", "degenerate", "tam", "iteration"):
-                    assert r[f][i] == o[f][i], (i, f)
-                assert r["total_tested_combinations"][i] == o["total_tested_combinations"][i] + tried, i
+        _check_learned(task, sess, det, lg, o, scores, per_column, budget, maxw)
+    sess.close()
+
+
+@pytest.fixture(scope="module")
+def dem_full_learned(oracle):
+    """The learned-order benchmark's DEM (216 x 3000, max_error_weight 4,
+    max_size_active_positions 64), 200 shots, and the oracle's plain
+    grand_decode_prob_cln results on them."""
+    task = learned.DemTask()
+    det, lg = task.sample(200, seed=6)
+    rc, o = T.run_cln(oracle, G.KIND_PROB_CLN, task.H, task.L, det.cpu().numpy(), lg.cpu().numpy(), cfg=task.cfg,
+                      file=task.phase_file)[:2]
+    assert rc == 0
+    return task, det, lg, o
+
+
+@pytest.mark.parametrize("per_column", [True, False])
+def test_learned_pass_dem216x3000_w4(dem_full_learned, per_column):
+    """Both heads (a score per mechanism, a score per active-list position) with
+    random scores, budgets 1e3 and 1e5, patterns up to weight 8."""
+    task, det, lg, o = dem_full_learned
+    assert (task.n, task.m, task.W, task.max_size) == (216, 3000, 4, 64)
+    rng = np.random.default_rng(7 if per_column else 8)
+    sess = task.session()
+    width = task.m if per_column else task.max_size
+    scores = rng.standard_normal((det.shape[0], width)).astype(np.float32)
+    for budget in (1000, 100_000):
+        hits = _check_learned(task, sess, det, lg, o, scores, per_column, budget, 8)
+        assert hits >= 20
     sess.close()
 
 
