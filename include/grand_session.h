@@ -54,6 +54,24 @@ int grand_session_decode_device(GrandSession *s, const int *d_input, int input_i
                                 int num_shots, GrandDecoderResult *d_results, unsigned *d_decoded,
                                 unsigned *d_decoded_logicals, unsigned long long *d_stats, void *stream);
 
+/* Decode without allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  As
+ * grand_session_decode_device, same arguments and results, but the decode's
+ * device scratch (packed syndromes, worklist, counters, shared-task table,
+ * the launch descriptor) comes from a fixed array of 32 MiB that the library
+ * holds on every GPU it runs on until the process ends; it calls no hipMalloc
+ * or hipFree.  The session itself (the uploaded code) is created beforehand
+ * with grand_session_create.  A batch whose scratch exceeds the array is
+ * rejected with -1 and a message on stderr that names the most shots of the
+ * session's code that fit (about 440,000 at BB144, 41,000
+ * on a 216 x 3000 DEM); so is the plain engine.  Every call on one GPU shares the
+ * array: calls that overlap on one GPU (concurrent streams or threads) race. */
+int grand_session_decode_device_noalloc(GrandSession *s, const int *d_input, int input_is_errors,
+                                        const int *d_logicals, int num_shots, GrandDecoderResult *d_results,
+                                        unsigned *d_decoded, unsigned *d_decoded_logicals,
+                                        unsigned long long *d_stats, void *stream);
+
 /* Learned search order.  As grand_session_decode_device, but every shot first
  * runs a learned pass over its first active list: positions sorted by
  * descending score, patterns of weight 1..max_weight tried so that a pattern

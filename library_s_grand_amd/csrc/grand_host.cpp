@@ -37,6 +37,7 @@ GRAND_INTERNAL hipError_t grand_launch_triage(const grand::DevCode *code, const 
 GRAND_INTERNAL hipError_t grand_launch_fused(const grand::DevCode *code, const grand::DevCall *call, const grand::DevCall *hcall,
                               int code_lds_words, int grid, hipStream_t st);
 GRAND_INTERNAL hipError_t grand_launch_expand(const uint32_t *packed, int words, int cols, int rows, int *out, hipStream_t st);
+GRAND_INTERNAL hipError_t grand_scratch_address(void **ptr, hipStream_t st);
 GRAND_INTERNAL size_t grand_plain_state_bytes();
 GRAND_INTERNAL size_t grand_plain_record_bytes();
 GRAND_INTERNAL hipError_t grand_launch_plain_active(const grand::DevCode *code, const grand::DevCall *call, int cap, int shot,
@@ -812,32 +813,41 @@ int run_plain(GrandSession *s, hipStream_t st) {
     return 0;
 }
 
-int decode_device(GrandSession *s, const int *d_input, int input_errors, const int *d_logicals, int num_shots,
-                  GrandDecoderResult *d_results, unsigned *d_dec, unsigned *d_declog, unsigned long long *d_stats,
-                  hipStream_t st) {
+// Device scratch of one decode: the session's Workspace, or regions of the
+// library's fixed scratch array (decode without allocation).
+struct WsView {
+    uint32_t *s_packed, *e_packed, *e_aux, *lg_packed, *s_sig, *dec, *declog, *task_arrays;
+    int *worklist;
+    unsigned *counters;
+    TaskSlot *tasks;
+    DevCall *dcall;
+};
+
+// Queues one decode with all device scratch given by `w`; allocates nothing.
+int decode_view(GrandSession *s, const WsView &w, const int *d_input, int input_errors, const int *d_logicals,
+                int num_shots, GrandDecoderResult *d_results, unsigned *d_dec, unsigned *d_declog,
+                unsigned long long *d_stats, hipStream_t st) {
     Runtime &R = rt();
-    if (num_shots <= 0) return 0;
     const DevCode &code = s->code->dev;
-    if (!s->ws.ensure(code, num_shots, s->call.cap)) return -1;
     DevCall &c = s->call;
     c.num_shots = num_shots;
     c.input_errors = (s->family == kCC) ? (input_errors ? 1 : 0) : 0;
     c.input = d_input;
     c.logicals = d_logicals;
-    c.s_packed = s->ws.s_packed.as<uint32_t>();
-    c.e_packed = s->ws.e_packed.as<uint32_t>();
-    c.e_aux = s->ws.e_aux.as<uint32_t>();
-    c.lg_packed = s->ws.lg_packed.as<uint32_t>();
-    c.s_sig = s->ws.s_sig.as<uint32_t>();
-    c.worklist = s->ws.worklist.as<int>();
-    c.counters = s->ws.counters.as<unsigned>();
+    c.s_packed = w.s_packed;
+    c.e_packed = w.e_packed;
+    c.e_aux = w.e_aux;
+    c.lg_packed = w.lg_packed;
+    c.s_sig = w.s_sig;
+    c.worklist = w.worklist;
+    c.counters = w.counters;
     c.results = d_results;
-    c.dec = d_dec ? d_dec : s->ws.dec.as<uint32_t>();
-    c.declog = d_declog ? d_declog : s->ws.declog.as<uint32_t>();
+    c.dec = d_dec ? d_dec : w.dec;
+    c.declog = d_declog ? d_declog : w.declog;
     c.stats = d_stats;
-    c.tasks = s->ws.tasks.as<TaskSlot>();
-    c.task_arrays = s->ws.task_arrays.as<uint32_t>();
-    c.task_counters = s->ws.counters.as<unsigned>() + 4;
+    c.tasks = w.tasks;
+    c.task_arrays = w.task_arrays;
+    c.task_counters = w.counters + 4;
     {
         const int st_ex = s->ex.split_threshold;
         c.split_threshold = st_ex < 0 ? 0ull
@@ -882,20 +892,13 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
         }
     }
 
-    // The task table is cleared once when allocated: every launch leaves all
-    // slots idle (each owner waits for its chunks and frees its slot).  The
-    // worklist length, claim cursor, error bits and task counters restart with
-    // every batch, in the launch that installs the batch descriptor.
-    if (s->ws.tasks_fresh) {
-        HIPX(grand_launch_fill(reinterpret_cast<unsigned *>(c.tasks), 0u, (int)(sizeof(TaskSlot) * kTaskSlots / 4), st));
-        s->ws.tasks_fresh = false;
-    }
+    // The worklist length, claim cursor, error bits and task counters restart
+    // with every batch, in the launch that installs the batch descriptor.
     const int groups = (num_shots + 63) / 64;
     const int tgrid = std::max(1, std::min(groups, R.cu_count * 16));
-    if (!s->dcall.ensure(sizeof(DevCall))) return -1;
-    HIPX(grand_launch_prep(&c, s->dcall.as<DevCall>(), c.counters, st));
+    HIPX(grand_launch_prep(&c, w.dcall, c.counters, st));
     const DevCode *dcode = s->code->dcode.as<DevCode>();
-    const DevCall *dcall = s->dcall.as<DevCall>();
+    const DevCall *dcall = w.dcall;
     hipEvent_t *tev = nullptr;
     if (s->timing) {
         const int k = (s->tslot + 1) % GrandSession::kTimingSlots;
@@ -922,6 +925,75 @@ int decode_device(GrandSession *s, const int *d_input, int input_errors, const i
     HIPX(grand_launch_fused(s->code->dcode_search.as<DevCode>(), dcall, &c, s->code->search_lds_words, grid, st));
     if (tev) { HIPX(hipEventRecord(tev[2], st)); s->tpending |= 1ull << s->tslot; }
     return 0;
+}
+
+// The regions of the fixed scratch array for a batch of num_shots shots, each
+// 256-byte aligned: the task table first (it stays valid from call to call),
+// then counters, the device descriptor, the task arrays and the per-shot
+// buffers.  Returns the bytes used; with base null only sizes.
+size_t scratch_layout(const DevCode &c, int num_shots, int cap, char *base, WsView &w) {
+    const size_t S = (size_t)std::max(num_shots, 1);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char *p = base ? base + off : nullptr;
+        off += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    w.tasks = reinterpret_cast<TaskSlot *>(take(sizeof(TaskSlot) * kTaskSlots));
+    w.counters = reinterpret_cast<unsigned *>(take(kCounterBytes));
+    w.dcall = reinterpret_cast<DevCall *>(take(sizeof(DevCall)));
+    w.task_arrays = reinterpret_cast<uint32_t *>(take((size_t)kTaskSlots * 3 * std::max(cap, 1) * 4));
+    w.s_packed = reinterpret_cast<uint32_t *>(take(S * c.D * 4));
+    w.e_packed = reinterpret_cast<uint32_t *>(take(S * c.M32 * 4));
+    w.e_aux = reinterpret_cast<uint32_t *>(take(S * 8));
+    w.lg_packed = reinterpret_cast<uint32_t *>(take(S * c.NL32 * 4));
+    w.s_sig = reinterpret_cast<uint32_t *>(take(S * 4));
+    w.worklist = reinterpret_cast<int *>(take((S + 63) / 64 * 64 * 4));
+    w.dec = reinterpret_cast<uint32_t *>(take(S * c.M32 * 4));
+    w.declog = reinterpret_cast<uint32_t *>(take(S * c.NL32 * 4));
+    return off;
+}
+
+// Largest batch whose scratch layout fits the fixed array.
+int noalloc_max_shots(GrandSession *s) {
+    WsView w{};
+    int lo = 0, hi = 1 << 26;
+    while (lo < hi) {                              // largest batch whose layout fits the array
+        const int mid = lo + (hi - lo + 1) / 2;
+        if (scratch_layout(s->code->dev, mid, s->call.cap, nullptr, w) <= kScratchBytes) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Queues one decode with the session's own scratch, grown on demand.
+int decode_device(GrandSession *s, const int *d_input, int input_errors, const int *d_logicals, int num_shots,
+                  GrandDecoderResult *d_results, unsigned *d_dec, unsigned *d_declog, unsigned long long *d_stats,
+                  hipStream_t st) {
+    if (num_shots <= 0) return 0;
+    const DevCode &code = s->code->dev;
+    if (!s->ws.ensure(code, num_shots, s->call.cap) || !s->dcall.ensure(sizeof(DevCall))) return -1;
+    // The task table is cleared once when allocated: every launch leaves all
+    // slots idle (each owner waits for its chunks and frees its slot).
+    if (s->ws.tasks_fresh) {
+        HIPX(grand_launch_fill(reinterpret_cast<unsigned *>(s->ws.tasks.p), 0u,
+                               (int)(sizeof(TaskSlot) * kTaskSlots / 4), st));
+        s->ws.tasks_fresh = false;
+    }
+    WsView w;
+    w.s_packed = s->ws.s_packed.as<uint32_t>();
+    w.e_packed = s->ws.e_packed.as<uint32_t>();
+    w.e_aux = s->ws.e_aux.as<uint32_t>();
+    w.lg_packed = s->ws.lg_packed.as<uint32_t>();
+    w.s_sig = s->ws.s_sig.as<uint32_t>();
+    w.dec = s->ws.dec.as<uint32_t>();
+    w.declog = s->ws.declog.as<uint32_t>();
+    w.task_arrays = s->ws.task_arrays.as<uint32_t>();
+    w.worklist = s->ws.worklist.as<int>();
+    w.counters = s->ws.counters.as<unsigned>();
+    w.tasks = s->ws.tasks.as<TaskSlot>();
+    w.dcall = s->dcall.as<DevCall>();
+    return decode_view(s, w, d_input, input_errors, d_logicals, num_shots, d_results, d_dec, d_declog, d_stats, st);
 }
 
 // ----------------------------------------------------------------- ABI path
@@ -1122,6 +1194,35 @@ int grand_session_decode_device(GrandSession *s, const int *d_input, int input_i
     return decode_device(s, d_input, input_is_errors, d_logicals, num_shots, d_results, d_decoded, d_decoded_logicals,
                          d_stats, st);
 }
+
+int grand_session_decode_device_noalloc(GrandSession *s, const int *d_input, int input_is_errors,
+                                        const int *d_logicals, int num_shots, GrandDecoderResult *d_results,
+                                        unsigned *d_decoded, unsigned *d_decoded_logicals,
+                                        unsigned long long *d_stats, void *stream) {
+    if (!s || num_shots < 0 || (num_shots > 0 && (!d_input || !d_results))) return -1;
+    if (s->family == kCLN && s->code->NL > 0 && num_shots > 0 && !d_logicals) return -1;
+    if (num_shots == 0) return 0;
+    if (s->ex.engine == GRAND_ENGINE_PLAIN) {
+        fprintf(stderr, "grand: decode without allocation is not available for the plain engine\n");
+        return -1;
+    }
+    WsView w{};
+    const size_t need = scratch_layout(s->code->dev, num_shots, s->call.cap, nullptr, w);
+    if (need > kScratchBytes) {
+        fprintf(stderr, "grand: %d shots need %zu bytes of device scratch; decode without allocation holds %zu "
+                        "(at most %d shots of this code)\n",
+                num_shots, need, kScratchBytes, noalloc_max_shots(s));
+        return -1;
+    }
+    std::lock_guard<std::recursive_mutex> lock(rt().mu);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    void *base = nullptr;
+    if (!hip_check(grand_scratch_address(&base, st), "device scratch address")) return -1;
+    scratch_layout(s->code->dev, num_shots, s->call.cap, static_cast<char *>(base), w);
+    return decode_view(s, w, d_input, input_is_errors, d_logicals, num_shots, d_results, d_decoded,
+                       d_decoded_logicals, d_stats, st);
+}
+
 
 int grand_session_decode_learned(GrandSession *s, const int *d_input, int input_is_errors, const int *d_logicals,
                                  int num_shots, const float *d_scores, int stride, int scores_per_column,

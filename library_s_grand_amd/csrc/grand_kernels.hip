@@ -29,6 +29,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <mutex>
 #include "grand_types.h"
 
 namespace grand {
@@ -3109,6 +3110,38 @@ GRAND_INTERNAL hipError_t grand_launch_expand(const uint32_t *packed, int words,
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL(grand::expand_bits_kernel, dim3(grid), dim3(256), 0, st, packed, words, cols, rows, out);
     return hipGetLastError();
+}
+
+// Fixed device scratch of grand_session_decode_device_noalloc: a file-scope
+// array every GPU holds (its own copy, at its own address) until the process
+// ends.  Host code never uses the array's name as a device address: its
+// address on the current device comes from hipGetSymbolAddress, resolved once
+// per device and kept, and only that address is passed to kernels and fills.
+// The task table sits at offset 0 and is cleared here, once per device, with
+// a fill kernel on `st`; every decode leaves it idle.  Every call on one GPU
+// shares the array, so calls that overlap on one GPU race.
+static __device__ __align__(256) unsigned char scratch[grand::kScratchBytes];
+
+GRAND_INTERNAL hipError_t grand_scratch_address(void **ptr, hipStream_t st) {
+    static std::mutex mu;
+    static void *addr[grand::kScratchDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= grand::kScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        const int words = (int)(sizeof(grand::TaskSlot) * grand::kTaskSlots / 4);
+        hipLaunchKernelGGL(grand::fill_u32_kernel, dim3((words + 255) / 256), dim3(256), 0, st, (unsigned *)p, 0u, words);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        addr[dev] = p;
+    }
+    *ptr = addr[dev];
+    return hipSuccess;
 }
 
 GRAND_INTERNAL size_t grand_plain_state_bytes() { return sizeof(grand::PlainState); }

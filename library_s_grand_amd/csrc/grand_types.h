@@ -98,6 +98,10 @@ constexpr int kWlBase = 2 * (kClaimBase + kClaimWords * kClaimStride);
 constexpr int kWlStride = 32;
 constexpr int kCounterBytes = 4 * (kWlBase + kWlParts * kWlStride);
 constexpr int kTaskClasses = 6;          // weight classes one shared task can hold
+// Fixed device scratch of the decode that allocates nothing: its size, and the
+// most devices whose address of it the library keeps.
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+constexpr int kScratchDevices = 64;
 struct TaskSlot {
     unsigned long long ctl;           // generation | nchunks | next chunk to claim
     unsigned long long best;          // first-hit: lowest matching pattern index over the task's classes
