@@ -8,12 +8,14 @@ does.  One step = one decode of the whole 100,000-shot batch with the errors
 already resident in HBM (session API: triage kernel + persistent search kernel).
 
 Reported beside the fused engine:
-  * the plain engine (host loop, one launch per weight class) on a sample,
+  * the plain engine (host loop, one launch per weight class) on the same
+    100,000 shots, bit-identical to the fused results and patterns,
   * per-shot service latency p50 / p99 from device timestamps,
   * the integer-VALU roofline of the search kernel and the HBM roofline of the
     triage kernel, from HIP events recorded on the decode stream,
   * the CPU oracle (a restatement of the reference CPU search) timed on 1 core,
-  * a bit-exactness check of the fused results against the oracle,
+  * a bit-exactness check of the fused results and decoded patterns of every
+    shot against the oracle,
   * the serving loop on a mixed BB72/BB144/BB288 Poisson request stream
     (request latency p50 / p99, served shots/s),
   * BB72 grand_decode_cc (10,000 shots, W=3) and the 216 x 3000 DEM with
@@ -46,7 +48,6 @@ METRIC = "decoded shots/s + p99 us/shot, BB144 w<=4; fused vs plain GPU, bit-exa
 SHOTS = 100_000
 SEED = 20240501
 W = 4
-PLAIN_SAMPLE = 3000
 CPU_TARGET_S = 10.0
 PARITY_FIELDS = G.PARITY_FIELDS
 
@@ -58,9 +59,16 @@ def workload(shots=SHOTS, seed=SEED):
     return hx, hz, errors
 
 
-def cpu_baseline(hx, hz, errors, prob_file, fused_res):
+def unpack_patterns(words, m):
+    """[shots][ceil(m/32)] uint32 words of decoded patterns -> [shots][m] 0/1."""
+    w = np.ascontiguousarray(words).astype(np.uint32).reshape(words.shape[0], -1)
+    return np.unpackbits(w.view(np.uint8), axis=1, bitorder="little")[:, :m].astype(np.int32)
+
+
+def cpu_baseline(hx, hz, errors, prob_file, fused_res, fused_dec):
     """Oracle (restatement of the reference CPU search) on 1 host core: passes
-    over the full workload until >= CPU_TARGET_S of work; also checks parity."""
+    over the full workload until >= CPU_TARGET_S of work; also checks parity of
+    every result field and every decoded pattern."""
     lib = G.bind(ctypes.CDLL(os.path.join(ROOT, "oracle", "libgrand_oracle.so")))
     lib.oracle_set_threads.argtypes = [ctypes.c_int]
     lib.oracle_set_threads(1)
@@ -78,6 +86,7 @@ def cpu_baseline(hx, hz, errors, prob_file, fused_res):
             break
     o = first[1]
     bad = sum(int((o[f] != fused_res[f]).sum()) for f in PARITY_FIELDS)
+    bad += int((first[2] != fused_dec).any(axis=1).sum())
     single = {"value": passes * errors.shape[0] / elapsed, "unit": "shots/s", "cores": 1, "kind": "port",
               "sample": f"full {errors.shape[0]}-shot workload x {passes} pass(es), {elapsed:.1f} s, "
                         f"per-shot printing off"}
@@ -356,26 +365,28 @@ def main():
     out["abi_host_buffers"] = {"shots_per_s": shots / abi_s, "ms_per_call": abi_s * 1e3, "rc": rc_abi,
                                "basis": "grand_decode_prob_cc on host int32 error rows: H2D of the rows, decode, "
                                         "D2H of the 72-byte results, per call"}
+    fused_dec = unpack_patterns(d_dec.cpu().numpy().view(np.uint32).reshape(shots, -1), hx.shape[1])
     if not a.no_plain:
-        k = min(PLAIN_SAMPLE, shots)
+        # the plain engine on the same shots (the whole workload, deep shots included)
         ex = G.make_ex(engine=G.GRAND_ENGINE_PLAIN)
         ps = Session(G.KIND_PROB_CC, hx, hz, cfg, type_of_error="x", file=prob_file, ex=ex, lib=lib)
-        d_res_p = torch.empty(k * 72, dtype=torch.uint8, device=dev)
-        ps.decode(d_err[:k], True, d_res_p, stream=stream)   # warm
+        d_res_p = torch.empty(shots * 72, dtype=torch.uint8, device=dev)
+        d_dec_p = torch.empty(shots * sess.pattern_words, dtype=torch.int32, device=dev)
+        ps.decode(d_err[:2000], True, d_res_p, stream=stream, num_shots=2000)   # warm
         torch.cuda.synchronize()
         t = time.perf_counter()
-        ps.decode(d_err[:k], True, d_res_p, stream=stream)
+        ps.decode(d_err, True, d_res_p, d_decoded=d_dec_p, stream=stream)
         torch.cuda.synchronize()
         tp = time.perf_counter() - t
         rp = results_from_bytes(d_res_p.cpu().numpy())
-        same = all(bool((rp[f] == res[f][:k]).all()) for f in PARITY_FIELDS)
-        plain_sps = k / tp
-        out["plain"] = {"shots_per_s": plain_sps, "sample": f"first {k} shots of the workload",
+        same = all(bool((rp[f] == res[f]).all()) for f in PARITY_FIELDS) and bool(torch.equal(d_dec_p, d_dec))
+        plain_sps = shots / tp
+        out["plain"] = {"shots_per_s": plain_sps, "seconds": tp, "sample": f"all {shots} shots of the workload",
                         "bit_identical_to_fused": same}
         out["fused_vs_plain"] = out["value"] / plain_sps
         ps.close()
     if not a.no_cpu:
-        cb, cbm, ok, _ = cpu_baseline(hx, hz, errors, prob_file, res)
+        cb, cbm, ok, _ = cpu_baseline(hx, hz, errors, prob_file, res, fused_dec)
         out["cpu_baseline"] = cb
         out["cpu_baseline_all_cores"] = cbm
         out["bit_exact_vs_oracle"] = ok
