@@ -1051,6 +1051,48 @@ __device__ __forceinline__ void wait_task_done(const DevCall &call, TaskSlot *T,
     }
 }
 
+// A free task slot, taken (busy = 1), or -1 when none is free.  64 candidates
+// per step; lane 0 takes the first free one.  Out of line: publications are
+// rare, and the scan's registers stay out of the callers.
+__device__ __attribute__((noinline)) int find_free_slot(TaskSlot *tasks) {
+    const int lane = lane_id();
+    // find a free slot: 64 candidates per step, lane 0 takes the first free one
+    int slot = -1;
+    const int start = (int)((blockIdx.x * 131u) & (kTaskSlots - 1));
+    // a slot is free when idle, or withdrawn with every chunk done (its last
+    // chunk's result atomics completed before that chunk counted as done)
+    for (int b0 = 0; b0 < kTaskSlots; b0 += 64) {
+        const int k = (start + b0 + lane) & (kTaskSlots - 1);
+        const unsigned bz = ld_relaxed(&tasks[k].busy);
+        const unsigned dn = ld_relaxed(&tasks[k].done);
+        const unsigned nc = ld_relaxed(&tasks[k].nchunks);
+        const unsigned want = (bz == 0u || (bz == 2u && dn >= nc)) ? bz : 1u;
+        uint64_t freem = ballot(want != 1u);
+        while (freem) {
+            const int bit = __ffsll((long long)freem) - 1;
+            const int pick = (int)uni((unsigned)(start + b0 + bit) & (kTaskSlots - 1));
+            const unsigned expect = (unsigned)__builtin_amdgcn_readlane((int)want, bit);
+            unsigned old = 1u;
+            if (lane == 0) old = atomicCAS(&tasks[pick].busy, expect, 1u);
+            if (uni(old) == expect) {
+                if (expect == 0u) { slot = pick; break; }
+                // a withdrawn slot: its done count, read after the claim, must
+                // show every chunk done (the loads above may predate it)
+                unsigned ok = 0u;
+                if (lane == 0) {
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    ok = ld_relaxed(&tasks[pick].done) >= ld_relaxed(&tasks[pick].nchunks) ? 1u : 0u;
+                    if (!ok) st_relaxed(&tasks[pick].busy, 2u);
+                }
+                if (uni(ok)) { slot = pick; break; }
+            }
+            freem &= freem - 1;
+        }
+        if (slot >= 0) break;
+    }
+    return slot;
+}
+
 // Owner side: publishes nclass consecutive weight classes of the list act/sg/llr
 // (weights in wpack, 4 bits each; bit i of ppack: class i by stems) as one
 // shared task and returns its slot, or -1 when no slot is free.  The task is
@@ -1060,21 +1102,7 @@ __device__ int publish_shared(const DevCode &code, const DevCall &call, int shot
                               int tam, bool ml, int nclass, unsigned wpack, unsigned ppack, const int LDS *act,
                               const uint32_t LDS *sg, const float LDS *llr) {
     const int lane = lane_id();
-    // find a free slot: 64 candidates per step, lane 0 takes the first free one
-    int slot = -1;
-    const int start = (int)((blockIdx.x * 131u) & (kTaskSlots - 1));
-    for (int b0 = 0; b0 < kTaskSlots; b0 += 64) {
-        const int k = (start + b0 + lane) & (kTaskSlots - 1);
-        uint64_t freem = ballot(ld_relaxed(&call.tasks[k].busy) == 0u);
-        while (freem) {
-            const int pick = (int)uni((unsigned)(start + b0 + __ffsll((long long)freem) - 1) & (kTaskSlots - 1));
-            unsigned old = 1u;
-            if (lane == 0) old = atomicCAS(&call.tasks[pick].busy, 0u, 1u);
-            if (uni(old) == 0u) { slot = pick; break; }
-            freem &= freem - 1;
-        }
-        if (slot >= 0) break;
-    }
+    const int slot = find_free_slot(call.tasks);
     if (slot < 0) return -1;
     TaskSlot *T = &call.tasks[slot];
     uint32_t *arr = call.task_arrays + (size_t)slot * 3 * call.cap;
@@ -1137,18 +1165,17 @@ __device__ void work_shared(const DevCode &code, const DevCall &call, int slot, 
 }
 
 // Owner side: withdraws its published task in `slot` (its result is no longer
-// needed).  One exchange on ctl ends the claims; the chunks nobody claimed
-// count as done (and the task leaves the open count, which its last claim
-// would otherwise have done); the claimed ones see best_q = 0 and skip their
-// search.  Waits for the chunks in flight and frees the slot.  (Only the owner
-// changes the slot's generation and chunk count, so its own read of ctl holds
-// them.)
+// needed), without waiting.  One exchange on ctl ends the claims; the chunks
+// nobody claimed count as done (and the task leaves the open count, which its
+// last claim would otherwise have done); chunks claimed later see best_q = 0
+// and skip their search.  The slot is marked withdrawn: a publisher takes it
+// once every chunk in flight is done.  (Only the owner changes the slot's
+// generation and chunk count, so its own read of ctl holds them.)
 __device__ void cancel_shared(const DevCall &call, int slot) {
     TaskSlot *T = &call.tasks[slot];
-    unsigned nch = 0;
     if (lane_id() == 0) {
         const unsigned long long c = ld_relaxed(&T->ctl);
-        nch = ctl_nch(c);
+        const unsigned nch = ctl_nch(c);
         atomicMin(&T->best_q, make_bq(ctl_gen(c), 0ull));
         const unsigned long long prev = __hip_atomic_exchange(&T->ctl, (c & ~0xFFFFFFull) | (unsigned long long)nch,
                                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1158,10 +1185,8 @@ __device__ void cancel_shared(const DevCall &call, int slot) {
             atomicSub(&call.task_counters[0], 1u);
         }
     }
-    nch = uni(nch);
-    wait_task_done(call, T, nch);
-    wait_mem();
-    if (lane_id() == 0) st_relaxed(&T->busy, 0u);
+    wait_mem();                                   // the unclaimed chunks count as done before the slot is withdrawn
+    if (lane_id() == 0) st_relaxed(&T->busy, 2u);
 }
 
 // Owner side: publish a task and work on it; false when no slot is free (the
@@ -1207,6 +1232,12 @@ __device__ __forceinline__ bool help_task(const DevCode &code, const DevCall &ca
     if (b0) slot = (int)uni((unsigned)(start + __ffsll((long long)b0) - 1) & (kTaskSlots - 1));
     else if (b1) slot = (int)uni((unsigned)(start + 64 + __ffsll((long long)b1) - 1) & (kTaskSlots - 1));
     if (slot < 0) return false;
+    if (hc.key >= 0) {
+        // stay on the task whose tables this wave holds while it has chunks
+        const int cs = hc.key & (kTaskSlots - 1);
+        const int rel = (cs - start) & (kTaskSlots - 1);
+        if ((((rel < 64) ? b0 : b1) >> (rel & 63)) & 1ull) slot = cs;
+    }
     TaskSlot *T = &call.tasks[slot];
     unsigned g;
     unsigned long long bq;
@@ -1441,8 +1472,6 @@ __device__ int first_hit_wave(const DevCode &code, const DevCall &call, int shot
                     }
                     ++kend;
                 }
-                if (need_pairs && !have_pairs) { build_pairs(L, L.sigA, tam); have_pairs = true; }
-                if (need_ht && !have_table) { hmask = build_ht(L.ht, L.sigA, tam); have_table = true; }
                 unsigned long long best;
                 bool shared = false;
                 if (spec >= 0) {
@@ -1454,6 +1483,12 @@ __device__ int first_hit_wave(const DevCode &code, const DevCall &call, int shot
                                       uni((unsigned)ld_relaxed(&T->ppack)) == ppack &&
                                       uni((unsigned)ld_relaxed(&T->tam)) == (unsigned)tam;
                     if (same) {
+                        // tables only when chunks are left to claim (other waves may have
+                        // claimed them all; the claims never come back)
+                        const unsigned long long snap = uni64(ld_relaxed(&T->ctl));
+                        const bool left = ctl_next(snap) < ctl_nch(snap);
+                        if (left && need_pairs && !have_pairs) { build_pairs(L, L.sigA, tam); have_pairs = true; }
+                        if (left && need_ht && !have_table) { hmask = build_ht(L.ht, L.sigA, tam); have_table = true; }
                         work_shared(code, call, spec, s, t0, tam, false, nc, wpack, ppack, L.A, L.sigA, nullptr, L.ht,
                                     hmask, L.pkey, L.pval, L.phead, L.pbm, best);
                         shared = true;
@@ -1463,9 +1498,13 @@ __device__ int first_hit_wave(const DevCode &code, const DevCall &call, int shot
                     spec = -1;
                     spec_done = 2;
                 }
-                if (!shared)
+                if (!shared) {
+                    // the list's tables, for this wave's own chunks of the run
+                    if (need_pairs && !have_pairs) { build_pairs(L, L.sigA, tam); have_pairs = true; }
+                    if (need_ht && !have_table) { hmask = build_ht(L.ht, L.sigA, tam); have_table = true; }
                     shared = run_shared(code, call, shot, s, t0, tam, false, nc, wpack, ppack, L.A, L.sigA, nullptr, L.ht,
                                         hmask, L.pkey, L.pval, L.phead, L.pbm, best);
+                }
                 if (shared) {
                     uint64_t span = 0;
                     for (int k = kc; k < kend; ++k) span += N[k];
@@ -2767,7 +2806,7 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
     // the registrations of all words and no class is open; waves that are not
     // resident yet hold no work.  The open-class poll rides along with the claim.
     unsigned long long *cw = reinterpret_cast<unsigned long long *>(call.counters) + kClaimBase;
-    int cur = (int)(blockIdx.x & (kClaimWords - 1)), dry = 0;
+    int cur = (int)(blockIdx.x & (kClaimWords - 1));
     unsigned claims = 0;
     bool drained = false;
     for (;;) {
@@ -2796,7 +2835,19 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
                 hc.key = -1;
                 continue;
             }
-            if (++dry < kClaimWords) { cur = (cur + 1) & (kClaimWords - 1); continue; }
+            // this word is dry: read every word's cursor at once (they only grow)
+            // and move to the next word that still holds positions, or drain
+            unsigned live = 0u;
+            if (lane_id() < kClaimWords) {
+                const unsigned c = (unsigned)ld_relaxed(cw + (size_t)lane_id() * kClaimStride);
+                live = ((unsigned)lane_id() + kClaimWords * c < total) ? 1u : 0u;
+            }
+            const uint64_t lv = ballot(live != 0u) & ~(1ull << cur);
+            if (lv) {
+                const uint64_t after = lv & ~((2ull << cur) - 1ull);   // words after cur, else the first one
+                cur = (int)uni((unsigned)(__ffsll((long long)(after ? after : lv)) - 1));
+                continue;
+            }
             drained = true;
             if (lane_id() == 0) atomicAdd(&call.task_counters[1], claims);
         }

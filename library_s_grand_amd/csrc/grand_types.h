@@ -109,7 +109,7 @@ struct TaskSlot {
     unsigned long long best_q;        // first-hit: generation << 44 | unit index of that hit over the task's
                                       // classes (later chunks are skipped)
     unsigned done;                    // chunks finished
-    unsigned busy;                    // 0 free, 1 owned
+    unsigned busy;                    // 0 free, 1 owned, 2 withdrawn (free once done == nchunks)
     unsigned nchunks;
     int shot, tam, ml;
     int nclass;                       // weight classes of the task, consecutive in the list (ML: 1)
