@@ -2861,7 +2861,7 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
         reg = uni(reg);
         const unsigned op = uni(ld_relaxed(&call.task_counters[0]));
         if (fin == reg && op == 0u) break;
-        __builtin_amdgcn_s_sleep(2);
+        __builtin_amdgcn_s_sleep(64);
     }
 }
 
