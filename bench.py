@@ -151,10 +151,10 @@ def leg_bb72_cc(lib, oracle_path, tune=True):
     return out
 
 
-def leg_dem_ml(lib, oracle_path, shots=1_000_000, sample=2000, tune=True):
+def leg_dem_ml(lib, oracle_path, info, shots=1_000_000, sample=2000, tune=True):
     """Synthetic DEM 216 x 3000 grand_decode_ML_cln, max_error_weight = 4, max_size = 64: throughput on
-    `shots` shots sampled on the device (knobs tuned on the first 100,000), and a bit-exact check against the
-    oracle on the first `sample`."""
+    `shots` shots sampled on the device (knobs tuned on the first 100,000), kernel times, the search kernel's
+    work model and rooflines, and a bit-exact check against the oracle on the first `sample`."""
     import torch
     from library_s_grand_amd import learned
     task = learned.DemTask(W=4, max_size=64)
@@ -172,19 +172,35 @@ def leg_dem_ml(lib, oracle_path, shots=1_000_000, sample=2000, tune=True):
         return s.last_timing()
 
     tuning = tune_leg(lib, run, tune)
+    d_st = torch.empty(shots * 3, dtype=torch.int64, device="cuda")
     t = time.perf_counter()
-    s.decode(det, False, d_r, d_logicals=lg)
+    s.decode(det, False, d_r, d_logicals=lg, d_stats=d_st)
     torch.cuda.synchronize()
     sec = time.perf_counter() - t
+    tri_ms, srch_ms = s.last_timing()
     res = results_from_bytes(d_r.cpu().numpy())
     if s.error_bits() != 0:
         raise RuntimeError("decoder error bits")
-    s.close()
     cnt = res["total_tested_combinations"].astype(np.float64)
+    wops = int(d_st.cpu().numpy().view(np.uint64).reshape(shots, 3)[:, 2].sum())
+    D32 = (task.n + 31) // 32
+    peak_int = info["cus"] * 64 * info["clock_khz"] * 1e3 / 1e12
+    bf = wops * D32 / (srch_ms * 1e-3) / 1e12
+    wm = work_model(s, lambda: s.decode(det, False, d_r, d_logicals=lg), srch_ms, float(cnt.sum()))
+    s.close()
     out = {"workload": "synthetic DEM 216x3000 (column weight 1..4, 6 windows), 12 logicals, grand_decode_ML_cln, "
                        f"{shots} shots, max_error_weight=4, max_size_active_positions=64",
            "shots_per_s": shots / sec, "seconds": sec, "patterns_per_s": cnt.sum() / sec,
-           "mean_patterns": float(cnt.mean()), "tuning": tuning}
+           "mean_patterns": float(cnt.mean()), "tuning": tuning,
+           "kernel_ms": {"triage": tri_ms, "search": srch_ms},
+           "roofline_bruteforce": {"achieved": bf, "peak": peak_int, "unit": "TOP/s", "frac": bf / peak_int,
+                                   "basis": f"(w+1)*ceil(n/32) int32 ops per oracle-counted pattern ({wops} weighted "
+                                            f"patterns x {D32} words) / search-kernel time: the ops a pattern-by-"
+                                            "pattern walk would issue; the prefix walk covers a class with "
+                                            "C(tam-1, w-1) probes, so this exceeds the VALU peak"},
+           "work_model": wm,
+           "roofline_hw": hardware_roofline(measured_counters("search_kernel_counters_dem216x3000_ml_cln.md"),
+                                            info, "grand::fused_kernel")}
     if oracle_path:
         olib = G.bind(ctypes.CDLL(oracle_path))
         olib.oracle_set_threads.argtypes = [ctypes.c_int]
@@ -212,6 +228,59 @@ def measured_traffic():
     return out
 
 
+def measured_counters(name):
+    """Per-launch hardware counters of the search kernel from a committed rocprofv3
+    PMC summary (lines 'hw <kernel> <counter> <value>')."""
+    path = os.path.join(ROOT, "profiles", name)
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            f = line.split()
+            if len(f) == 4 and f[0] == "hw":
+                out.setdefault(f[1], {})[f[2]] = float(f[3])
+    return out
+
+
+def hardware_roofline(c, info, kernel):
+    """VALU issue and LDS activity of one launch from its SQ counters (cycle counters in units of 4
+    cycles, a wave64 VALU instruction occupies its SIMD for 4 cycles) over the kernel's trace duration."""
+    k = c.get(kernel)
+    if not k or "duration_us" not in k:
+        return None
+    cyc = k["duration_us"] * 1e-6 * info["clock_khz"] * 1e3
+    simd_cycles = cyc * info["cus"] * 4
+    out = {"kernel": kernel, "duration_us": k["duration_us"],
+           "valu_busy": 4 * k["SQ_INSTS_VALU"] / simd_cycles,
+           "basis": "SQ_INSTS_VALU x 4 cycles / (trace duration x clock x 4 SIMDs per CU x CUs), from the "
+                    "committed profile; the int32 ops the VALU could issue in that time are the peak"}
+    if "SQ_WAIT_ANY" in k and "SQ_WAVE_CYCLES" in k:
+        out["wave_cycles_waiting"] = k["SQ_WAIT_ANY"] / k["SQ_WAVE_CYCLES"]
+    if "SQ_LDS_IDX_ACTIVE" in k:
+        out["lds_active"] = 4 * k["SQ_LDS_IDX_ACTIVE"] / (cyc * info["cus"])
+        if "SQ_LDS_BANK_CONFLICT" in k:
+            out["lds_bank_conflict_share"] = k["SQ_LDS_BANK_CONFLICT"] / k["SQ_LDS_IDX_ACTIVE"]
+    return out
+
+
+def work_model(sess, run, srch_ms, patterns):
+    """Units the search kernel executed in one decode, counted on the device: (w-1)-prefix steps (one
+    signature-table probe each), (w-2)-stem steps (one pair-table lookup each) and patterns tested one per
+    lane; their rate over the mean search-kernel time, and the oracle-counted patterns each unit covers."""
+    import torch
+    d_work = torch.zeros(3, dtype=torch.int64, device="cuda")
+    sess.set_work_counters(d_work)
+    run()
+    torch.cuda.synchronize()
+    sess.set_work_counters(None)
+    w = [int(x) for x in d_work.cpu().numpy()]
+    units = sum(w)
+    return {"prefix_steps": w[0], "stem_steps": w[1], "flat_patterns": w[2], "units": units,
+            "units_per_s": units / (srch_ms * 1e-3) if srch_ms > 0 else None,
+            "patterns_per_unit": patterns / units if units else None,
+            "basis": "device counters of one decode (steps past a lane's hit not counted) / mean search-kernel "
+                     "time of the timed steps"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,6 +293,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-order leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in tuning defaults")
     ap.add_argument("--no-legs", action="store_true", help="skip the BB72 and DEM ML legs")
+    ap.add_argument("--dem-ml-only", type=int, default=0, metavar="SHOTS",
+                    help="run only the DEM grand_decode_ML_cln leg on this many shots, untuned (for profiling)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB twice (a kernel of known traffic "
                          "for scaling hardware byte counters)")
@@ -235,6 +306,10 @@ def main():
 
     import torch
     dev = torch.device("cuda:0")
+    if a.dem_ml_only:
+        lib = G.load_library()
+        print(json.dumps({"dem_ml_cln": leg_dem_ml(lib, None, device_info(lib), shots=a.dem_ml_only, tune=False)}))
+        return
     hx, hz, errors = workload(a.shots)
     tmp = tempfile.mkdtemp(prefix="grand_bench_")
     prob_file = os.path.join(tmp, "phase.txt")
@@ -324,6 +399,8 @@ def main():
     achieved_bw = tri_bytes / (tri * 1e-3) / 1e9
 
     traffic = measured_traffic() if shots == SHOTS else {}
+    wm = work_model(sess, step, srch, patterns)
+    hw = hardware_roofline(measured_counters("search_kernel_counters_bb144_prob_cc.md"), info, "grand::fused_kernel")
     out = {
         "metric": METRIC, "value": shots / (ms_per_step * 1e-3), "unit": "shots/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
@@ -349,6 +426,8 @@ def main():
                      "basis": f"(w+1)*ceil(n/32) int32 ops per tested pattern, {wops} weighted patterns "
                               f"x {D32} words per launch / mean search-kernel time; peak = {info['cus']} CU x 64 "
                               f"lanes x {info['clock_khz'] / 1e6:.2f} GHz"},
+        "roofline_hw": hw,
+        "work_model": wm,
         "roofline_triage": {"bound": "hbm", "achieved": achieved_bw, "peak": 8000.0, "unit": "GB/s",
                             "frac": achieved_bw / 8000.0, "traffic": traffic.get("grand::triage_kernel"),
                             "kernel": "grand::triage_kernel",
@@ -398,7 +477,7 @@ def main():
         oracle_path = None if a.no_cpu else os.path.join(ROOT, "oracle", "libgrand_oracle.so")
         out["bb72_cc"] = leg_bb72_cc(lib, oracle_path, tune=not a.no_tune)
         autotune.reset(lib)
-        out["dem_ml_cln"] = leg_dem_ml(lib, oracle_path, tune=not a.no_tune)
+        out["dem_ml_cln"] = leg_dem_ml(lib, oracle_path, info, tune=not a.no_tune)
         autotune.reset(lib)
     if not a.no_serving:
         from library_s_grand_amd import serving

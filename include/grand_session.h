@@ -128,6 +128,16 @@ int grand_session_last_timing(GrandSession *s, float *triage_ms, float *search_m
 int grand_session_timing_totals(GrandSession *s, double *triage_ms, double *search_ms, long long *decodes);
 int grand_session_timing_reset(GrandSession *s);
 
+/* Work counters of the search kernel for later decodes on this session (no
+ * counterpart in the reference): d_work is device memory of 3 uint64 that the
+ * kernels add to (the caller zeroes it): [0] (w-1)-prefix steps walked, one
+ * signature-table probe each; [1] (w-2)-stem steps walked, one pair-table
+ * lookup each; [2] patterns tested one per lane (small classes, weight 1).
+ * Steps walked past a lane's hit are not counted; every other step is, so
+ * these are the units the hardware executed.  NULL stops counting.
+ * Returns 0, or -1 for a NULL session. */
+int grand_session_set_work_counters(GrandSession *s, unsigned long long *d_work);
+
 /* Device wall-clock period in milliseconds per tick (for d_stats). */
 double grand_session_ms_per_tick(GrandSession *s);
 

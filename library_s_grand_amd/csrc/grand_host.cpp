@@ -1355,6 +1355,12 @@ int grand_session_timing_reset(GrandSession *s) {
     return 0;
 }
 
+int grand_session_set_work_counters(GrandSession *s, unsigned long long *d_work) {
+    if (!s) return -1;
+    s->call.work = d_work;
+    return 0;
+}
+
 double grand_session_ms_per_tick(GrandSession *s) { return s ? s->call.ms_per_tick : 0.0; }
 int grand_session_pattern_words(GrandSession *s) { return s ? s->code->dev.M32 : 0; }
 int grand_session_logical_words(GrandSession *s) { return s ? s->code->dev.NL32 : 0; }

@@ -850,6 +850,18 @@ __device__ PrefHit weight_one(const DevCode &code, const uint32_t LDS *s, uint32
     return best;
 }
 
+// Adds each lane's count to call.work[idx] (one atomic per wave), when counting.
+__device__ __forceinline__ void count_work(unsigned long long *work, int idx, uint32_t n) {
+    if (!work) return;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) n += (uint32_t)__shfl_xor((int)n, o, 64);
+    if (lane_id() == 0 && n) atomicAdd(&work[idx], (unsigned long long)n);
+}
+// Steps a lane's walk of len units executed: up to and including its hit.
+__device__ __forceinline__ uint32_t walked(const PrefHit &h, int len, bool ml) {
+    return (uint32_t)((!ml && h.j >= 0) ? h.j + 1 : (len > 0 ? len : 0));
+}
+
 // ---------------------------------------------------------------- shared tasks
 // ctl word: generation (20 bits) | nchunks (20 bits) | next chunk (24 bits).
 // Chunks are claimed with one fetch-and-add on ctl: a returned `next` below
@@ -966,7 +978,8 @@ __device__ unsigned long long task_chunk(const DevCode &code, TaskSlot *T, const
                                         const uint32_t LDS *sg, const float LDS *llr,
                                         const unsigned long long LDS *ht, unsigned hmask,
                                         const uint32_t LDS *pkey, const uint32_t LDS *pval,
-                                        const uint32_t LDS *phead, const uint32_t LDS *pbm) {
+                                        const uint32_t LDS *phead, const uint32_t LDS *pbm,
+                                        unsigned long long *work) {
     const int lane = lane_id();
     int w = 0, R = 1;
     bool pairs = false;
@@ -993,6 +1006,7 @@ __device__ unsigned long long task_chunk(const DevCode &code, TaskSlot *T, const
     const PrefHit h = ml ? scan_prefix_w<1>(w, code, s, t0, q0, len, tam, act, sg, llr, ht, hmask)
                     : pairs ? scan_stems_w(w, code, s, t0, q0, len, tam, act, sg, pkey, pval, phead, pbm)
                             : scan_prefix_w<0>(w, code, s, t0, q0, len, tam, act, sg, llr, ht, hmask);
+    count_work(work, pairs ? 1 : 0, walked(h, len, ml));
     unsigned long long out = ~0ull;
     if (!ml) {
         const uint64_t b = ballot(h.j >= 0);
@@ -1154,7 +1168,7 @@ __device__ void work_shared(const DevCode &code, const DevCall &call, int slot, 
         const unsigned ch = claim_chunk(T, &call.task_counters[0], g, bq);
         if (ch == ~0u) break;
         (void)task_chunk(code, T, s, t0, tam, ml, ch, g, bq, nclass, wpack, ppack, cchv, myR, act, sg, llr, ht, hmask,
-                         pkey, pval, phead, pbm);
+                         pkey, pval, phead, pbm, call.work);
         wait_mem();                               // the chunk's result atomics complete before it counts as done
         if (lane == 0) atomicAdd(&T->done, 1u);
     }
@@ -1287,7 +1301,7 @@ __device__ __forceinline__ bool help_task(const DevCode &code, const DevCall &ca
     const int LDS *act = ml ? L.P : L.A;
     const uint32_t LDS *sg = ml ? L.sigP : L.sigA;
     (void)task_chunk(code, T, s, hc.t0, hc.tam, ml, ch, g, bq, hc.nclass, hc.wpack, hc.ppack, hc.cch, hc.cR, act, sg,
-                     ml ? L.llrP : nullptr, L.ht, hc.hmask, L.pkey, L.pval, L.phead, L.pbm);
+                     ml ? L.llrP : nullptr, L.ht, hc.hmask, L.pkey, L.pval, L.phead, L.pbm, call.work);
     wait_mem();                                   // the chunk's result atomics complete before it counts as done
     if (lane == 0) atomicAdd(&T->done, 1u);
     return true;
@@ -1304,7 +1318,11 @@ __device__ uint64_t class_first_hit(const DevCode &code, const DevCall &call, in
                                     bool pairs, const uint32_t LDS *pkey, const uint32_t LDS *pval,
                                     const uint32_t LDS *phead, const uint32_t LDS *pbm) {
     const int lane = lane_id();
-    if (w == 1) return weight_one<0>(code, s, t0, tam, act, sg, nullptr).rank;
+    if (w == 1) {
+        const PrefHit h1 = weight_one<0>(code, s, t0, tam, act, sg, nullptr);
+        count_work(call.work, 2, lane == 0 ? (uint32_t)(h1.j >= 0 ? h1.rank + 1 : tam) : 0u);
+        return h1.rank;
+    }
     // units: stems with the pair table, else prefixes
     const uint64_t NP = class_units(code, tam, w, pairs);
     uint64_t pos = 0;
@@ -1316,6 +1334,7 @@ __device__ uint64_t class_first_hit(const DevCode &code, const DevCall &call, in
         const int len = q0 >= NP ? 0 : (int)((NP - q0) < (uint64_t)R ? (NP - q0) : (uint64_t)R);
         const PrefHit h = pairs ? scan_stems_w(w, code, s, t0, q0, len, tam, act, sg, pkey, pval, phead, pbm)
                                 : scan_prefix_w<0>(w, code, s, t0, q0, len, tam, act, sg, nullptr, ht, hmask);
+        count_work(call.work, pairs ? 1 : 0, walked(h, len, false));
         const uint64_t b = ballot(h.j >= 0);
         if (b) return (uint64_t)__shfl((long long)h.rank, __ffsll((long long)b) - 1, 64);
         pos += 64ull * (uint64_t)R;
@@ -1443,6 +1462,7 @@ __device__ int first_hit_wave(const DevCode &code, const DevCall &call, int shot
                     }
                 }
                 const uint64_t b = ballot(cand);
+                count_work(call.work, 2, idx < flat_end ? 1u : 0u);
                 if (b) {
                     hit = true;
                     hit_lane = __ffsll((long long)b) - 1;
@@ -1591,6 +1611,7 @@ __device__ bool ml_wave(const DevCode &code, const DevCall &call, int shot, cons
         uint64_t crank = 0;
         if (w == 1) {
             const PrefHit h = weight_one<1>(code, s, t0, tam, L.P, L.sigP, L.llrP);
+            count_work(call.work, 2, lane == 0 ? (uint32_t)tam : 0u);
             if (h.j >= 0) { chit = true; cbest = h.cost; crank = h.rank; }
         } else {
             const uint64_t NP = ncr_w(code, tam - 1, w - 1);
@@ -1612,6 +1633,7 @@ __device__ bool ml_wave(const DevCode &code, const DevCall &call, int shot, cons
                     const uint64_t q0 = pos + (uint64_t)lane * (uint64_t)R;
                     const int len = q0 >= NP ? 0 : (int)((NP - q0) < (uint64_t)R ? (NP - q0) : (uint64_t)R);
                     const PrefHit h = scan_prefix_w<1>(w, code, s, t0, q0, len, tam, L.P, L.sigP, L.llrP, L.ht, hmask);
+                    count_work(call.work, 0, walked(h, len, true));
                     const uint32_t key = (h.j >= 0) ? order_key(h.cost) : 0xFFFFFFFFu;
                     const uint32_t kmin = wave_min(key);
                     const uint64_t b = ballot(h.j >= 0 && key == kmin);

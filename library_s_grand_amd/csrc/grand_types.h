@@ -167,6 +167,11 @@ struct DevCall {
     uint32_t *task_arrays;            // [kTaskSlots][3][cap]: active list, signatures, sorted LLRs
     unsigned *task_counters;          // [0] open tasks, [1] waves finished with their own shots
     unsigned long long split_threshold;  // classes larger than this are shared (0 = never)
+    // optional work counts of the search kernel (null: not counted), summed over
+    // the launch: [0] (w-1)-prefix steps (one hash-table probe each, hit or
+    // not), [1] (w-2)-stem steps (one pair-table lookup each), [2] patterns
+    // tested one per lane (small classes and weight 1)
+    unsigned long long *work;
     // learned search order, tried before the plain search (lscores == null: off)
     const float *lscores;             // per shot: [lstride] scores, by slot of the first active list or by column
     int lstride;

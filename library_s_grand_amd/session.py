@@ -37,6 +37,7 @@ def bind_session(lib):
     lib.grand_session_timing_totals.argtypes = [_VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                                 ctypes.POINTER(ctypes.c_longlong)]
     lib.grand_session_timing_reset.argtypes = [_VP]
+    lib.grand_session_set_work_counters.argtypes = [_VP, _VP]
     lib.grand_session_ms_per_tick.argtypes = [_VP]
     lib.grand_session_ms_per_tick.restype = ctypes.c_double
     lib.grand_session_pattern_words.argtypes = [_VP]
@@ -142,6 +143,10 @@ class Session:
     def timing_reset(self):
         if self.lib.grand_session_timing_reset(self.h) != 0:
             raise RuntimeError("timing reset failed")
+
+    def set_work_counters(self, d_work):
+        """d_work: int64 CUDA tensor of 3 (zeroed by the caller), or None to stop counting."""
+        self.lib.grand_session_set_work_counters(self.h, _ptr(d_work))
 
     def footprint(self, shots):
         return self.lib.grand_session_footprint(self.h, shots)
