@@ -221,35 +221,51 @@ def setup_bb_server(code_sizes=(72, 144, 288), W=4, server=None, **kw):
     return srv, ids
 
 
-def measure(rate_hz=2000.0, n_requests=4000, seed=7, lanes_per_code=2, max_batch_shots=16384):
-    """Open-loop run of the mixed BB72/BB144/BB288 workload; returns the
-    summary dict (latency percentiles over requests, throughput)."""
-    reqs = mixed_bb_workload(n_requests, rate_hz, seed)
-    srv, ids = setup_bb_server(lanes_per_code=lanes_per_code, max_batch_shots=max_batch_shots)
-    try:
-        # warm each lane's kernels once
-        for cid, q in zip(ids, (72, 144, 288)):
-            srv.wait(srv.submit(cid, np.zeros((1, q), np.int32)))
-        before = srv.counters()
-        out, wall = serve_open_loop(srv, ids, reqs)
-        after = srv.counters()
-        fp = srv.footprint()
-    finally:
-        srv.close()
+def _summary(reqs, out, wall, rate_hz, before, after, fp, lanes_per_code):
     lat = np.array([o[1] for o in out]) * 1e3
     shots = sum(r.errors.shape[0] for r in reqs)
     batches = after["batches"] - before["batches"]
     return {
-        "workload": f"BB72/BB144/BB288 grand_decode_cc W=4, {n_requests} requests, Poisson {rate_hz:g}/s, "
+        "workload": f"BB72/BB144/BB288 grand_decode_cc W=4, {len(reqs)} requests, Poisson {rate_hz:g}/s, "
                     f"shots per request log-uniform [1,4096], p ~ U[0.002,0.008]",
-        "requests": n_requests, "shots": shots, "wall_s": wall,
+        "offered_requests_per_s": rate_hz, "requests": len(reqs), "shots": shots, "wall_s": wall,
         "offered_shots_per_s": shots / reqs[-1].t if reqs[-1].t > 0 else None,
-        "served_shots_per_s": shots / wall, "requests_per_s": n_requests / wall,
+        "served_shots_per_s": shots / wall, "requests_per_s": len(reqs) / wall,
         "p50_us_per_request": float(np.percentile(lat, 50)), "p99_us_per_request": float(np.percentile(lat, 99)),
         "max_us_per_request": float(lat.max()),
-        "batches": batches, "mean_requests_per_batch": n_requests / max(batches, 1),
+        "batches": batches, "mean_requests_per_batch": len(reqs) / max(batches, 1),
         "lanes_per_code": lanes_per_code, "hbm_footprint_bytes": fp,
     }
+
+
+def measure(rate_hz=2000.0, n_requests=4000, seed=7, lanes_per_code=2, max_batch_shots=16384):
+    """Open-loop run of the mixed BB72/BB144/BB288 workload; returns the
+    summary dict (latency percentiles over requests, throughput)."""
+    return sweep((rate_hz,), n_requests, seed, lanes_per_code, max_batch_shots)[0]
+
+
+def sweep(rates, n_requests=4000, seed=7, lanes_per_code=2, max_batch_shots=16384):
+    """The same request mix (one set of requests: codes, shot counts, errors)
+    offered at each Poisson rate in turn on one server; one summary per rate.
+    Past the rate the decoder sustains, served < offered and requests queue:
+    batches then hold several requests each and the latency grows."""
+    base = mixed_bb_workload(n_requests, 1.0, seed)
+    srv, ids = setup_bb_server(lanes_per_code=lanes_per_code, max_batch_shots=max_batch_shots)
+    out_rows = []
+    try:
+        # warm each lane's kernels once
+        for cid, q in zip(ids, (72, 144, 288)):
+            srv.wait(srv.submit(cid, np.zeros((1, q), np.int32)))
+        fp = srv.footprint()
+        for rate in rates:
+            reqs = [Request(r.t / rate, r.code, r.errors) for r in base]
+            before = srv.counters()
+            out, wall = serve_open_loop(srv, ids, reqs)
+            after = srv.counters()
+            out_rows.append(_summary(reqs, out, wall, rate, before, after, fp, lanes_per_code))
+    finally:
+        srv.close()
+    return out_rows
 
 
 def main():
@@ -258,8 +274,14 @@ def main():
     ap.add_argument("--requests", type=int, default=4000)
     ap.add_argument("--lanes", type=int, default=2)
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--sweep", default="", help="comma-separated request rates (per second) offered in turn")
     a = ap.parse_args()
-    print(json.dumps(measure(a.rate, a.requests, a.seed, a.lanes)))
+    if a.sweep:
+        rates = [float(x) for x in a.sweep.split(",") if x]
+        for row in sweep(rates, a.requests, a.seed, a.lanes):
+            print(json.dumps(row), flush=True)
+    else:
+        print(json.dumps(measure(a.rate, a.requests, a.seed, a.lanes)))
 
 
 if __name__ == "__main__":
