@@ -1,6 +1,6 @@
 // grand_server.cpp -- continuous-batching decode server over device sessions.
 //
-// One worker thread drives every lane of every registered code:
+// Every lane has a worker thread of its own:
 //
 //   lane = one GrandSession (its own workspace, reserved for a full batch)
 //        + one non-blocking stream + device buffers for inputs and results
@@ -8,10 +8,10 @@
 //
 // A free lane takes every pending request of its code (whole requests, in
 // arrival order, up to max_batch_shots shots), copies their rows into its pinned
-// staging, and queues H2D copy -> decode -> D2H copy -> event on its stream.  A
-// finished lane (event complete) scatters the results to the requests' buffers.
-// Lanes of different codes, and the lanes of one code, run concurrently, so new
-// arrivals never wait for a batch that is already running.
+// staging, queues H2D copy -> decode -> D2H copy -> event on its stream, waits
+// for the event and scatters the results to the requests' buffers.  Lanes of
+// different codes, and the lanes of one code, run concurrently (their host
+// copies too), so new arrivals never wait for a batch that is already running.
 //
 // Nothing on the request path allocates device memory: every workspace and
 // buffer is sized when the code is admitted, and admission is refused when the
@@ -104,6 +104,7 @@ struct Code {
     long long footprint = 0;
     std::vector<std::unique_ptr<Lane>> lanes;
     std::deque<ReqPtr> queue;     // guarded by the server mutex
+    std::condition_variable cv;   // submit / stop -> this code's lanes
 };
 
 }  // namespace
@@ -112,18 +113,17 @@ struct GrandServer {
     GrandServerConfig cfg{};
     int device = 0;
     std::mutex mu;
-    std::condition_variable work_cv;   // submit / stop -> worker
-    std::condition_variable done_cv;   // worker -> waiters
+    std::condition_variable done_cv;   // lanes -> waiters
     std::vector<std::unique_ptr<Code>> codes;
     std::unordered_map<long long, ReqPtr> reqs;
     long long next_ticket = 0;
     long long queued_shots = 0;
     long long used_bytes = 0;
     bool stop = false;
-    std::thread worker;
+    std::vector<std::thread> workers;  // one per lane
     std::atomic<long long> n_requests{0}, n_shots{0}, n_batches{0};
 
-    void run();
+    void lane_loop(Code *c, Lane *L);
     bool launch(Code &c, Lane &L);
     void finish(Code &c, Lane &L, bool failed);
 };
@@ -196,63 +196,33 @@ void GrandServer::finish(Code &c, Lane &L, bool failed) {
     L.busy = false;
 }
 
-void GrandServer::run() {
+// A lane's worker: takes every pending request of its code, decodes them as
+// one batch and scatters the results; sleeps while its code has nothing queued.
+void GrandServer::lane_loop(Code *c, Lane *L) {
     (void)hipSetDevice(device);
-    std::vector<Code *> snap;
     for (;;) {
-        bool progressed = false, any_busy = false;
         {
-            std::lock_guard<std::mutex> g(mu);
-            snap.clear();
-            for (auto &c : codes) snap.push_back(c.get());
+            std::unique_lock<std::mutex> lk(mu);
+            c->cv.wait(lk, [&] { return stop || !c->queue.empty(); });
+            if (c->queue.empty()) break;                 // stopping, nothing left for this code
+            const int ie = c->queue.front()->input_errors;
+            while (!c->queue.empty() && c->queue.front()->input_errors == ie &&
+                   L->batch_shots + c->queue.front()->shots <= cfg.max_batch_shots) {
+                ReqPtr r = c->queue.front();
+                c->queue.pop_front();
+                L->batch_shots += r->shots;
+                L->batch.push_back(std::move(r));
+            }
+            queued_shots -= L->batch_shots;
         }
-        // retire finished batches
-        for (Code *c : snap)
-            for (auto &lp : c->lanes) {
-                Lane &L = *lp;
-                if (!L.busy) continue;
-                const hipError_t q = hipEventQuery(L.done);
-                if (q == hipSuccess) { finish(*c, L, false); progressed = true; }
-                else if (q != hipErrorNotReady) { ok(q, "hipEventQuery"); finish(*c, L, true); progressed = true; }
-                else any_busy = true;
-            }
-        // fill free lanes with everything pending for their code
-        for (Code *c : snap)
-            for (auto &lp : c->lanes) {
-                Lane &L = *lp;
-                if (L.busy) continue;
-                {
-                    std::lock_guard<std::mutex> g(mu);
-                    if (c->queue.empty()) break;
-                    const int ie = c->queue.front()->input_errors;
-                    while (!c->queue.empty() && c->queue.front()->input_errors == ie &&
-                           L.batch_shots + c->queue.front()->shots <= cfg.max_batch_shots) {
-                        ReqPtr r = c->queue.front();
-                        c->queue.pop_front();
-                        L.batch_shots += r->shots;
-                        L.batch.push_back(std::move(r));
-                    }
-                    queued_shots -= L.batch_shots;
-                }
-                L.want_dec = false;
-                for (const ReqPtr &r : L.batch) L.want_dec = L.want_dec || r->decoded;
-                L.busy = true;
-                any_busy = true;
-                progressed = true;
-                ++n_batches;
-                if (!launch(*c, L)) {
-                    (void)hipStreamSynchronize(L.stream);
-                    finish(*c, L, true);
-                }
-            }
-        if (progressed) continue;
-        if (any_busy) { std::this_thread::yield(); continue; }
-        std::unique_lock<std::mutex> lk(mu);
-        bool pending = false;
-        for (auto &c : codes) pending = pending || !c->queue.empty();
-        if (pending) continue;
-        if (stop) break;
-        work_cv.wait(lk);
+        L->want_dec = false;
+        for (const ReqPtr &r : L->batch) L->want_dec = L->want_dec || r->decoded;
+        L->busy = true;
+        ++n_batches;
+        bool failed = !launch(*c, *L);
+        if (failed) (void)hipStreamSynchronize(L->stream);
+        else failed = !ok(hipEventSynchronize(L->done), "hipEventSynchronize");
+        finish(*c, *L, failed);
     }
 }
 
@@ -270,7 +240,6 @@ GrandServer *grand_server_create(const GrandServerConfig *cfg) {
         if (!ok(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo")) { delete s; return nullptr; }
         s->cfg.hbm_budget_bytes = (long long)(free_b / 2);
     }
-    s->worker = std::thread([s] { s->run(); });
     return s;
 }
 
@@ -324,7 +293,12 @@ int grand_server_add_code(GrandServer *srv, int kind, const int *H, const int *H
         srv->used_bytes -= c->footprint;
         return -1;
     }
+    Code *cp = c.get();
     srv->codes.push_back(std::move(c));
+    for (auto &L : cp->lanes) {
+        Lane *lp = L.get();
+        srv->workers.emplace_back([srv, cp, lp] { srv->lane_loop(cp, lp); });
+    }
     return (int)srv->codes.size() - 1;
 }
 
@@ -340,6 +314,7 @@ long long grand_server_submit(GrandServer *srv, int code_id, const int *input, i
                               const int *logicals, int num_shots, GrandDecoderResult *results, int *decoded) {
     if (!srv || !input || !results || num_shots <= 0) return -1;
     auto r = std::make_shared<Request>();
+    Code *cp = nullptr;
     r->code = code_id; r->input = input; r->input_errors = input_is_errors ? 1 : 0; r->logicals = logicals;
     r->shots = num_shots; r->results = results; r->decoded = decoded;
     r->t_submit = Clock::now();
@@ -359,8 +334,9 @@ long long grand_server_submit(GrandServer *srv, int code_id, const int *input, i
         srv->queued_shots += num_shots;
         srv->reqs[r->ticket] = r;
         c.queue.push_back(r);
+        cp = &c;
     }
-    srv->work_cv.notify_one();
+    cp->cv.notify_one();
     return r->ticket;
 }
 
@@ -398,9 +374,10 @@ void grand_server_destroy(GrandServer *srv) {
     {
         std::lock_guard<std::mutex> g(srv->mu);
         srv->stop = true;
+        for (auto &c : srv->codes) c->cv.notify_all();
     }
-    srv->work_cv.notify_all();
-    if (srv->worker.joinable()) srv->worker.join();
+    for (auto &t : srv->workers)
+        if (t.joinable()) t.join();
     delete srv;
 }
 

@@ -25,7 +25,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import GrandDecoderConfig, GrandDecoderResult, KIND_CC, load_library, make_config, results_to_numpy
+from . import GrandDecoderConfig, KIND_CC, RESULT_DTYPE, load_library, make_config
 from . import codes as C
 
 REQ_DONE, REQ_FAILED, REQ_PENDING = 0, -1, 1
@@ -48,8 +48,8 @@ def bind_server(lib):
     lib.grand_server_footprint.restype = ctypes.c_longlong
     lib.grand_server_footprint.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.grand_server_submit.restype = ctypes.c_longlong
-    lib.grand_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_int, IP, ctypes.c_int, IP, ctypes.c_int,
-                                        ctypes.POINTER(GrandDecoderResult), IP]
+    lib.grand_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
     lib.grand_server_poll.restype = ctypes.c_int
     lib.grand_server_poll.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.POINTER(ctypes.c_double)]
     lib.grand_server_wait.restype = ctypes.c_int
@@ -104,10 +104,11 @@ class Server:
         rows = np.ascontiguousarray(rows, dtype=np.int32)
         lg = None if logicals is None else np.ascontiguousarray(logicals, dtype=np.int32)
         shots = rows.shape[0]
-        res = (GrandDecoderResult * shots)()
+        res = np.empty(shots, dtype=RESULT_DTYPE)          # the native records, written in place
         dec = np.zeros((shots, self._codes[code_id]["m"]), np.int32) if want_decoded else None
-        t = self.lib.grand_server_submit(self.h, code_id, _ip(rows), int(bool(input_is_errors)), _ip(lg), shots,
-                                         res, _ip(dec))
+        t = self.lib.grand_server_submit(self.h, code_id, rows.ctypes.data, int(bool(input_is_errors)),
+                                         None if lg is None else lg.ctypes.data, shots, res.ctypes.data,
+                                         None if dec is None else dec.ctypes.data)
         if t == -2:
             raise BlockingIOError("server queue full")
         if t < 0:
@@ -122,7 +123,7 @@ class Server:
         rows, lg, res, dec = self._live.pop(ticket)
         if st != REQ_DONE:
             raise RuntimeError(f"request {ticket} failed")
-        return results_to_numpy(res), dec, lat.value
+        return res, dec, lat.value
 
     def counters(self):
         a, b, c = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
