@@ -132,7 +132,9 @@ typedef struct {
     int split_threshold; /* weight classes larger than this are shared across waves; 0 = autotuned, -1 = never */
     int triage_search;   /* fused engine: shots whose first iteration is small are searched one per lane in the
                             triage kernel; 0 = on where it applies, -1 = off (every shot goes to the search kernel) */
-    int reserved[3];
+    int pair_table_kb;   /* first-hit kinds: LDS per search wave for the pair table of deep classes; 0 = autotuned,
+                            -1 = no pair table, k > 0 = at most k KiB (never more than the wave's share of the CU) */
+    int reserved[2];
 } GrandDecoderExConfig;
 
 int grand_decode_cc_ex(const int *Hx, const int *Hz, int n, int m, int *errors, int *syndromes, int num_shots,

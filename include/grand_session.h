@@ -165,6 +165,16 @@ int grand_device_info(int *cus, int *clock_khz, int *wall_clock_khz);
 void grand_set_tuning(int waves_per_cu, int claim_batch, int rmax, int split_threshold);
 void grand_get_tuning(int *waves_per_cu, int *claim_batch, int *rmax, int *split_threshold);
 
+/* LDS packing of the search kernel (first-hit kinds), process-wide like the
+ * knobs above: the pair table of deep weight classes takes the LDS a wave's
+ * share of the CU has left after its hash table, bitmap, code and claimed
+ * shots.  kb = 0 gives it all of that (built-in default), kb > 0 at most kb
+ * KiB, kb < 0 none (deep classes are then walked prefix by prefix); classes
+ * whose pairs do not fit are walked too, so every value decodes the same.
+ * grand_set_tuning with all four values negative also restores 0. */
+void grand_set_pair_table_kb(int kb);
+int grand_get_pair_table_kb(void);
+
 /* Degeneracy words of a CSS pair (host only, no GPU needed): H is the matrix
  * the syndromes come from and other the one whose row space decides
  * `degenerate` (n x m, row-major int 0/1 each).  For v in ker(H), v is in

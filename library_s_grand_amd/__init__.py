@@ -64,7 +64,8 @@ class GrandDecoderExConfig(ctypes.Structure):
         ("claim_batch", ctypes.c_int),
         ("split_threshold", ctypes.c_int),
         ("triage_search", ctypes.c_int),
-        ("reserved", ctypes.c_int * 3),
+        ("pair_table_kb", ctypes.c_int),
+        ("reserved", ctypes.c_int * 2),
     ]
 
 
@@ -141,10 +142,10 @@ def make_config(max_error_weight=4, threads_per_block=64, use_gpu_sum=1, max_siz
 
 
 def make_ex(engine=GRAND_ENGINE_FUSED, waves_per_cu=0, claim_batch=0, split_threshold=0,
-            triage_search=0) -> GrandDecoderExConfig:
+            triage_search=0, pair_table_kb=0) -> GrandDecoderExConfig:
     ex = GrandDecoderExConfig()
     ex.engine, ex.waves_per_cu, ex.claim_batch, ex.split_threshold = engine, waves_per_cu, claim_batch, split_threshold
-    ex.triage_search = triage_search
+    ex.triage_search, ex.pair_table_kb = triage_search, pair_table_kb
     return ex
 
 

@@ -10,6 +10,10 @@ not override them in its GrandDecoderExConfig):
   rmax              prefixes per lane per chunk (chunk = 64 x rmax prefixes)
   split_threshold   weight classes with more prefixes than this are shared
                     between waves through the task table
+  pair_table_kb     LDS packing (first-hit kinds): KiB of a wave's LDS share
+                    given to the pair table of deep classes (0 = all that is
+                    left, -1 = none); a smaller table is cheaper to build per
+                    class and leaves more classes to the prefix walk
 
 The tuner measures the search kernel with HIP events on the caller's workload
 (device-resident inputs, a session with timing enabled) and walks the knobs one
@@ -31,15 +35,18 @@ SPACE = {
     "claim_batch": (1, 2, 4, 8),
     "rmax": (8, 16, 32, 64),
     "split_threshold": (256, 1024, 2048, 8192, 32768),
+    "pair_table_kb": (0, 8, 16, -1),
 }
-KEYS = ("waves_per_cu", "claim_batch", "rmax", "split_threshold")
+KEYS = ("waves_per_cu", "claim_batch", "rmax", "split_threshold", "pair_table_kb")
 
 
 def get(lib):
     bind_session(lib)
-    v = [ctypes.c_int() for _ in KEYS]
+    v = [ctypes.c_int() for _ in KEYS[:4]]
     lib.grand_get_tuning(*[ctypes.byref(x) for x in v])
-    return {k: x.value for k, x in zip(KEYS, v)}
+    cfg = {k: x.value for k, x in zip(KEYS, v)}
+    cfg["pair_table_kb"] = int(lib.grand_get_pair_table_kb())
+    return cfg
 
 
 def reset(lib):
@@ -50,7 +57,9 @@ def reset(lib):
 
 def apply(lib, cfg):
     bind_session(lib)
-    lib.grand_set_tuning(*[int(cfg[k]) for k in KEYS])
+    lib.grand_set_tuning(*[int(cfg[k]) for k in KEYS[:4]])
+    if "pair_table_kb" in cfg:
+        lib.grand_set_pair_table_kb(int(cfg["pair_table_kb"]))
 
 
 def measure(run, reps=5):
@@ -70,7 +79,7 @@ def tune(lib, run, reps=5, margin=0.02, max_rounds=3, space=SPACE, log=None):
     for _ in range(max_rounds):
         improved = False
         for key in KEYS:
-            for val in space[key]:
+            for val in space.get(key, ()):
                 if val == best[key]:
                     continue
                 cand = dict(best, **{key: val})
@@ -89,7 +98,7 @@ def tune(lib, run, reps=5, margin=0.02, max_rounds=3, space=SPACE, log=None):
 
 def save(path, cfg):
     with open(path, "w", encoding="utf-8") as f:
-        json.dump({k: int(cfg[k]) for k in KEYS}, f)
+        json.dump({k: int(cfg[k]) for k in KEYS if k in cfg}, f)
 
 
 def load(path):

@@ -578,9 +578,11 @@ int kind_search(int kind) {
 // untuned: 292 M shots/s with the former single set 4, 2, 32, 4096, 577 M with
 // this one).  ML: 8 waves per CU (the DEM ML workload, tuned: 10.1 M shots/s,
 // against 5.2 M with the former set and 2.8 M with the first-hit set).
-struct Tuning { int waves_per_cu; int claim_batch; int rmax; int split_threshold; };
-constexpr Tuning kTuningFirstHit{2, 2, 64, 32768};
-constexpr Tuning kTuningMl{8, 2, 64, 32768};
+// The pair-table LDS (pair_table_kb: 0 = the wave's whole remaining share,
+// -1 = none, k = at most k KiB) applies to the first-hit kinds only.
+struct Tuning { int waves_per_cu; int claim_batch; int rmax; int split_threshold; int pair_table_kb; };
+constexpr Tuning kTuningFirstHit{2, 2, 64, 32768, 0};
+constexpr Tuning kTuningMl{8, 2, 64, 32768, 0};
 Tuning g_tuning = kTuningFirstHit;
 Tuning g_tuning_ml = kTuningMl;
 const Tuning &tuning_for(int search) { return search == kML ? g_tuning_ml : g_tuning; }
@@ -885,8 +887,10 @@ int decode_view(GrandSession *s, const WsView &w, const int *d_input, int input_
                             4 * ((size_t)kPairBitmapWords + kPairHeadWords);
         const size_t budget = std::min<size_t>(65536, kLdsPerCu / (size_t)std::max(wpc, 1));
         const long long all = (long long)c.cap * (c.cap - 1) / 2;
-        if (budget > base) {
-            const long long room = (long long)((budget - base) / 8);
+        const int kb = (s->ex.pair_table_kb != 0) ? s->ex.pair_table_kb : tuning_for(s->search).pair_table_kb;
+        if (kb >= 0 && budget > base) {
+            long long room = (long long)((budget - base) / 8);
+            if (kb > 0) room = std::min(room, (long long)kb * 1024 / 8);
             const long long pc = std::min(room, std::min(all, (long long)kPairNone));
             if (pc >= 256) c.pair_cap = (int)pc;
         }
@@ -1420,6 +1424,13 @@ void grand_set_tuning(int waves_per_cu, int claim_batch, int rmax, int split_thr
         if (split_threshold > 0) t->split_threshold = split_threshold;
     }
 }
+
+void grand_set_pair_table_kb(int kb) {
+    g_tuning.pair_table_kb = kb;
+    g_tuning_ml.pair_table_kb = kb;
+}
+
+int grand_get_pair_table_kb(void) { return g_tuning.pair_table_kb; }
 
 void grand_get_tuning(int *waves_per_cu, int *claim_batch, int *rmax, int *split_threshold) {
     if (waves_per_cu) *waves_per_cu = g_tuning.waves_per_cu;

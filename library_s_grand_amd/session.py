@@ -48,6 +48,9 @@ def bind_session(lib):
     lib.grand_device_info.argtypes = [ctypes.POINTER(ctypes.c_int)] * 3
     lib.grand_set_tuning.argtypes = [ctypes.c_int] * 4
     lib.grand_get_tuning.argtypes = [ctypes.POINTER(ctypes.c_int)] * 4
+    lib.grand_set_pair_table_kb.argtypes = [ctypes.c_int]
+    lib.grand_get_pair_table_kb.argtypes = []
+    lib.grand_get_pair_table_kb.restype = ctypes.c_int
     lib._grand_session_bound = True
     return lib
 

@@ -74,19 +74,26 @@ def test_product_exports_every_declared_function():
 
 def test_tuning_knobs_set_get_and_reset():
     """Host-side state only (no GPU): a set replaces the knobs, values <= 0 keep them, all four negative
-    restores the built-in first-hit defaults (2 waves per CU, claim batch 2, 64 prefixes, split 32768)."""
+    restores the built-in first-hit defaults (2 waves per CU, claim batch 2, 64 prefixes, split 32768,
+    the pair table filling the wave's LDS share)."""
     from library_s_grand_amd import autotune
     lib = ctypes.CDLL(G.DEFAULT_LIB)
     try:
-        autotune.apply(lib, {"waves_per_cu": 3, "claim_batch": 4, "rmax": 16, "split_threshold": 100})
-        assert autotune.get(lib) == {"waves_per_cu": 3, "claim_batch": 4, "rmax": 16, "split_threshold": 100}
+        autotune.apply(lib, {"waves_per_cu": 3, "claim_batch": 4, "rmax": 16, "split_threshold": 100,
+                             "pair_table_kb": 16})
+        assert autotune.get(lib) == {"waves_per_cu": 3, "claim_batch": 4, "rmax": 16, "split_threshold": 100,
+                                     "pair_table_kb": 16}
         lib.grand_set_tuning(0, 0, 8, 0)
-        assert autotune.get(lib) == {"waves_per_cu": 3, "claim_batch": 4, "rmax": 8, "split_threshold": 100}
+        assert autotune.get(lib) == {"waves_per_cu": 3, "claim_batch": 4, "rmax": 8, "split_threshold": 100,
+                                     "pair_table_kb": 16}
         lib.grand_set_tuning(0, 99, 0, 0)                     # claim batch is capped at 8
         assert autotune.get(lib)["claim_batch"] == 8
+        lib.grand_set_pair_table_kb(-1)                      # no pair table
+        assert autotune.get(lib)["pair_table_kb"] == -1
     finally:
         autotune.reset(lib)
-    assert autotune.get(lib) == {"waves_per_cu": 2, "claim_batch": 2, "rmax": 64, "split_threshold": 32768}
+    assert autotune.get(lib) == {"waves_per_cu": 2, "claim_batch": 2, "rmax": 64, "split_threshold": 32768,
+                                 "pair_table_kb": 0}
 
 
 def test_every_exported_function_is_declared():

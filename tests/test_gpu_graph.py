@@ -90,7 +90,8 @@ def test_autotuner_applies_best_and_keeps_results(tmp_path):
         sess.decode(d_err, True, d_res)
         return sess.last_timing()
 
-    space = {"waves_per_cu": (4, 8), "claim_batch": (1, 4), "rmax": (16, 32), "split_threshold": (1024, 2048)}
+    space = {"waves_per_cu": (4, 8), "claim_batch": (1, 4), "rmax": (16, 32), "split_threshold": (1024, 2048),
+             "pair_table_kb": (8, -1)}
     try:
         best, ms, table = autotune.tune(lib, run, reps=2, space=space, max_rounds=1)
         assert autotune.get(lib) == best and ms > 0 and len(table) >= 2

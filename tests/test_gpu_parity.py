@@ -222,6 +222,17 @@ def test_pair_table_deep_classes(oracle, product, tmp_path, wpc):
         print(what, "wpc", wpc, "max count", int(o[1]["total_tested_combinations"].max()), flush=True)
 
 
+@pytest.mark.parametrize("kb", [4, 16, -1])
+def test_pair_table_lds_knob(oracle, product, tmp_path, kb):
+    """The LDS-packing knob: a pair table capped at 4 KiB (512 pairs: the deep classes of long lists
+    fall back to the prefix walk), at 16 KiB, and none at all decode exactly as the oracle."""
+    for what, (runner, args, kw, o) in _pair_cases(oracle, tmp_path).items():
+        for split in (256, -1):
+            p = runner(product, *args, ex=G.make_ex(split_threshold=split, pair_table_kb=kb), **kw)
+            T.assert_same(o, p, f"{what} pair_table_kb={kb} split={split}")
+        print(what, "pair_table_kb", kb, flush=True)
+
+
 def test_deep_shots_in_small_batches(oracle, product, tmp_path):
     """Shots of the BB144 prob_cc bench workload whose second active list has 16 or more columns (the
     third iteration's shared run is then published before the second is searched: joined when the
