@@ -31,7 +31,7 @@ import numpy as np
 from .session import bind_session
 
 SPACE = {
-    "waves_per_cu": (2, 4, 8, 16),
+    "waves_per_cu": (1, 2, 4, 8, 16),
     "claim_batch": (1, 2, 4, 8),
     "rmax": (8, 16, 32, 64),
     "split_threshold": (256, 1024, 2048, 8192, 32768),

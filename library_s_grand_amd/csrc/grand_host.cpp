@@ -373,16 +373,30 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
     // member, 16-byte aligned: the column and its D words, padded to 4 words
     // (D <= 3) or 8.  A group holds the nonempty columns whose lowest check is
     // that check: a column whose support lies inside supp(s) is in the group of
-    // one of s's checks, so the lane search tests only those groups.
+    // one of s's checks, so the lane search tests only those groups.  For the
+    // CSS family, when every nonempty column has weight >= 2, second groups
+    // follow (starts [n + 1] at t_gs2, records at t_rec2): the columns by their
+    // second-lowest check.  A column with at most one check outside supp(s)
+    // (the second iteration's active set, score >= max_soma - 1) has its
+    // lowest or its second-lowest check in supp(s), so the lane search of the
+    // second iteration tests the first and second groups of s's checks.
     std::vector<uint32_t> timg;
-    int t_sig, t_dg, t_gs, t_rec, t_recw;
+    int t_sig, t_dg, t_gs, t_rec, t_recw, t_gs2 = 0, t_rec2 = 0;
     {
-        std::vector<int> low(m, -1), gs(n + 1, 0);
+        std::vector<int> low(m, -1), low2(m, -1), gs(n + 1, 0), gs2(n + 1, 0);
         for (int c = 0; c < m; ++c)
-            for (int d = 0; d < D && low[c] < 0; ++d)
-                if (Hcm[(size_t)c * D + d]) low[c] = d * 32 + __builtin_ctz(Hcm[(size_t)c * D + d]);
+            for (int r = 0; r < n && low2[c] < 0; ++r)
+                if ((Hcm[(size_t)c * D + (r >> 5)] >> (r & 31)) & 1u) {
+                    if (low[c] < 0) low[c] = r;
+                    else low2[c] = r;
+                }
+        bool second = (family == kCC);
+        for (int c = 0; c < m; ++c)
+            if (colw[c] == 1) second = false;          // active at every threshold below max_soma
         for (int c = 0; c < m; ++c) if (low[c] >= 0) ++gs[low[c] + 1];
         for (int r = 0; r < n; ++r) gs[r + 1] += gs[r];
+        for (int c = 0; c < m; ++c) if (low2[c] >= 0) ++gs2[low2[c] + 1];
+        for (int r = 0; r < n; ++r) gs2[r + 1] += gs2[r];
         t_recw = (D + 1 <= 4) ? 4 : 8 * ((D + 8) / 8);
         timg.assign(Hcw.begin(), Hcw.end());
         t_sig = (int)timg.size();
@@ -400,6 +414,20 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
             uint32_t *rec = timg.data() + t_rec + (size_t)fill[low[c]]++ * t_recw;
             rec[0] = (uint32_t)c;
             for (int d = 0; d < D; ++d) rec[1 + d] = Hcm[(size_t)c * D + d];
+        }
+        if (second) {
+            t_gs2 = (int)timg.size();
+            timg.insert(timg.end(), gs2.begin(), gs2.end());
+            timg.resize((timg.size() + 3) & ~(size_t)3, 0u);
+            t_rec2 = (int)timg.size();
+            timg.resize(timg.size() + (size_t)gs2[n] * t_recw, 0u);
+            std::vector<int> fill2(gs2.begin(), gs2.end() - 1);
+            for (int c = 0; c < m; ++c) {
+                if (low2[c] < 0) continue;
+                uint32_t *rec = timg.data() + t_rec2 + (size_t)fill2[low2[c]]++ * t_recw;
+                rec[0] = (uint32_t)c;
+                for (int d = 0; d < D; ++d) rec[1 + d] = Hcm[(size_t)c * D + d];
+            }
         }
     }
 
@@ -443,6 +471,7 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
     dc.timg = (const uint32_t *)(base + segs[iTimg].off);
     dc.timg_words = (int)timg.size();
     dc.t_sig = t_sig; dc.t_dg = t_dg; dc.t_gs = t_gs; dc.t_rec = t_rec; dc.t_recw = t_recw;
+    dc.t_gs2 = t_gs2; dc.t_rec2 = t_rec2;
 
     dc.llr = (const float *)(base + segs[iLlr].off);
     dc.nCr = rt().nCr.as<unsigned long long>();
@@ -573,15 +602,19 @@ int kind_search(int kind) {
 
 // Tuning knobs: built-in defaults per search kind, replaced for every kind by
 // grand_set_tuning (the autotuner in the Python package), overridable per
-// session.  First-hit kinds: 2 waves per CU, so a wave's share of the CU's LDS
-// holds the pair table of lists up to 81 columns (BB144 prob_cc, 100k shots,
-// untuned: 292 M shots/s with the former single set 4, 2, 32, 4096, 577 M with
-// this one).  ML: 8 waves per CU (the DEM ML workload, tuned: 10.1 M shots/s,
-// against 5.2 M with the former set and 2.8 M with the first-hit set).
+// session.  First-hit kinds: 1 wave per CU.  A wave's LDS share holds the
+// pair table of lists up to 81 columns, and with the first two iterations of
+// most shots searched by lane in the triage kernel, the search kernel's batch
+// is a few hundred shots: one wave per CU decodes them as fast as two and
+// leaves the CUs' other SIMDs and LDS to the next batch's triage on another
+// stream (BB144 prob_cc, 100k shots, per decode: 146.6 us on one stream and
+// 84.7 us on two, against 150.9 and 120.0 us at 2 waves per CU).  ML: 8 waves
+// per CU (the DEM ML workload, tuned: 10.1 M shots/s, against 5.2 M with the
+// former set and 2.8 M with the first-hit set).
 // The pair-table LDS (pair_table_kb: 0 = the wave's whole remaining share,
 // -1 = none, k = at most k KiB) applies to the first-hit kinds only.
 struct Tuning { int waves_per_cu; int claim_batch; int rmax; int split_threshold; int pair_table_kb; };
-constexpr Tuning kTuningFirstHit{2, 2, 64, 32768, 0};
+constexpr Tuning kTuningFirstHit{1, 2, 64, 32768, 0};
 constexpr Tuning kTuningMl{8, 2, 64, 32768, 0};
 Tuning g_tuning = kTuningFirstHit;
 Tuning g_tuning_ml = kTuningMl;
@@ -1323,9 +1356,12 @@ int grand_session_reserve(GrandSession *s, int num_shots) {
 int grand_session_enable_timing(GrandSession *s, int on) {
     if (!s) return -1;
     if (on && s->ev.empty()) {
+        // timing-only events: no system-scope fence (cache writeback and
+        // invalidate) when they complete, which otherwise stalls the stream
+        // between the triage and search kernels
         s->ev.assign(3 * GrandSession::kTimingSlots, nullptr);
         for (hipEvent_t &e : s->ev)
-            if (hipEventCreate(&e) != hipSuccess) return -1;
+            if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return -1;
     }
     s->timing = on != 0;
     return 0;

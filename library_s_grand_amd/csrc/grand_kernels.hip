@@ -2172,17 +2172,19 @@ __device__ __forceinline__ void write_zero_result(GrandDecoderResult *dst, int o
 
 // ------------------------------------------------------------ lane search
 // A shot searched by its own lane in the triage kernel: the first iteration of
-// decode_shot_wave (threshold max_soma) when its active list has at most
-// kLaneMax positions and the leading weight classes of its list are of weight
-// <= 3 with at most kLaneBudget patterns in all.  At threshold max_soma a
-// column is active iff its support lies inside supp(s) (score = |supp(h_c) &
-// supp(s)| + max_soma - colw[c], grand_decoder.cu:406-416, :495-504).  Classes
-// are enumerated in the order of the wave path (lexicographic within a class,
+// decode_shot_wave (threshold max_soma), and for the CSS family the second
+// (threshold max_soma - 1) after a first searched whole without a hit, when
+// the iteration's active list has at most kLaneMax positions and the leading
+// weight classes of its list are of weight <= 3 with at most kLaneBudget
+// patterns in all.  At threshold max_soma - j a column is active iff at most
+// j of its checks lie outside supp(s) (score = |supp(h_c) & supp(s)| +
+// max_soma - colw[c], grand_decoder.cu:406-416, :495-504).  Classes are
+// enumerated in the order of the wave path (lexicographic within a class,
 // classes in list order), each candidate filtered by its signature and checked
-// exactly, so the hit, its count and its columns are the wave path's.  Returns
-// false, having written nothing, when the shot needs the search kernel (list
-// too long, a class too large, a cap of the DEM family, or no match in the
-// first iteration).
+// exactly, so the hit, its count, its iteration and its columns are the wave
+// path's.  Returns false, having written nothing, when the shot needs the
+// search kernel (list too long, a class too large, a cap of the DEM family, or
+// no match in the iterations searched here).
 // Weight list of the first iteration for an active list of `tam` positions:
 // the weights of decode_shot_wave's list in order, 4 bits each, first in the
 // low bits, without the slots whose class is empty (weight above tam); 0 ends
@@ -2222,14 +2224,17 @@ struct TriTables {
     U dg;
     U gs;
     U rec;
+    U gs2;                             // second groups (second is true)
+    U rec2;
     bool has_dg;
+    bool second;
 };
 using TriLds = TriTables<const uint32_t LDS *>;
 using TriGlobal = TriTables<const uint32_t *>;
 template <class U>
 __device__ __forceinline__ TriTables<U> tri_tables(const DevCode &code, U img) {
     return TriTables<U>{img, img + code.t_sig, img + code.t_dg, img + code.t_gs, img + code.t_rec,
-                        code.dg != nullptr};
+                        img + code.t_gs2, img + code.t_rec2, code.dg != nullptr, code.t_gs2 > 0};
 }
 
 template <int D, class T>
@@ -2253,178 +2258,199 @@ __device__ __forceinline__ bool lane_search(const DevCode &code, const DevCall &
                             unsigned long long t0) {
     const int lane = lane_id();
     const int m = code.m;
-    // Active columns: nonempty, support inside supp(s).  Such a column holds
-    // its lowest check, so only the groups of s's checks are tested; hits set
-    // bits of the lane's bitmask AB[wd * 64 + lane], read back in column order
-    // (the order of the wave path's active list), at most kLaneMax + 1 of them.
     const int M32 = code.M32;
-    for (int wd = 0; wd < M32; ++wd) AB[wd * 64 + lane] = 0u;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        uint32_t bs = s[d];
-        while (bs) {
-            const int r = 32 * d + __ffs(bs) - 1;
-            bs &= bs - 1u;
-            const int k1 = (int)tb.gs[r + 1];
-            for (int k = (int)tb.gs[r]; k < k1; ++k) {
-                // the record: the column, then its words (read together)
-                constexpr int RW = (D + 1 <= 4) ? 4 : 8;
-                const int c = (int)tb.rec[k * RW];
-                uint32_t out = 0u;
-#pragma unroll
-                for (int e = 0; e < D; ++e) out |= tb.rec[k * RW + 1 + e] & ~s[e];
-                if (out == 0u)
-                    __hip_atomic_fetch_or(&AB[(c >> 5) * 64 + lane], 1u << (c & 31), __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
-        }
-    }
-    int tam = 0;
-    for (int wd = 0; wd < M32 && tam <= kLaneMax; ++wd) {
-        uint32_t w = AB[wd * 64 + lane];
-        while (w && tam <= kLaneMax) {
-            if (tam < kLaneMax) LA[tam * 64 + lane] = 32 * wd + __ffs(w) - 1;
-            w &= w - 1u;
-            ++tam;
-        }
-    }
-    const unsigned long long ta = now_ticks();     // the active list is built (time_sum_ms)
-    if (tam == 0 || tam > kLaneMax || tam > call.cap) return false;
-    if (call.family == kCLN && tam > call.max_size) return false;   // tam falls back to 0: second iteration
-    const int br = branch_of(call, tam);
-    if (br == 0) return false;
-    // the weight list of decode_shot_wave for this active-list size
-    const uint64_t wl = WL[tam];
     auto nc2 = [](int n) { return n * (n - 1) / 2; };
     auto nc3 = [](int n) { return n * (n - 1) * (n - 2) / 6; };
-    // the weights the walk reaches before it would give up (a class heavier
-    // than 3, or the budget exceeded)
-    unsigned need = 0u;
-    {
-        unsigned cum = 0;
+    // Iterations of decode_shot_wave searched here: the first, and for the CSS
+    // family with second groups the second, when the first is searched whole
+    // without a hit (or has no list).  done / wdone: patterns (weighted
+    // patterns) of the iterations searched without a hit; hib: the hibrido
+    // code as decode_shot_wave updates it.
+    const int itmax = (call.family == kCC && tb.second) ? 2 : 1;
+    unsigned long long ta = 0;
+    unsigned done = 0u, wdone = 0u, cum = 0u, wsum = 0u, hidx = 0u;
+    int hib = -2, it = 1, tam = 0, hw = 0, ha = 0, hb = 0, hc = 0;
+    for (; it <= itmax; ++it) {
+        // Active columns at threshold max_soma - (it - 1): nonempty, at most
+        // it - 1 of their checks outside supp(s).  Such a column holds its
+        // lowest check (it = 1) or its lowest or second-lowest check (it = 2),
+        // so only the groups (and second groups) of s's checks are tested;
+        // hits set bits of the lane's bitmask AB[wd * 64 + lane], read back in
+        // column order (the order of the wave path's active list), at most
+        // kLaneMax + 1 of them.
+        for (int wd = 0; wd < M32; ++wd) AB[wd * 64 + lane] = 0u;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            uint32_t bs = s[d];
+            while (bs) {
+                const int r = 32 * d + __ffs(bs) - 1;
+                bs &= bs - 1u;
+                for (int g = 0; g < it; ++g) {
+                    const auto gsp = g ? tb.gs2 : tb.gs;
+                    const auto rp = g ? tb.rec2 : tb.rec;
+                    const int k1 = (int)gsp[r + 1];
+                    for (int k = (int)gsp[r]; k < k1; ++k) {
+                        // the record: the column, then its words (read together)
+                        constexpr int RW = (D + 1 <= 4) ? 4 : 8;
+                        const int c = (int)rp[k * RW];
+                        int nout = 0;
+#pragma unroll
+                        for (int e = 0; e < D; ++e) nout += __popc(rp[k * RW + 1 + e] & ~s[e]);
+                        if (nout < it)
+                            __hip_atomic_fetch_or(&AB[(c >> 5) * 64 + lane], 1u << (c & 31), __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                }
+            }
+        }
+        tam = 0;
+        for (int wd = 0; wd < M32 && tam <= kLaneMax; ++wd) {
+            uint32_t w = AB[wd * 64 + lane];
+            while (w && tam <= kLaneMax) {
+                if (tam < kLaneMax) LA[tam * 64 + lane] = 32 * wd + __ffs(w) - 1;
+                w &= w - 1u;
+                ++tam;
+            }
+        }
+        if (it == 1) ta = now_ticks();             // the first active list is built (time_sum_ms)
+        if (tam == 0) continue;                     // no list: the next iteration
+        if (tam > kLaneMax || tam > call.cap) return false;
+        if (call.family == kCLN && tam > call.max_size) return false;   // tam falls back to 0: second iteration
+        const int br = branch_of(call, tam);
+        if (br == 0) continue;                      // not searched in this mode: the next iteration
+        hib = (br > 0) ? (hib == -1 ? 1 : 0) : -1;
+        // the weight list of decode_shot_wave for this active-list size
+        const uint64_t wl = WL[tam];
+        // the weights the walk reaches before it would give up (a class heavier
+        // than 3, or the budget exceeded); `whole`: the walk reaches every class of
+        // the list, of tot patterns (wtot weighted by w + 1) in all
+        unsigned need = 0u, tot = 0u, wtot = 0u;
+        bool whole = true;
+        for (uint64_t l = wl; l; l >>= 4) {
+            const int w = (int)(l & 15u);
+            const unsigned N = (w > 3) ? 0u : (unsigned)((w == 1) ? tam : (w == 2) ? nc2(tam) : nc3(tam));
+            if (w > 3 || tot + N > (unsigned)kLaneBudget) { whole = false; break; }
+            need |= 1u << w;
+            tot += N;
+            wtot += N * (unsigned)(w + 1);
+        }
+        // First signature match of each needed class, lexicographic order, from the
+        // signatures in registers: the loops run backwards and the last match
+        // written is the first in order.  Packed as a | b << 4 | c << 8, -1 = none.
+        // The signatures of the list, looked up together, in registers and in LS.
+        // tmax: the longest list among the lanes searching (bounds the loops).
+        uint32_t S[kLaneMax];
+#pragma unroll
+        for (int i = 0; i < kLaneMax; ++i) S[i] = (i < tam) ? tb.sig[LA[i * 64 + lane]] : 0u;
+#pragma unroll
+        for (int i = 0; i < kLaneMax; ++i)
+            if (i < tam) LS[i * 64 + lane] = S[i];
+        int tmax = 1;
+        for (int t = kLaneMax; t > 1; --t)
+            if (ballot(tam >= t)) { tmax = t; break; }
+        int f1 = -1, f2 = -1, f3 = -1;
+        if (ballot(need & 2u)) {
+#pragma unroll
+            for (int x = kLaneMax - 1; x >= 0; --x)
+                if (x < tmax && x < tam && S[x] == sg) f1 = x;
+        }
+        // (the outer loops read S[x] from LDS: x varies at run time)
+        if (ballot(need & 4u)) {
+#pragma unroll 1
+            for (int x = tmax - 2; x >= 0; --x) {
+                const uint32_t t = sg ^ LS[x * 64 + lane];
+#pragma unroll
+                for (int y = kLaneMax - 1; y >= 1; --y)
+                    if (y > x && y < tmax && y < tam && S[y] == t) f2 = x | (y << 4);
+            }
+        }
+        if (ballot(need & 8u)) {
+#pragma unroll 1
+            for (int x = tmax - 3; x >= 0; --x) {
+                const uint32_t tx = sg ^ LS[x * 64 + lane];
+#pragma unroll
+                for (int y = kLaneMax - 2; y >= 1; --y) {
+                    if (y <= x || y >= tmax - 1) continue;
+                    const uint32_t t = tx ^ S[y];
+#pragma unroll
+                    for (int z = kLaneMax - 1; z > y; --z)
+                        if (z < tmax && z < tam && S[z] == t) f3 = x | (y << 4) | (z << 8);
+                }
+            }
+        }
+        // walk the classes: the first class with a signature match holds the
+        // candidate; its rank within the class is in closed form
+        cum = 0; wsum = 0;
         for (uint64_t l = wl; l; l >>= 4) {
             const int w = (int)(l & 15u);
             if (w > 3) break;
             const unsigned N = (unsigned)((w == 1) ? tam : (w == 2) ? nc2(tam) : nc3(tam));
             if (cum + N > (unsigned)kLaneBudget) break;
-            need |= 1u << w;
-            cum += N;
-        }
-    }
-    // First signature match of each needed class, lexicographic order, from the
-    // signatures in registers: the loops run backwards and the last match
-    // written is the first in order.  Packed as a | b << 4 | c << 8, -1 = none.
-    // The signatures of the list, looked up together, in registers and in LS.
-    // tmax: the longest list among the lanes searching (bounds the loops).
-    uint32_t S[kLaneMax];
-#pragma unroll
-    for (int i = 0; i < kLaneMax; ++i) S[i] = (i < tam) ? tb.sig[LA[i * 64 + lane]] : 0u;
-#pragma unroll
-    for (int i = 0; i < kLaneMax; ++i)
-        if (i < tam) LS[i * 64 + lane] = S[i];
-    int tmax = 1;
-    for (int t = kLaneMax; t > 1; --t)
-        if (ballot(tam >= t)) { tmax = t; break; }
-    int f1 = -1, f2 = -1, f3 = -1;
-    if (ballot(need & 2u)) {
-#pragma unroll
-        for (int x = kLaneMax - 1; x >= 0; --x)
-            if (x < tmax && x < tam && S[x] == sg) f1 = x;
-    }
-    // (the outer loops read S[x] from LDS: x varies at run time)
-    if (ballot(need & 4u)) {
-#pragma unroll 1
-        for (int x = tmax - 2; x >= 0; --x) {
-            const uint32_t t = sg ^ LS[x * 64 + lane];
-#pragma unroll
-            for (int y = kLaneMax - 1; y >= 1; --y)
-                if (y > x && y < tmax && y < tam && S[y] == t) f2 = x | (y << 4);
-        }
-    }
-    if (ballot(need & 8u)) {
-#pragma unroll 1
-        for (int x = tmax - 3; x >= 0; --x) {
-            const uint32_t tx = sg ^ LS[x * 64 + lane];
-#pragma unroll
-            for (int y = kLaneMax - 2; y >= 1; --y) {
-                if (y <= x || y >= tmax - 1) continue;
-                const uint32_t t = tx ^ S[y];
-#pragma unroll
-                for (int z = kLaneMax - 1; z > y; --z)
-                    if (z < tmax && z < tam && S[z] == t) f3 = x | (y << 4) | (z << 8);
+            const int f = (w == 1) ? f1 : (w == 2) ? f2 : f3;
+            if (f >= 0) {
+                const int x = f & 15, y = (f >> 4) & 15, z = f >> 8;
+                const int rank = (w == 1) ? x
+                               : (w == 2) ? nc2(tam) - nc2(tam - x) + (y - x - 1)
+                                          : nc3(tam) - nc3(tam - x) + nc2(tam - x - 1) - nc2(tam - y) + (z - y - 1);
+                hw = w; ha = x; hb = y; hc = z; hidx = cum + (unsigned)rank;
+                break;
             }
+            cum += N;
+            wsum += N * (unsigned)(w + 1);
         }
-    }
-    // walk the classes: the first class with a signature match holds the
-    // candidate; its rank within the class is in closed form
-    unsigned cum = 0, wsum = 0;
-    int hw = 0, ha = 0, hb = 0, hc = 0;
-    unsigned hidx = 0;
-    for (uint64_t l = wl; l; l >>= 4) {
-        const int w = (int)(l & 15u);
-        if (w > 3) break;
-        const unsigned N = (unsigned)((w == 1) ? tam : (w == 2) ? nc2(tam) : nc3(tam));
-        if (cum + N > (unsigned)kLaneBudget) break;
-        const int f = (w == 1) ? f1 : (w == 2) ? f2 : f3;
-        if (f >= 0) {
-            const int x = f & 15, y = (f >> 4) & 15, z = f >> 8;
-            const int rank = (w == 1) ? x
-                           : (w == 2) ? nc2(tam) - nc2(tam - x) + (y - x - 1)
-                                      : nc3(tam) - nc3(tam - x) + nc2(tam - x - 1) - nc2(tam - y) + (z - y - 1);
-            hw = w; ha = x; hb = y; hc = z; hidx = cum + (unsigned)rank;
-            break;
-        }
-        cum += N;
-        wsum += N * (unsigned)(w + 1);
-    }
-    // The candidate is confirmed exactly.  A signature collision (an earlier
-    // pattern matching the signature but not the syndrome) redoes the walk
-    // pattern by pattern with the exact check on every signature match.
-    if (hw != 0 && !lane_exact<D>(tb, m, s, LA[ha * 64 + lane], LA[hb * 64 + lane], LA[hc * 64 + lane], hw)) {
-        cum = 0; wsum = 0; hw = 0;
-        for (uint64_t l = wl; l && hw == 0; l >>= 4) {
-            const int w = (int)(l & 15u);
-            if (w > 3) break;
-            const unsigned N = (unsigned)((w == 1) ? tam : (w == 2) ? nc2(tam) : nc3(tam));
-            if (cum + N > (unsigned)kLaneBudget) break;
-            unsigned r = 0;
-            if (w == 1) {
-                for (int x = 0; x < tam && hw == 0; ++x, ++r)
-                    if (LS[x * 64 + lane] == sg && lane_exact<D>(tb, m, s, LA[x * 64 + lane], 0, 0, 1)) {
-                        hw = 1; ha = x; hidx = cum + r;
-                    }
-            } else if (w == 2) {
-                for (int x = 0; x + 1 < tam && hw == 0; ++x) {
-                    const uint32_t tx = sg ^ LS[x * 64 + lane];
-                    for (int y = x + 1; y < tam && hw == 0; ++y, ++r)
-                        if (LS[y * 64 + lane] == tx &&
-                            lane_exact<D>(tb, m, s, LA[x * 64 + lane], LA[y * 64 + lane], 0, 2)) {
-                            hw = 2; ha = x; hb = y; hidx = cum + r;
+        // The candidate is confirmed exactly.  A signature collision (an earlier
+        // pattern matching the signature but not the syndrome) redoes the walk
+        // pattern by pattern with the exact check on every signature match.
+        if (hw != 0 && !lane_exact<D>(tb, m, s, LA[ha * 64 + lane], LA[hb * 64 + lane], LA[hc * 64 + lane], hw)) {
+            cum = 0; wsum = 0; hw = 0;
+            for (uint64_t l = wl; l && hw == 0; l >>= 4) {
+                const int w = (int)(l & 15u);
+                if (w > 3) break;
+                const unsigned N = (unsigned)((w == 1) ? tam : (w == 2) ? nc2(tam) : nc3(tam));
+                if (cum + N > (unsigned)kLaneBudget) break;
+                unsigned r = 0;
+                if (w == 1) {
+                    for (int x = 0; x < tam && hw == 0; ++x, ++r)
+                        if (LS[x * 64 + lane] == sg && lane_exact<D>(tb, m, s, LA[x * 64 + lane], 0, 0, 1)) {
+                            hw = 1; ha = x; hidx = cum + r;
                         }
-                }
-            } else {
-                for (int x = 0; x + 2 < tam && hw == 0; ++x) {
-                    const uint32_t tx = sg ^ LS[x * 64 + lane];
-                    for (int y = x + 1; y + 1 < tam && hw == 0; ++y) {
-                        const uint32_t txy = tx ^ LS[y * 64 + lane];
-                        for (int z = y + 1; z < tam && hw == 0; ++z, ++r)
-                            if (LS[z * 64 + lane] == txy &&
-                                lane_exact<D>(tb, m, s, LA[x * 64 + lane], LA[y * 64 + lane], LA[z * 64 + lane], 3)) {
-                                hw = 3; ha = x; hb = y; hc = z; hidx = cum + r;
+                } else if (w == 2) {
+                    for (int x = 0; x + 1 < tam && hw == 0; ++x) {
+                        const uint32_t tx = sg ^ LS[x * 64 + lane];
+                        for (int y = x + 1; y < tam && hw == 0; ++y, ++r)
+                            if (LS[y * 64 + lane] == tx &&
+                                lane_exact<D>(tb, m, s, LA[x * 64 + lane], LA[y * 64 + lane], 0, 2)) {
+                                hw = 2; ha = x; hb = y; hidx = cum + r;
                             }
                     }
+                } else {
+                    for (int x = 0; x + 2 < tam && hw == 0; ++x) {
+                        const uint32_t tx = sg ^ LS[x * 64 + lane];
+                        for (int y = x + 1; y + 1 < tam && hw == 0; ++y) {
+                            const uint32_t txy = tx ^ LS[y * 64 + lane];
+                            for (int z = y + 1; z < tam && hw == 0; ++z, ++r)
+                                if (LS[z * 64 + lane] == txy &&
+                                    lane_exact<D>(tb, m, s, LA[x * 64 + lane], LA[y * 64 + lane], LA[z * 64 + lane], 3)) {
+                                    hw = 3; ha = x; hb = y; hc = z; hidx = cum + r;
+                                }
+                        }
+                    }
                 }
+                if (hw == 0) { cum += N; wsum += N * (unsigned)(w + 1); }
             }
-            if (hw == 0) { cum += N; wsum += N * (unsigned)(w + 1); }
         }
+        if (hw != 0) break;
+        if (!whole) return false;                   // classes the lane does not search
+        done += tot;
+        wdone += wtot;
     }
     if (hw == 0) return false;
     const unsigned long long t1 = now_ticks();
     const int c0 = LA[ha * 64 + lane];
     const int c1 = hw > 1 ? LA[hb * 64 + lane] : c0;
     const int c2 = hw > 2 ? LA[hc * 64 + lane] : c0;
-    const unsigned count = hidx + 1u;
-    wsum += (hidx + 1u - cum) * (unsigned)(hw + 1);
+    const unsigned count = done + hidx + 1u;
+    wsum = wdone + wsum + (hidx + 1u - cum) * (unsigned)(hw + 1);
     for (int wd = 0; wd < M32; ++wd) {
         uint32_t v = 0u;
         if ((c0 >> 5) == wd) v |= 1u << (c0 & 31);
@@ -2461,7 +2487,7 @@ __device__ __forceinline__ bool lane_search(const DevCode &code, const DevCall &
     const uint64_t tub = (uint64_t)__double_as_longlong((double)(ta - t0) * call.ms_per_tick);
     uint32_t *p = reinterpret_cast<uint32_t *>(&call.results[shot]);
     p[0] = 1u; p[1] = (uint32_t)hw; p[2] = (uint32_t)ow; p[3] = (uint32_t)deg; p[4] = (uint32_t)eq;
-    p[5] = (uint32_t)tam; p[6] = count; p[7] = 0u; p[8] = (uint32_t)((br > 0) ? 0 : -1); p[9] = 1u;
+    p[5] = (uint32_t)tam; p[6] = count; p[7] = 0u; p[8] = (uint32_t)hib; p[9] = (uint32_t)it;
     p[10] = (uint32_t)tub; p[11] = (uint32_t)(tub >> 32); p[12] = (uint32_t)tsb; p[13] = (uint32_t)(tsb >> 32);
     p[14] = 0u; p[15] = 0u;
     p[16] = (uint32_t)tsb; p[17] = (uint32_t)(tsb >> 32);

@@ -47,9 +47,10 @@ struct DevCode {
     // [D][m] at 0, signatures [m] at t_sig, degeneracy words [m] at t_dg (when
     // dg is set), group starts [n + 1] at t_gs, group-member records of t_recw
     // words (column, then its D words) at t_rec; groups: nonempty columns by
-    // their lowest check
+    // their lowest check; t_gs2 > 0: second groups (columns of weight >= 2 by
+    // their second-lowest check), starts at t_gs2 and records at t_rec2
     const uint32_t *timg;
-    int timg_words, t_sig, t_dg, t_gs, t_rec, t_recw;
+    int timg_words, t_sig, t_dg, t_gs, t_rec, t_recw, t_gs2, t_rec2;
     const float *llr;                 // [m] (ML)
     const unsigned long long *nCr;    // [(kMaxTam + 1) * kNcrStride]
     // Small codes are staged in the search kernel's LDS.  The search kernel gets
@@ -74,7 +75,7 @@ constexpr int kPairBitmapWords = 1024;   // 2^15-bit filter, by signature bits 0
 constexpr unsigned kPairNone = 0xFFFu;   // end of a bucket chain; also the most pairs a table holds
 constexpr int kPairHeadWords = kPairBuckets + 4;   // bucket heads, a spare head (never looked up), padding
 constexpr size_t kLdsPerCu = 160 * 1024; // gfx950
-// Search of a shot by its own lane in the triage kernel (first iteration only)
+// Search of a shot by its own lane in the triage kernel (first two iterations)
 constexpr int kLaneMax = 12;             // active-list entries a lane keeps
 constexpr int kLaneBudget = 300;         // patterns a lane may test
 constexpr int kLaneMaxD = 6;             // codes with at most 6 syndrome words ...
