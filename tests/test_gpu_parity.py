@@ -188,6 +188,45 @@ def test_triage_search_matches_oracle_and_search_kernel(oracle, product, tmp_pat
         print(what, "found", int(o[1]["found"].sum()), flush=True)
 
 
+def test_triage_second_iteration(oracle, product, tmp_path):
+    """The lane search of the second iteration (CSS family, every nonempty column of weight >= 2): BB144
+    prob_cc shots of the bench's error model, about 1.3 % of them decoded in the second iteration, bit-exact
+    with the oracle and with the search kernel alone.  Then a code with a weight-1 and a weight-2 column
+    (the weight-1 column is active at every lower threshold, so the lane search stays on the first
+    iteration) and one whose columns all have weight 2 after the change (the second groups exist),
+    syndromes as input, hybrid mode included."""
+    pf = T.phase_file(tmp_path)
+    hx, hz, e = T.cc_workload(144, 20000, "loguniform", seed=35)
+    kw = dict(errors=e, cfg=G.make_config(max_error_weight=4), prob_file=pf)
+    o = T.run_cc(oracle, G.KIND_PROB_CC, hx, hz, **kw)
+    it2 = int(((o[1]["iteration"] == 2) & (o[1]["found"] == 1)).sum())
+    assert it2 >= 150, it2
+    on = T.run_cc(product, G.KIND_PROB_CC, hx, hz, ex=G.make_ex(triage_search=0), **kw)
+    off = T.run_cc(product, G.KIND_PROB_CC, hx, hz, ex=G.make_ex(triage_search=-1), **kw)
+    T.assert_same(o, on, "BB144 second iteration (triage search on)")
+    T.assert_same(on, off, "BB144 second iteration (on vs off)")
+    hx72, hz72, e72 = T.cc_workload(72, 3000, 0.01, seed=36)
+    low = hz72.copy()
+    rows = np.nonzero(low[:, 5])[0]
+    low[rows[1:], 5] = 0                          # column 5: weight 1
+    rows = np.nonzero(low[:, 9])[0]
+    low[rows[2:], 9] = 0                          # column 9: weight 2
+    two = hz72.copy()
+    for c in range(two.shape[1]):                 # every column: weight 2
+        rows = np.nonzero(two[:, c])[0]
+        two[rows[2:], c] = 0
+    for name, h in (("weight-1 column", low), ("weight-2 columns", two)):
+        syn = T.codes.syndromes(h, e72)
+        for cfg in (G.make_config(max_error_weight=4),
+                    G.make_config(max_error_weight=4, mode_of_search=1, threshold_for_hybrid=3,
+                                  max_size_active_positions=30)):
+            args = (G.KIND_CC, hx72, h)
+            o = T.run_cc(oracle, *args, syndromes=syn, cfg=cfg)
+            on = T.run_cc(product, *args, syndromes=syn, cfg=cfg, ex=G.make_ex(triage_search=0))
+            T.assert_same(o, on, f"{name} mode {cfg.mode_of_search}")
+            print(name, "mode", cfg.mode_of_search, "iterations", np.bincount(o[1]["iteration"]), flush=True)
+
+
 # ---------------------------------------------------------------- pair table
 # Deep first-hit classes (weight >= 3, many more prefixes than pairs) are walked
 # by (w-2)-stems and the LDS pair table when it fits the wave's share of the
