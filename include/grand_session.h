@@ -159,7 +159,7 @@ int grand_device_info(int *cus, int *clock_khz, int *wall_clock_khz);
  * (values <= 0 keep the current setting): workgroups per CU, shots per
  * worklist claim (at most 8), prefixes per lane per chunk, and the class size
  * in prefixes above which a weight class is shared by all waves.  Built-in
- * defaults: 1, 2, 64, 32768 for the first-hit kinds and 8, 2, 64, 32768 for the
+ * defaults: 1, 2, 64, 65536 for the first-hit kinds and 8, 2, 64, 32768 for the
  * ML kind; a set replaces both, a get reads the first-hit set, and a set with
  * all four values negative restores the built-in defaults. */
 void grand_set_tuning(int waves_per_cu, int claim_batch, int rmax, int split_threshold);

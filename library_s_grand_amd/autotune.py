@@ -34,7 +34,7 @@ SPACE = {
     "waves_per_cu": (1, 2, 4, 8, 16),
     "claim_batch": (1, 2, 4, 8),
     "rmax": (8, 16, 32, 64),
-    "split_threshold": (256, 1024, 2048, 8192, 32768),
+    "split_threshold": (256, 1024, 2048, 8192, 32768, 65536),
     "pair_table_kb": (0, 8, 16, -1),
 }
 KEYS = ("waves_per_cu", "claim_batch", "rmax", "split_threshold", "pair_table_kb")

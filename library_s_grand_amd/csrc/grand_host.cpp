@@ -608,13 +608,15 @@ int kind_search(int kind) {
 // is a few hundred shots: one wave per CU decodes them as fast as two and
 // leaves the CUs' other SIMDs and LDS to the next batch's triage on another
 // stream (BB144 prob_cc, 100k shots, per decode: 146.6 us on one stream and
-// 84.7 us on two, against 150.9 and 120.0 us at 2 waves per CU).  ML: 8 waves
+// 84.7 us on two, against 150.9 and 120.0 us at 2 waves per CU).  Runs are
+// shared above 65536 prefix steps of remaining work (144.4 and 82.3 us; 32768:
+// 146.6 and 85.5 us; 131072: 171.7 and 92.5 us).  ML: 8 waves
 // per CU (the DEM ML workload, tuned: 10.1 M shots/s, against 5.2 M with the
 // former set and 2.8 M with the first-hit set).
 // The pair-table LDS (pair_table_kb: 0 = the wave's whole remaining share,
 // -1 = none, k = at most k KiB) applies to the first-hit kinds only.
 struct Tuning { int waves_per_cu; int claim_batch; int rmax; int split_threshold; int pair_table_kb; };
-constexpr Tuning kTuningFirstHit{1, 2, 64, 32768, 0};
+constexpr Tuning kTuningFirstHit{1, 2, 64, 65536, 0};
 constexpr Tuning kTuningMl{8, 2, 64, 32768, 0};
 Tuning g_tuning = kTuningFirstHit;
 Tuning g_tuning_ml = kTuningMl;

@@ -74,7 +74,7 @@ def test_product_exports_every_declared_function():
 
 def test_tuning_knobs_set_get_and_reset():
     """Host-side state only (no GPU): a set replaces the knobs, values <= 0 keep them, all four negative
-    restores the built-in first-hit defaults (1 wave per CU, claim batch 2, 64 prefixes, split 32768,
+    restores the built-in first-hit defaults (1 wave per CU, claim batch 2, 64 prefixes, split 65536,
     the pair table filling the wave's LDS share)."""
     from library_s_grand_amd import autotune
     lib = ctypes.CDLL(G.DEFAULT_LIB)
@@ -92,7 +92,7 @@ def test_tuning_knobs_set_get_and_reset():
         assert autotune.get(lib)["pair_table_kb"] == -1
     finally:
         autotune.reset(lib)
-    assert autotune.get(lib) == {"waves_per_cu": 1, "claim_batch": 2, "rmax": 64, "split_threshold": 32768,
+    assert autotune.get(lib) == {"waves_per_cu": 1, "claim_batch": 2, "rmax": 64, "split_threshold": 65536,
                                  "pair_table_kb": 0}
 
 
