@@ -76,10 +76,15 @@ constexpr unsigned kPairNone = 0xFFFu;   // end of a bucket chain; also the most
 constexpr int kPairHeadWords = kPairBuckets + 4;   // bucket heads, a spare head (never looked up), padding
 constexpr size_t kLdsPerCu = 160 * 1024; // gfx950
 // Search of a shot by its own lane in the triage kernel (first two iterations)
-constexpr int kLaneMax = 12;             // active-list entries a lane keeps
+// active-list entries a lane keeps: a longer list makes every lane of its wave
+// run the longer register compares, which costs the triage kernel more than the
+// search kernel spends on those shots (BB144 prob_cc, 100k shots, per decode on
+// one / two streams: 138.0 / 78.7 us at 7, 139.2 / 80.1 at 8, 144.7 / 83.1 at 12,
+// 147.8 / 82.6 at 5)
+constexpr int kLaneMax = 7;
 constexpr int kLaneBudget = 300;         // patterns a lane may test
 constexpr int kLaneMaxD = 6;             // codes with at most 6 syndrome words ...
-constexpr int kLaneMaxMD = 2048;         // ... and m * D at most this (the lane scans every column)
+constexpr int kLaneMaxMD = 2048;         // ... and m * D at most this
 constexpr int kTriageWlWords = 32;       // LDS words of the lane search's weight lists (one u64 per list size)
 
 // Claim words of the persistent search kernel: kClaimWords 64-bit words, one
