@@ -367,68 +367,48 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
                 if (L[(size_t)c * NL + q]) Lp[(size_t)c * NL32 + (q >> 5)] |= 1u << (q & 31);
     }
 
-    // The triage kernel's tables as one image (staged in its LDS when small):
-    // column words [D][m], signatures [m], degeneracy words [m] (when there are
-    // any), the start of each check's group [n + 1], then one record per group
-    // member, 16-byte aligned: the column and its D words, padded to 4 words
-    // (D <= 3) or 8.  A group holds the nonempty columns whose lowest check is
-    // that check: a column whose support lies inside supp(s) is in the group of
-    // one of s's checks, so the lane search tests only those groups.  For the
-    // CSS family, when every nonempty column has weight >= 2, second groups
-    // follow (starts [n + 1] at t_gs2, records at t_rec2): the columns by their
-    // second-lowest check.  A column with at most one check outside supp(s)
-    // (the second iteration's active set, score >= max_soma - 1) has its
-    // lowest or its second-lowest check in supp(s), so the lane search of the
-    // second iteration tests the first and second groups of s's checks.
+    // The triage kernel's tables as one image (staged in its LDS when small): column words [D][m], signatures [m], degeneracy words [m]
+    // (when there are any); then, 16-byte aligned, the rows of H [n][t_rw] with
+    // the odd stride t_rw = M32 | 1 (lanes reading the rows of different checks
+    // then spread over the LDS banks), and the column-weight planes
+    // [7][M32]: the nonempty columns, the three bit planes of colw, the three
+    // bit planes of colw - 1.  s = H e uses the first part; the lane search
+    // counts the checks of s in every column as the bit-sliced
+    // sum of the rows of H at s's checks, and a column is active at threshold
+    // max_soma - j iff its count is within j of colw.
     std::vector<uint32_t> timg;
-    int t_sig, t_dg, t_gs, t_rec, t_recw, t_gs2 = 0, t_rec2 = 0;
+    int t_sig, t_dg, t_rows, t_rw, t_pl, lane_ok;
     {
-        std::vector<int> low(m, -1), low2(m, -1), gs(n + 1, 0), gs2(n + 1, 0);
-        for (int c = 0; c < m; ++c)
-            for (int r = 0; r < n && low2[c] < 0; ++r)
-                if ((Hcm[(size_t)c * D + (r >> 5)] >> (r & 31)) & 1u) {
-                    if (low[c] < 0) low[c] = r;
-                    else low2[c] = r;
-                }
-        bool second = (family == kCC);
-        for (int c = 0; c < m; ++c)
-            if (colw[c] == 1) second = false;          // active at every threshold below max_soma
-        for (int c = 0; c < m; ++c) if (low[c] >= 0) ++gs[low[c] + 1];
-        for (int r = 0; r < n; ++r) gs[r + 1] += gs[r];
-        for (int c = 0; c < m; ++c) if (low2[c] >= 0) ++gs2[low2[c] + 1];
-        for (int r = 0; r < n; ++r) gs2[r + 1] += gs2[r];
-        t_recw = (D + 1 <= 4) ? 4 : 8 * ((D + 8) / 8);
+        int maxw = 0;
+        for (int c = 0; c < m; ++c) maxw = std::max(maxw, colw[c]);
+        lane_ok = (D <= kLaneMaxD && M32 <= kLaneMaxM32 && maxw <= kLaneMaxColw) ? 1 : 0;
+        t_rw = M32 | 1;
         timg.assign(Hcw.begin(), Hcw.end());
         t_sig = (int)timg.size();
         timg.insert(timg.end(), sig.begin(), sig.end());
         t_dg = (int)timg.size();
         timg.insert(timg.end(), dg.begin(), dg.end());
-        t_gs = (int)timg.size();
-        timg.insert(timg.end(), gs.begin(), gs.end());
         timg.resize((timg.size() + 3) & ~(size_t)3, 0u);
-        t_rec = (int)timg.size();
-        timg.resize(timg.size() + (size_t)gs[n] * t_recw, 0u);
-        std::vector<int> fill(gs.begin(), gs.end() - 1);
+        t_rows = (int)timg.size();
+        timg.resize(timg.size() + (size_t)n * t_rw, 0u);
+        for (int r = 0; r < n; ++r)
+            for (int wd = 0; wd < M32; ++wd) timg[(size_t)t_rows + (size_t)r * t_rw + wd] = Hrow[(size_t)r * M32 + wd];
+        t_pl = (int)timg.size();
+        timg.resize(timg.size() + (size_t)7 * M32, 0u);
+        uint32_t *pl = timg.data() + t_pl;
         for (int c = 0; c < m; ++c) {
-            if (low[c] < 0) continue;
-            uint32_t *rec = timg.data() + t_rec + (size_t)fill[low[c]]++ * t_recw;
-            rec[0] = (uint32_t)c;
-            for (int d = 0; d < D; ++d) rec[1 + d] = Hcm[(size_t)c * D + d];
-        }
-        if (second) {
-            t_gs2 = (int)timg.size();
-            timg.insert(timg.end(), gs2.begin(), gs2.end());
-            timg.resize((timg.size() + 3) & ~(size_t)3, 0u);
-            t_rec2 = (int)timg.size();
-            timg.resize(timg.size() + (size_t)gs2[n] * t_recw, 0u);
-            std::vector<int> fill2(gs2.begin(), gs2.end() - 1);
-            for (int c = 0; c < m; ++c) {
-                if (low2[c] < 0) continue;
-                uint32_t *rec = timg.data() + t_rec2 + (size_t)fill2[low2[c]]++ * t_recw;
-                rec[0] = (uint32_t)c;
-                for (int d = 0; d < D; ++d) rec[1 + d] = Hcm[(size_t)c * D + d];
+            if (colw[c] == 0) continue;
+            const uint32_t bit = 1u << (c & 31);
+            const int wd = c >> 5;
+            pl[wd] |= bit;
+            for (int b = 0; b < 3; ++b) {
+                if ((colw[c] >> b) & 1) pl[(1 + b) * M32 + wd] |= bit;
+                if (((colw[c] - 1) >> b) & 1) pl[(4 + b) * M32 + wd] |= bit;
             }
         }
+        // zero words after the last table: the lane search reads rows and
+        // planes kLaneMaxM32 words wide (the words past M32 are not used)
+        timg.resize((timg.size() + kLaneMaxM32 + 2 + 3) & ~(size_t)3, 0u);
     }
 
     // One allocation, 256-byte aligned segments.
@@ -470,8 +450,8 @@ std::shared_ptr<CodeEntry> upload_code(uint64_t key, int family, const int *H, c
     dc.dg = dg.empty() ? nullptr : (const uint32_t *)(base + segs[iDg].off);
     dc.timg = (const uint32_t *)(base + segs[iTimg].off);
     dc.timg_words = (int)timg.size();
-    dc.t_sig = t_sig; dc.t_dg = t_dg; dc.t_gs = t_gs; dc.t_rec = t_rec; dc.t_recw = t_recw;
-    dc.t_gs2 = t_gs2; dc.t_rec2 = t_rec2;
+    dc.t_sig = t_sig; dc.t_dg = t_dg; dc.t_rows = t_rows; dc.t_rw = t_rw; dc.t_pl = t_pl;
+    dc.lane_ok = lane_ok;
 
     dc.llr = (const float *)(base + segs[iLlr].off);
     dc.nCr = rt().nCr.as<unsigned long long>();
@@ -899,7 +879,7 @@ int decode_view(GrandSession *s, const WsView &w, const int *d_input, int input_
     // triage kernel (fused engine, first-hit kinds, small codes); with errors
     // as input the degeneracy words must exist
     c.triage_search = (s->ex.engine != GRAND_ENGINE_PLAIN && s->ex.triage_search >= 0 && s->search != kML &&
-                       c.lscores == nullptr && code.D <= kLaneMaxD && (long long)code.m * code.D <= kLaneMaxMD &&
+                       c.lscores == nullptr && code.lane_ok &&
                        !(s->family == kCC && c.input_errors && code.dg == nullptr)) ? 1 : 0;
     c.split_worklist = (s->ex.engine != GRAND_ENGINE_PLAIN) ? 1 : 0;
     c.wl_parts = (s->ex.engine != GRAND_ENGINE_PLAIN) ? kWlParts : 1;
@@ -946,9 +926,7 @@ int decode_view(GrandSession *s, const WsView &w, const int *d_input, int input_
         tev = s->ev.data() + 3 * k;
         HIPX(hipEventRecord(tev[0], st));
     }
-    // (the lane search reuses the block as its M32-word active bitmask per lane)
-    const int block_words = std::max(std::max(c.input_errors ? code.M32 : code.D, s->family == kCLN ? code.NL32 : 0),
-                                     c.triage_search ? code.M32 : 0);
+    const int block_words = std::max(c.input_errors ? code.M32 : code.D, s->family == kCLN ? code.NL32 : 0);
     HIPX(grand_launch_triage(dcode, dcall, tgrid, block_words, c.triage_stage_words, st));
     if (tev) HIPX(hipEventRecord(tev[1], st));
     if (s->ex.engine == GRAND_ENGINE_PLAIN) {

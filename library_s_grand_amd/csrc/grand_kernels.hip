@@ -2178,7 +2178,8 @@ __device__ __forceinline__ void write_zero_result(GrandDecoderResult *dst, int o
 // weight classes of its list are of weight <= 3 with at most kLaneBudget
 // patterns in all.  At threshold max_soma - j a column is active iff at most
 // j of its checks lie outside supp(s) (score = |supp(h_c) & supp(s)| +
-// max_soma - colw[c], grand_decoder.cu:406-416, :495-504).  Classes are
+// max_soma - colw[c], grand_decoder.cu:406-416, :495-504): its count of s's
+// checks is within j of colw[c].  Classes are
 // enumerated in the order of the wave path (lexicographic within a class,
 // classes in list order), each candidate filtered by its signature and checked
 // exactly, so the hit, its count, its iteration and its columns are the wave
@@ -2212,104 +2213,161 @@ __device__ uint64_t lane_weight_list(const DevCall &call, int tam) {
     return wl;
 }
 
-// The triage kernel's code tables (the image DevCode::timg): column words
-// (word-major, hcw[d * m + c]), signatures, degeneracy words (has_dg), the
-// start of each check's group (gs, n + 1 words) and the group-member records
-// (rec, a column and its D words per record).  In the wave's LDS when the image
-// is small enough to stage there, else read from global memory.
+// The code tables of the triage and lane kernels (the image DevCode::timg):
+// column words (word-major, hcw[d * m + c]), signatures, degeneracy words
+// (has_dg), the rows of H (rows[r * rw + wd], column bits of check r) and the
+// column-weight planes pl[7][M32] (nonempty columns, colw's three bit planes,
+// colw - 1's three bit planes).  In the wave's LDS when staged there, else read
+// from global memory.
 template <class U>
 struct TriTables {
     U hcw;
     U sig;
     U dg;
-    U gs;
-    U rec;
-    U gs2;                             // second groups (second is true)
-    U rec2;
+    U rows;
+    U pl;
+    int rw;
     bool has_dg;
-    bool second;
 };
 using TriLds = TriTables<const uint32_t LDS *>;
 using TriGlobal = TriTables<const uint32_t *>;
 template <class U>
 __device__ __forceinline__ TriTables<U> tri_tables(const DevCode &code, U img) {
-    return TriTables<U>{img, img + code.t_sig, img + code.t_dg, img + code.t_gs, img + code.t_rec,
-                        img + code.t_gs2, img + code.t_rec2, code.dg != nullptr, code.t_gs2 > 0};
+    return TriTables<U>{img, img + code.t_sig, img + code.t_dg, img + code.t_rows, img + code.t_pl, code.t_rw,
+                        code.dg != nullptr};
 }
 
+// s == h_a (+ h_b for w > 1, + h_c for w > 2); a, b and c are columns of the
+// code whatever w is (the words of all three are loaded together).
 template <int D, class T>
 __device__ __forceinline__ bool lane_exact(const T &tb, int m, const uint32_t (&s)[kMaxWords], int a, int b, int c,
                                            int w) {
-    uint32_t bad = 0u;
+    uint32_t xa[D], xb[D], xc[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        uint32_t x = s[d] ^ tb.hcw[d * m + a];
-        if (w > 1) x ^= tb.hcw[d * m + b];
-        if (w > 2) x ^= tb.hcw[d * m + c];
-        bad |= x;
+        xa[d] = tb.hcw[d * m + a];
+        xb[d] = tb.hcw[d * m + b];
+        xc[d] = tb.hcw[d * m + c];
     }
+    uint32_t bad = 0u;
+#pragma unroll
+    for (int d = 0; d < D; ++d) bad |= s[d] ^ xa[d] ^ (w > 1 ? xb[d] : 0u) ^ (w > 2 ? xc[d] : 0u);
     return bad == 0u;
+}
+
+// Words per row the lane search reads: M32 rounded up to 4, 6 or kLaneMaxM32.
+// The image has kLaneMaxM32 + 2 zero words after its last table, so reading NW
+// words from any row or plane start stays inside it; the words past M32 are
+// never used.
+__device__ __forceinline__ int lane_words(int M32) { return M32 <= 4 ? 4 : M32 <= 6 ? 6 : kLaneMaxM32; }
+
+// Bit-sliced counts q2 q1 q0 (0..7) of s's checks in every column: the sum of
+// the rows of H at the set bits of s.  One loop over all set bits of the D
+// words; a row's NW words are loaded together.
+template <int NW, int D, class T>
+__device__ __forceinline__ void lane_counts(const T &tb, const uint32_t (&s)[kMaxWords], uint32_t (&q0)[kLaneMaxM32],
+                                            uint32_t (&q1)[kLaneMaxM32], uint32_t (&q2)[kLaneMaxM32]) {
+#pragma unroll
+    for (int wd = 0; wd < kLaneMaxM32; ++wd) { q0[wd] = 0u; q1[wd] = 0u; q2[wd] = 0u; }
+    int d = 0;
+    uint32_t bs = s[0];
+    for (;;) {
+        while (bs == 0u && d + 1 < D) {
+            ++d;
+            uint32_t nx = s[0];
+#pragma unroll
+            for (int e = 1; e < D; ++e) nx = (d == e) ? s[e] : nx;
+            bs = nx;
+        }
+        if (bs == 0u) break;
+        const int r = 32 * d + __ffs(bs) - 1;
+        bs &= bs - 1u;
+        const auto row = tb.rows + r * tb.rw;
+        uint32_t x[NW];
+#pragma unroll
+        for (int wd = 0; wd < NW; ++wd) x[wd] = row[wd];
+#pragma unroll
+        for (int wd = 0; wd < NW; ++wd) {
+            const uint32_t c0 = q0[wd] & x[wd];
+            q0[wd] ^= x[wd];
+            const uint32_t c1 = q1[wd] & c0;
+            q1[wd] ^= c0;
+            q2[wd] ^= c1;
+        }
+    }
+}
+
+// Active columns at iteration it (count == colw, or for it = 2 also colw - 1;
+// nonempty columns only) as words act[0 .. M32); returns how many.
+template <int NW, class T>
+__device__ __forceinline__ int lane_active(const T &tb, int M32, int it, const uint32_t (&q0)[kLaneMaxM32],
+                                           const uint32_t (&q1)[kLaneMaxM32], const uint32_t (&q2)[kLaneMaxM32],
+                                           uint32_t (&act)[kLaneMaxM32]) {
+    uint32_t pl[7][NW];
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+#pragma unroll
+        for (int wd = 0; wd < NW; ++wd) pl[k][wd] = tb.pl[k * M32 + wd];
+    int tam = 0;
+#pragma unroll
+    for (int wd = 0; wd < kLaneMaxM32; ++wd) {
+        act[wd] = 0u;
+        if (wd < NW) {
+            uint32_t a = ~((q0[wd] ^ pl[1][wd]) | (q1[wd] ^ pl[2][wd]) | (q2[wd] ^ pl[3][wd]));
+            if (it == 2) a |= ~((q0[wd] ^ pl[4][wd]) | (q1[wd] ^ pl[5][wd]) | (q2[wd] ^ pl[6][wd]));
+            act[wd] = (wd < M32) ? (a & pl[0][wd]) : 0u;
+            tam += __popc(act[wd]);
+        }
+    }
+    return tam;
 }
 
 template <int D, class T>
 __device__ __forceinline__ bool lane_search(const DevCode &code, const DevCall &call, const T &tb, int shot,
                             const uint32_t (&s)[kMaxWords], uint32_t sg, int ow, uint32_t dgv, int LDS *LA,
-                            uint32_t LDS *LS, const unsigned long long LDS *WL, uint32_t LDS *AB,
-                            unsigned long long t0) {
+                            uint32_t LDS *LS, const unsigned long long LDS *WL, unsigned long long t0) {
     const int lane = lane_id();
     const int m = code.m;
     const int M32 = code.M32;
     auto nc2 = [](int n) { return n * (n - 1) / 2; };
     auto nc3 = [](int n) { return n * (n - 1) * (n - 2) / 6; };
     // Iterations of decode_shot_wave searched here: the first, and for the CSS
-    // family with second groups the second, when the first is searched whole
+    // family the second (when max_soma >= 2), when the first is searched whole
     // without a hit (or has no list).  done / wdone: patterns (weighted
     // patterns) of the iterations searched without a hit; hib: the hibrido
     // code as decode_shot_wave updates it.
-    const int itmax = (call.family == kCC && tb.second) ? 2 : 1;
+    const int itmax = (call.family == kCC && code.max_soma >= 2) ? 2 : 1;
     unsigned long long ta = 0;
     unsigned done = 0u, wdone = 0u, cum = 0u, wsum = 0u, hidx = 0u;
     int hib = -2, it = 1, tam = 0, hw = 0, ha = 0, hb = 0, hc = 0;
+    // The score of column c is |supp(h_c) & supp(s)| + max_soma - colw[c]
+    // (nonempty columns), so at threshold max_soma - (it - 1) the column is
+    // active iff its count of s's checks is colw[c] (it = 1) or colw[c] - 1 or
+    // colw[c] (it = 2).  The counts of all columns, bit-sliced over the column
+    // words (planes q0, q1, q2: counts 0..7), are the sum of the rows of H at
+    // s's checks; both iterations' active sets are read off them.
+    uint32_t q0[kLaneMaxM32], q1[kLaneMaxM32], q2[kLaneMaxM32];
+    const int nw = lane_words(M32);
+    if (nw == 4) lane_counts<4, D>(tb, s, q0, q1, q2);
+    else if (nw == 6) lane_counts<6, D>(tb, s, q0, q1, q2);
+    else lane_counts<kLaneMaxM32, D>(tb, s, q0, q1, q2);
     for (; it <= itmax; ++it) {
-        // Active columns at threshold max_soma - (it - 1): nonempty, at most
-        // it - 1 of their checks outside supp(s).  Such a column holds its
-        // lowest check (it = 1) or its lowest or second-lowest check (it = 2),
-        // so only the groups (and second groups) of s's checks are tested;
-        // hits set bits of the lane's bitmask AB[wd * 64 + lane], read back in
-        // column order (the order of the wave path's active list), at most
-        // kLaneMax + 1 of them.
-        for (int wd = 0; wd < M32; ++wd) AB[wd * 64 + lane] = 0u;
+        // the active columns, in column order (the order of the wave path's
+        // active list); a list longer than kLaneMax is not searched here
+        uint32_t act[kLaneMaxM32];
+        if (nw == 4) tam = lane_active<4>(tb, M32, it, q0, q1, q2, act);
+        else if (nw == 6) tam = lane_active<6>(tb, M32, it, q0, q1, q2, act);
+        else tam = lane_active<kLaneMaxM32>(tb, M32, it, q0, q1, q2, act);
+        if (tam > 0 && tam <= kLaneMax) {
+            int k = 0;
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            uint32_t bs = s[d];
-            while (bs) {
-                const int r = 32 * d + __ffs(bs) - 1;
-                bs &= bs - 1u;
-                for (int g = 0; g < it; ++g) {
-                    const auto gsp = g ? tb.gs2 : tb.gs;
-                    const auto rp = g ? tb.rec2 : tb.rec;
-                    const int k1 = (int)gsp[r + 1];
-                    for (int k = (int)gsp[r]; k < k1; ++k) {
-                        // the record: the column, then its words (read together)
-                        constexpr int RW = (D + 1 <= 4) ? 4 : 8;
-                        const int c = (int)rp[k * RW];
-                        int nout = 0;
-#pragma unroll
-                        for (int e = 0; e < D; ++e) nout += __popc(rp[k * RW + 1 + e] & ~s[e]);
-                        if (nout < it)
-                            __hip_atomic_fetch_or(&AB[(c >> 5) * 64 + lane], 1u << (c & 31), __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
-                    }
+            for (int wd = 0; wd < kLaneMaxM32; ++wd) {
+                uint32_t w = act[wd];
+                while (w) {
+                    LA[k * 64 + lane] = 32 * wd + __ffs(w) - 1;
+                    w &= w - 1u;
+                    ++k;
                 }
-            }
-        }
-        tam = 0;
-        for (int wd = 0; wd < M32 && tam <= kLaneMax; ++wd) {
-            uint32_t w = AB[wd * 64 + lane];
-            while (w && tam <= kLaneMax) {
-                if (tam < kLaneMax) LA[tam * 64 + lane] = 32 * wd + __ffs(w) - 1;
-                w &= w - 1u;
-                ++tam;
             }
         }
         if (it == 1) ta = now_ticks();             // the first active list is built (time_sum_ms)
@@ -2334,14 +2392,21 @@ __device__ __forceinline__ bool lane_search(const DevCode &code, const DevCall &
             tot += N;
             wtot += N * (unsigned)(w + 1);
         }
-        // First signature match of each needed class, lexicographic order, from the
-        // signatures in registers: the loops run backwards and the last match
-        // written is the first in order.  Packed as a | b << 4 | c << 8, -1 = none.
-        // The signatures of the list, looked up together, in registers and in LS.
-        // tmax: the longest list among the lanes searching (bounds the loops).
+        // The list's columns (C) and signatures (S) in registers, loaded
+        // together (entries past tam read column 0), and the signatures in LS
+        // for the pattern-by-pattern walk after a collision.  First signature
+        // match of each needed class, lexicographic order: the fully unrolled
+        // loops run backwards and the last match written is the first in
+        // order.  Packed as a | b << 4 | c << 8, -1 = none.  tmax: the longest
+        // list among the wave's searching lanes (bounds the loops).
+        int C[kLaneMax];
         uint32_t S[kLaneMax];
 #pragma unroll
-        for (int i = 0; i < kLaneMax; ++i) S[i] = (i < tam) ? tb.sig[LA[i * 64 + lane]] : 0u;
+        for (int i = 0; i < kLaneMax; ++i) C[i] = LA[i * 64 + lane];
+#pragma unroll
+        for (int i = 0; i < kLaneMax; ++i) C[i] = (i < tam) ? C[i] : 0;
+#pragma unroll
+        for (int i = 0; i < kLaneMax; ++i) S[i] = tb.sig[C[i]];
 #pragma unroll
         for (int i = 0; i < kLaneMax; ++i)
             if (i < tam) LS[i * 64 + lane] = S[i];
@@ -2352,29 +2417,30 @@ __device__ __forceinline__ bool lane_search(const DevCode &code, const DevCall &
         if (ballot(need & 2u)) {
 #pragma unroll
             for (int x = kLaneMax - 1; x >= 0; --x)
-                if (x < tmax && x < tam && S[x] == sg) f1 = x;
+                if (x < tam && S[x] == sg) f1 = x;
         }
-        // (the outer loops read S[x] from LDS: x varies at run time)
         if (ballot(need & 4u)) {
-#pragma unroll 1
-            for (int x = tmax - 2; x >= 0; --x) {
-                const uint32_t t = sg ^ LS[x * 64 + lane];
 #pragma unroll
-                for (int y = kLaneMax - 1; y >= 1; --y)
-                    if (y > x && y < tmax && y < tam && S[y] == t) f2 = x | (y << 4);
+            for (int x = kLaneMax - 2; x >= 0; --x) {
+                if (x + 2 > tmax) continue;
+                const uint32_t t = sg ^ S[x];
+#pragma unroll
+                for (int y = kLaneMax - 1; y > x; --y)
+                    if (y < tam && S[y] == t) f2 = x | (y << 4);
             }
         }
         if (ballot(need & 8u)) {
-#pragma unroll 1
-            for (int x = tmax - 3; x >= 0; --x) {
-                const uint32_t tx = sg ^ LS[x * 64 + lane];
 #pragma unroll
-                for (int y = kLaneMax - 2; y >= 1; --y) {
-                    if (y <= x || y >= tmax - 1) continue;
+            for (int x = kLaneMax - 3; x >= 0; --x) {
+                if (x + 3 > tmax) continue;
+                const uint32_t tx = sg ^ S[x];
+#pragma unroll
+                for (int y = kLaneMax - 2; y > x; --y) {
+                    if (y + 2 > tmax) continue;
                     const uint32_t t = tx ^ S[y];
 #pragma unroll
                     for (int z = kLaneMax - 1; z > y; --z)
-                        if (z < tmax && z < tam && S[z] == t) f3 = x | (y << 4) | (z << 8);
+                        if (z < tam && S[z] == t) f3 = x | (y << 4) | (z << 8);
                 }
             }
         }
@@ -2401,7 +2467,14 @@ __device__ __forceinline__ bool lane_search(const DevCode &code, const DevCall &
         // The candidate is confirmed exactly.  A signature collision (an earlier
         // pattern matching the signature but not the syndrome) redoes the walk
         // pattern by pattern with the exact check on every signature match.
-        if (hw != 0 && !lane_exact<D>(tb, m, s, LA[ha * 64 + lane], LA[hb * 64 + lane], LA[hc * 64 + lane], hw)) {
+        int ca = C[0], cb = C[0], cc = C[0];
+#pragma unroll
+        for (int i = 1; i < kLaneMax; ++i) {
+            if (ha == i) ca = C[i];
+            if (hb == i) cb = C[i];
+            if (hc == i) cc = C[i];
+        }
+        if (hw != 0 && !lane_exact<D>(tb, m, s, ca, cb, cc, hw)) {
             cum = 0; wsum = 0; hw = 0;
             for (uint64_t l = wl; l && hw == 0; l >>= 4) {
                 const int w = (int)(l & 15u);
@@ -2501,16 +2574,16 @@ __device__ __forceinline__ bool lane_search(const DevCode &code, const DevCall &
 
 template <class T>
 __device__ __forceinline__ bool lane_search_any(const DevCode &code, const DevCall &call, const T &tb, int shot,
-                                             const uint32_t (&s)[kMaxWords], uint32_t sg, int ow, uint32_t dgv,
-                                             int LDS *LA, uint32_t LDS *LS, const unsigned long long LDS *WL,
-                                             uint32_t LDS *AB, unsigned long long t0) {
+                                                const uint32_t (&s)[kMaxWords], uint32_t sg, int ow, uint32_t dgv,
+                                                int LDS *LA, uint32_t LDS *LS, const unsigned long long LDS *WL,
+                                                unsigned long long t0) {
     switch (code.D) {
-        case 1: return lane_search<1>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, AB, t0);
-        case 2: return lane_search<2>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, AB, t0);
-        case 3: return lane_search<3>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, AB, t0);
-        case 4: return lane_search<4>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, AB, t0);
-        case 5: return lane_search<5>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, AB, t0);
-        case 6: return lane_search<6>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, AB, t0);
+        case 1: return lane_search<1>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, t0);
+        case 2: return lane_search<2>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, t0);
+        case 3: return lane_search<3>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, t0);
+        case 4: return lane_search<4>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, t0);
+        case 5: return lane_search<5>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, t0);
+        case 6: return lane_search<6>(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, t0);
         default: return false;
     }
 }
@@ -2674,8 +2747,7 @@ __device__ __forceinline__ void triage_groups(const DevCode &code, const DevCall
         bool solved = false;
         if (call.triage_search && ballot(nz)) {
             const unsigned long long t0 = now_ticks();
-            // the packed block is free again: its words hold the lanes' active bitmasks
-            if (nz) solved = lane_search_any(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, bits, t0);
+            if (nz) solved = lane_search_any(code, call, tb, shot, s, sg, ow, dgv, LA, LS, WL, t0);
         }
         const bool queued = nz && !solved;
         if (valid) {
@@ -2702,12 +2774,11 @@ __device__ __forceinline__ void triage_groups(const DevCode &code, const DevCall
         // search kernel: shots whose syndrome weight is at least that of two
         // disjoint columns go to the front of the worklist (their deep classes then
         // overlap with the rest of the batch), the others to the back; the plain
-        // engine reads one list
+        // engine reads one list.  The group's partition (g % wl_parts) spreads the
+        // counter atomics over wl_parts cache lines.
         int sw = 0;
 #pragma unroll
         for (int d = 0; d < kMaxWords; ++d) sw += __popc(s[d]);
-        // The group's partition (g % wl_parts) spreads the counter atomics over
-        // wl_parts cache lines.
         const bool front = queued && (!call.split_worklist || sw >= 2 * code.max_soma);
         const uint64_t mf = ballot(front), mb = ballot(queued && !front);
         const int part = g % call.wl_parts;
@@ -2730,8 +2801,8 @@ __device__ __forceinline__ void triage_groups(const DevCode &code, const DevCall
     if (ballot(bad != 0u) && lane == 0) atomicOr(&call.counters[2], kErrNonBinary);
 }
 
-// LDS: the lane search's [kLaneMax + 1][64] positions and signatures and its weight
-// lists by active-list size [kTriageWlWords / 2]; the code tables when
+// LDS: the lane search's [kLaneMax + 1][64] positions and signatures and its
+// weight lists by active-list size [kTriageWlWords / 2]; the code tables when
 // call.triage_stage_words > 0 (the image DevCode::timg); then the packed
 // block, 64 rows of the larger of the input row's words (M32 for errors, D for
 // syndromes) and, for the DEM family, the NL32 words of the logicals.

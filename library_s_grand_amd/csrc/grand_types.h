@@ -43,14 +43,17 @@ struct DevCode {
     const int *pivrow;                // [m] RREF row whose pivot is this column, -1 for other columns
     const uint32_t *Lp;               // [m][NL32] logical flips per mechanism (cln)
     const uint32_t *dg;               // [m] degeneracy words (cc), or null when more than 32 are needed
-    // the triage kernel's tables as one image of timg_words words: column words
-    // [D][m] at 0, signatures [m] at t_sig, degeneracy words [m] at t_dg (when
-    // dg is set), group starts [n + 1] at t_gs, group-member records of t_recw
-    // words (column, then its D words) at t_rec; groups: nonempty columns by
-    // their lowest check; t_gs2 > 0: second groups (columns of weight >= 2 by
-    // their second-lowest check), starts at t_gs2 and records at t_rec2
+    // the triage kernel's tables as one image of timg_words words: column
+    // words [D][m] at 0, signatures [m] at t_sig, degeneracy words [m] at t_dg
+    // (when dg is set); then, 16-byte aligned at t_rows, the rows of H
+    // [n][t_rw] (the odd stride t_rw = M32 | 1: no bank conflicts between lanes
+    // reading different rows), and at t_pl the column-weight planes [7][M32]:
+    // the nonempty columns, the three bit planes of colw and the three bit
+    // planes of colw - 1, then kLaneMaxM32 + 2 zero words.  lane_ok: the code
+    // fits the lane search (D <= kLaneMaxD, M32 <= kLaneMaxM32, colw <=
+    // kLaneMaxColw).
     const uint32_t *timg;
-    int timg_words, t_sig, t_dg, t_gs, t_rec, t_recw, t_gs2, t_rec2;
+    int timg_words, t_sig, t_dg, t_rows, t_rw, t_pl, lane_ok;
     const float *llr;                 // [m] (ML)
     const unsigned long long *nCr;    // [(kMaxTam + 1) * kNcrStride]
     // Small codes are staged in the search kernel's LDS.  The search kernel gets
@@ -75,8 +78,8 @@ constexpr int kPairBitmapWords = 1024;   // 2^15-bit filter, by signature bits 0
 constexpr unsigned kPairNone = 0xFFFu;   // end of a bucket chain; also the most pairs a table holds
 constexpr int kPairHeadWords = kPairBuckets + 4;   // bucket heads, a spare head (never looked up), padding
 constexpr size_t kLdsPerCu = 160 * 1024; // gfx950
-// Search of a shot by its own lane in the triage kernel (first two iterations)
-// active-list entries a lane keeps: a longer list makes every lane of its wave
+// Search of a shot by its own lane in the triage kernel (first two iterations).
+// Active-list entries a lane keeps: a longer list makes every lane of its wave
 // run the longer register compares, which costs the triage kernel more than the
 // search kernel spends on those shots (BB144 prob_cc, 100k shots, per decode on
 // one / two streams: 138.0 / 78.7 us at 7, 139.2 / 80.1 at 8, 144.7 / 83.1 at 12,
@@ -84,7 +87,8 @@ constexpr size_t kLdsPerCu = 160 * 1024; // gfx950
 constexpr int kLaneMax = 7;
 constexpr int kLaneBudget = 300;         // patterns a lane may test
 constexpr int kLaneMaxD = 6;             // codes with at most 6 syndrome words ...
-constexpr int kLaneMaxMD = 2048;         // ... and m * D at most this
+constexpr int kLaneMaxM32 = 10;          // ... at most 320 columns (bit-sliced column counts in registers) ...
+constexpr int kLaneMaxColw = 7;          // ... and column weights of at most 7 (three count planes)
 constexpr int kTriageWlWords = 32;       // LDS words of the lane search's weight lists (one u64 per list size)
 
 // Claim words of the persistent search kernel: kClaimWords 64-bit words, one

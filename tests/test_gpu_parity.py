@@ -189,12 +189,11 @@ def test_triage_search_matches_oracle_and_search_kernel(oracle, product, tmp_pat
 
 
 def test_triage_second_iteration(oracle, product, tmp_path):
-    """The lane search of the second iteration (CSS family, every nonempty column of weight >= 2): BB144
-    prob_cc shots of the bench's error model, about 1.3 % of them decoded in the second iteration, bit-exact
-    with the oracle and with the search kernel alone.  Then a code with a weight-1 and a weight-2 column
-    (the weight-1 column is active at every lower threshold, so the lane search stays on the first
-    iteration) and one whose columns all have weight 2 after the change (the second groups exist),
-    syndromes as input, hybrid mode included."""
+    """The lane search of the second iteration (CSS family): BB144 prob_cc shots of the bench's error
+    model, about 1.3 % of them decoded in the second iteration, bit-exact with the oracle and with the
+    search kernel alone.  Then a code with a weight-1 and a weight-2 column (a weight-1 column is active
+    at every lower threshold: its count of the syndrome's checks, 0, is within one of its weight) and one
+    whose columns all have weight 2 after the change, syndromes as input, hybrid mode included."""
     pf = T.phase_file(tmp_path)
     hx, hz, e = T.cc_workload(144, 20000, "loguniform", seed=35)
     kw = dict(errors=e, cfg=G.make_config(max_error_weight=4), prob_file=pf)
