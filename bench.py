@@ -290,8 +290,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--shots", type=int, default=SHOTS)
-    ap.add_argument("--streams", type=int, default=2,
-                    help="decode streams for the headline steps (consecutive batches overlap across streams)")
+    ap.add_argument("--streams", type=int, default=0,
+                    help="decode streams for the headline steps (consecutive batches overlap across streams); "
+                         "0: the faster of 2 and 3 in a short probe")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-engine leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop leg")
@@ -354,30 +355,41 @@ def main():
 
         best, best_ms, table = autotune.tune(lib, tune_run, reps=3, max_rounds=2)
         tuning = {"chosen": best, "search_ms": best_ms, "configs_measured": len(table)}
-    # Headline: consecutive batches decoded by `streams` sessions on as many
-    # streams, step k on session k % streams (each with its own workspace and
-    # outputs), so one batch's triage overlaps the previous batch's search tail.
+    # Headline: consecutive batches decoded by S sessions on as many streams,
+    # step k on session k % S (each with its own workspace and outputs), so
+    # one batch's triage and search overlap the other batches' search tails.
+    # S is given, or the faster of 2 and 3 in a probe of 10 steps each (the
+    # streams share 4 hardware queues; 4 streams have measured slower than 3).
     # No HIP events in this pass.
-    S = max(1, a.streams)
+    cands = [a.streams] if a.streams > 0 else [2, 3]
     lanes = [(sess, torch.cuda.current_stream(), d_res, d_dec, d_stats)]
-    for _ in range(S - 1):
+    for _ in range(max(cands) - 1):
         lanes.append((Session(G.KIND_PROB_CC, hx, hz, cfg, type_of_error="x", file=prob_file, lib=lib),
                       torch.cuda.Stream(), torch.empty_like(d_res), torch.empty_like(d_dec), torch.empty_like(d_stats)))
     sess.enable_timing(False)
 
-    def lane_step(k):
+    def lane_step(k, S):
         s_, st_, r_, dd_, ds_ = lanes[k % S]
         s_.decode(d_err, True, r_, d_decoded=dd_, d_stats=ds_, stream=st_.cuda_stream)
 
-    for k in range(max(a.warmup, S)):
-        lane_step(k)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(a.steps):
-        lane_step(k)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = 1e3 * wall / a.steps
+    def timed_steps(S, n, warm):
+        for k in range(warm):
+            lane_step(k, S)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for k in range(n):
+            lane_step(k, S)
+        torch.cuda.synchronize()
+        return 1e3 * (time.perf_counter() - t) / n
+
+    probe = {}
+    if len(cands) > 1:
+        for S_ in cands:
+            probe[S_] = timed_steps(S_, 10, 2 * S_)
+        S = min(probe, key=probe.get)
+    else:
+        S = cands[0]
+    ms_per_step = timed_steps(S, a.steps, max(a.warmup, S))
     lanes_same = True
     for s_, st_, r_, dd_, ds_ in lanes[1:]:
         if s_.error_bits(st_.cuda_stream) != 0:
@@ -453,8 +465,10 @@ def main():
                    "shots": shots, "code": "BB144", "entry_point": "grand_decode_prob_cc", "max_error_weight": W,
                    "engine": "fused", "parallelism": f"single GPU, {S} decode streams"},
         "streams": {"count": S, "bit_identical_across_streams": lanes_same,
+                    "probe_ms_per_step": {str(k): v for k, v in probe.items()},
                     "basis": f"step k decodes the whole batch on session k % {S}, each on its own stream with its own "
-                             "workspace and outputs: a batch's triage overlaps the previous batch's search tail"},
+                             "workspace and outputs: a batch's triage and search overlap the other batches' search "
+                             "tails; the count is the faster of 2 and 3 in a 10-step probe before the timed steps"},
         "single_stream": {"ms_per_step": ms_single, "shots_per_s": shots / (ms_single * 1e-3),
                           "ms_per_step_with_events": ms_events},
         "p50_us_per_shot": float(np.percentile(lat_us, 50)) if lat_us.size else 0.0,
