@@ -295,3 +295,51 @@ def test_deep_shots_in_small_batches(oracle, product, tmp_path):
                        cfg=cfg, prob_file=pf)
         ref = (want[0], want[1][sub], None if want[2] is None else want[2][sub], None)
         T.assert_same(ref, got, f"shots {[sel[i] for i in sub]}")
+
+
+# ------------------------------------------------------- lane search limits
+def _random_code(rng, n, m, wmax):
+    """n x m binary matrix whose column c has weight uniform in 1..wmax (distinct rows)."""
+    H = np.zeros((n, m), dtype=np.int32)
+    for c in range(m):
+        H[rng.choice(n, size=int(rng.integers(1, wmax + 1)), replace=False), c] = 1
+    return H
+
+
+@pytest.mark.parametrize("m,wmax", [(320, 7), (320, 8), (321, 7)])
+def test_lane_search_limits(oracle, product, tmp_path, m, wmax):
+    """The lane search's bit-sliced column counts at their limits: 320 columns (10 words per count
+    plane) with column weights up to 7 (three planes) searched by lane; a weight-8 column or a 321st
+    column turns the lane search off for the code.  Both families, uniform and phase-ordered, errors and
+    syndromes as input: bit-exact with the oracle and with the search kernel alone."""
+    rng = np.random.default_rng(1000 + m + wmax)
+    pf = T.phase_file(tmp_path)
+    hz = _random_code(rng, 96, m, wmax)
+    hx = _random_code(rng, 96, m, 3)
+    e = codes.sample_errors(rng, 3000, 0.003, m)
+    Hd = _random_code(rng, 64, m, wmax)
+    Ld = (rng.random((m, 8)) < 0.1).astype(np.int32)
+    ed = codes.sample_errors(rng, 3000, 0.003, m)
+    det = codes.syndromes(Hd, ed)
+    lg = ((ed.astype(np.int64) @ Ld) % 2).astype(np.int32)
+    cases = [
+        ("cc errors", T.run_cc, (G.KIND_CC, hx, hz), dict(errors=e, cfg=G.make_config(max_error_weight=3))),
+        ("cc syndromes", T.run_cc, (G.KIND_CC, hx, hz),
+         dict(syndromes=codes.syndromes(hz, e), cfg=G.make_config(max_error_weight=3))),
+        # (shots with at most two errors: weights 1..5 in three iterations over lists of 100+ columns
+        # would cost the oracle minutes on a few deep shots)
+        ("prob_cc errors", T.run_cc, (G.KIND_PROB_CC, hx, hz),
+         dict(errors=e[e.sum(axis=1) <= 2][:600], cfg=G.make_config(max_error_weight=4), prob_file=pf)),
+        ("cln", T.run_cln, (G.KIND_CLN, Hd, Ld, det, lg),
+         dict(cfg=G.make_config(max_error_weight=3, max_size_active_positions=40))),
+        ("prob_cln", T.run_cln, (G.KIND_PROB_CLN, Hd, Ld, det, lg),
+         dict(cfg=G.make_config(max_error_weight=4, max_size_active_positions=40), file=pf)),
+    ]
+    for what, runner, args, kw in cases:
+        o = runner(oracle, *args, **kw)
+        on = runner(product, *args, ex=G.make_ex(triage_search=0), **kw)
+        off = runner(product, *args, ex=G.make_ex(triage_search=-1), **kw)
+        name = f"m={m} colw<={wmax} {what}"
+        T.assert_same(o, on, name + " (lane search on)")
+        T.assert_same(on, off, name + " (on vs off)")
+        print(name, "found", int(o[1]["found"].sum()), "iterations", np.bincount(o[1]["iteration"]), flush=True)
