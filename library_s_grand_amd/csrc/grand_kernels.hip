@@ -2227,6 +2227,7 @@ struct TriTables {
     U rows;
     U pl;
     int rw;
+    int D;
     bool has_dg;
 };
 using TriLds = TriTables<const uint32_t LDS *>;
@@ -2234,7 +2235,7 @@ using TriGlobal = TriTables<const uint32_t *>;
 template <class U>
 __device__ __forceinline__ TriTables<U> tri_tables(const DevCode &code, U img) {
     return TriTables<U>{img, img + code.t_sig, img + code.t_dg, img + code.t_rows, img + code.t_pl, code.t_rw,
-                        code.dg != nullptr};
+                        code.D, code.dg != nullptr};
 }
 
 // s == h_a (+ h_b for w > 1, + h_c for w > 2); a, b and c are columns of the
@@ -2676,6 +2677,35 @@ __device__ __forceinline__ void flush_stage(const uint4 (&sv)[kTriageStageVec], 
     __syncthreads();
 }
 
+// s = H e, sig(s), the original weight and the XOR of the degeneracy words over
+// the lane's set error bits (its packed row in `bits`), for codes of at most D
+// syndrome words: a set bit's D column words, signature and degeneracy word are
+// loaded together.  D = kMaxWords covers every code, with the words past code.D
+// guarded.
+template <int D, class T>
+__device__ __forceinline__ void syndrome_of(const T &tb, int m, const uint32_t LDS *bits, int M32,
+                                            uint32_t (&s)[kMaxWords], uint32_t &sg, uint32_t &dgv, int &ow) {
+    const int lane = lane_id();
+    const int Dc = tb.D;
+    for (int wd = 0; wd < M32; ++wd) {
+        uint32_t b = bits[lane * M32 + wd];
+        ow += __popc(b);
+        while (b) {
+            const int c = 32 * wd + __ffs(b) - 1;
+            b &= b - 1u;
+            uint32_t x[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) x[d] = (D < kMaxWords || d < Dc) ? tb.hcw[d * m + c] : 0u;
+            const uint32_t g = tb.sig[c];
+            const uint32_t h = tb.dg[tb.has_dg ? c : 0];
+#pragma unroll
+            for (int d = 0; d < D; ++d) s[d] ^= x[d];
+            sg ^= g;
+            if (tb.has_dg) dgv ^= h;
+        }
+    }
+}
+
 // One lane per shot, 64 shots per wave: packs the input rows (errors or
 // syndromes, and logicals), forms s = H e and sig(s) as XORs over the set error
 // bits (both maps are GF(2)-linear), searches small first iterations by lane,
@@ -2706,18 +2736,16 @@ __device__ __forceinline__ void triage_groups(const DevCode &code, const DevCall
             if (g == (int)blockIdx.x) flush_stage(sv, sv_vecs, tab);
             // the block's packed rows are consecutive in e_packed
             for (int i = lane; i < cnt * M32; i += 64) call.e_packed[(size_t)base * M32 + i] = bits[i];
-            for (int wd = 0; wd < M32; ++wd) {
-                uint32_t b = bits[lane * M32 + wd];
-                ow += __popc(b);
-                while (b) {
-                    const int c = 32 * wd + __ffs(b) - 1;
-                    b &= b - 1u;
-                    sg ^= tb.sig[c];
-                    if (tb.has_dg) dgv ^= tb.dg[c];
-#pragma unroll
-                    for (int d = 0; d < kMaxWords; ++d)
-                        if (d < code.D) s[d] ^= tb.hcw[d * m + c];
-                }
+            switch (code.D) {
+                case 1: syndrome_of<1>(tb, m, bits, M32, s, sg, dgv, ow); break;
+                case 2: syndrome_of<2>(tb, m, bits, M32, s, sg, dgv, ow); break;
+                case 3: syndrome_of<3>(tb, m, bits, M32, s, sg, dgv, ow); break;
+                case 4: syndrome_of<4>(tb, m, bits, M32, s, sg, dgv, ow); break;
+                case 5: syndrome_of<5>(tb, m, bits, M32, s, sg, dgv, ow); break;
+                case 6: syndrome_of<6>(tb, m, bits, M32, s, sg, dgv, ow); break;
+                case 7: syndrome_of<7>(tb, m, bits, M32, s, sg, dgv, ow); break;
+                case 8: syndrome_of<8>(tb, m, bits, M32, s, sg, dgv, ow); break;
+                default: syndrome_of<kMaxWords>(tb, m, bits, M32, s, sg, dgv, ow); break;
             }
         } else {
             const int D = code.D;
