@@ -293,39 +293,64 @@ __device__ __forceinline__ bool exact_match(const DevCode &code, const uint32_t 
 // Active set for threshold thr: ascending columns with score >= thr, where
 // score[c] = |supp(h_c) & supp(s)| + off[c]  (grand_decoder.cu:352-416, :495-504).
 // Writes the first min(tam, cap) entries; returns the full tam.
+// Active columns at threshold thr (score = |supp(h_c) & supp(s)| + off[c] >= thr)
+// in column order, at most cap stored in A / sigA.  The columns are scored 64 per
+// round, kBuildRounds rounds with their loads (offsets, signatures, column words)
+// issued together.  Returns the number of active columns, or, once that passes
+// cap, some number above cap: every caller treats a count above cap alike (an
+// active set too large, or for the DEM family the reference's fallback to the
+// previous list size), and only the first cap entries are ever kept.
+constexpr int kBuildRounds = 6;
 template <class PW, class PO, class PS>
 __device__ __forceinline__ int build_active_t(PW Hcw, PO off, PS sig, int D, int m, const uint32_t LDS *s, int thr,
                                               int LDS *A, uint32_t LDS *sigA, int cap) {
     const int lane = lane_id();
     int tam = 0;
-    for (int c0 = 0; c0 < m; c0 += 64) {
-        const int c = c0 + lane;
-        const int cc = c < m ? c : m - 1;            // in-bounds column for the loads
-        int sc = off[cc];
+    for (int c0 = 0; c0 < m && tam <= cap; c0 += 64 * kBuildRounds) {
+        int cc[kBuildRounds], sc[kBuildRounds];
+        uint32_t sv[kBuildRounds];
+#pragma unroll
+        for (int r = 0; r < kBuildRounds; ++r) {
+            const int c = c0 + 64 * r + lane;
+            cc[r] = c < m ? c : m - 1;               // in-bounds column for the loads
+            sc[r] = off[cc[r]];
+            sv[r] = sig[cc[r]];
+        }
         // column words four at a time (indices clamped, so the loads issue together)
         for (int d0 = 0; d0 < D; d0 += 4) {
-            uint32_t h[4];
+            uint32_t h[kBuildRounds][4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) h[j] = Hcw[(size_t)((d0 + j < D) ? d0 + j : D - 1) * m + cc];
+            for (int r = 0; r < kBuildRounds; ++r)
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if (d0 + j < D) sc += __popc(h[j] & s[d0 + j]);
+                for (int j = 0; j < 4; ++j) h[r][j] = Hcw[(size_t)((d0 + j < D) ? d0 + j : D - 1) * m + cc[r]];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (d0 + j < D) {
+                    const uint32_t sw = s[d0 + j];
+#pragma unroll
+                    for (int r = 0; r < kBuildRounds; ++r) sc[r] += __popc(h[r][j] & sw);
+                }
+            }
         }
-        const bool on = c < m && sc >= thr;
-        const uint64_t b = ballot(on);
-        if (on) {
-            const int p = tam + lanes_below(b);
-            if (p < cap) { A[p] = c; sigA[p] = sig[c]; }
+#pragma unroll
+        for (int r = 0; r < kBuildRounds; ++r) {
+            const int c = c0 + 64 * r + lane;
+            const bool on = c < m && sc[r] >= thr;
+            const uint64_t b = ballot(on);
+            if (on) {
+                const int p = tam + lanes_below(b);
+                if (p < cap) { A[p] = c; sigA[p] = sv[r]; }
+            }
+            tam += __popcll(b);
         }
-        tam += __popcll(b);
     }
     __syncthreads();
     return tam;
 }
-
-// Active list at threshold thr into A / sigA (the wave's LDS).
-__device__ int build_active_into(const DevCode &code, const uint32_t LDS *s, int thr, int LDS *A, uint32_t LDS *sigA,
-                                 int cap) {
+// Active list at threshold thr into A / sigA (the wave's LDS).  Out of line: its
+// rounds of loads in flight would otherwise add to the registers of every caller.
+__device__ __attribute__((noinline)) int build_active_into(const DevCode &code, const uint32_t LDS *s, int thr,
+                                                           int LDS *A, uint32_t LDS *sigA, int cap) {
     if (code.lds_words)
         return build_active_t(code_lds(), (const int LDS *)code_lds_off(code), code_lds_sig(code), code.D, code.m, s,
                               thr, A, sigA, cap);
