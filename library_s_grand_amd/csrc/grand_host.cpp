@@ -590,14 +590,15 @@ int kind_search(int kind) {
 // stream (BB144 prob_cc, 100k shots, per decode: 146.6 us on one stream and
 // 84.7 us on two, against 150.9 and 120.0 us at 2 waves per CU).  Runs are
 // shared above 65536 prefix steps of remaining work (144.4 and 82.3 us; 32768:
-// 146.6 and 85.5 us; 131072: 171.7 and 92.5 us).  ML: 8 waves
-// per CU (the DEM ML workload, tuned: 10.1 M shots/s, against 5.2 M with the
-// former set and 2.8 M with the first-hit set).
+// 146.6 and 85.5 us; 131072: 171.7 and 92.5 us).  ML: 4 waves
+// per CU, whose LDS share holds the pair table the deep classes are walked
+// by (the DEM ML workload, 1M shots: 14.4 M shots/s, against 13.2 M at 8
+// waves per CU, where the table does not fit).
 // The pair-table LDS (pair_table_kb: 0 = the wave's whole remaining share,
-// -1 = none, k = at most k KiB) applies to the first-hit kinds only.
+// -1 = none, k = at most k KiB) applies to both.
 struct Tuning { int waves_per_cu; int claim_batch; int rmax; int split_threshold; int pair_table_kb; };
 constexpr Tuning kTuningFirstHit{1, 2, 64, 65536, 0};
-constexpr Tuning kTuningMl{8, 2, 64, 32768, 0};
+constexpr Tuning kTuningMl{4, 2, 64, 32768, 0};
 Tuning g_tuning = kTuningFirstHit;
 Tuning g_tuning_ml = kTuningMl;
 const Tuning &tuning_for(int search) { return search == kML ? g_tuning_ml : g_tuning; }
@@ -892,12 +893,12 @@ int decode_view(GrandSession *s, const WsView &w, const int *d_input, int input_
     // one claimed shot in the search kernel's LDS: id, signature, syndrome words,
     // then the packed errors (cc with errors) or the observed logicals (cln)
     c.stage_words = 2 + code.D + (c.input_errors ? 2 + code.M32 : 0) + (s->family == kCLN ? code.NL32 : 0);
-    // search-kernel pair table (first-hit kinds): what is left of the wave's share
+    // search-kernel pair table (deep first-hit and ML classes): what is left of the wave's share
     // of the CU's LDS (at most 64 KB per workgroup) after everything else
     const int wpc = (s->ex.waves_per_cu > 0) ? s->ex.waves_per_cu : tuning_for(s->search).waves_per_cu;
     c.pair_cap = 0;
-    if (s->ex.engine != GRAND_ENGINE_PLAIN && s->search != kML) {
-        const size_t base = grand_fused_lds_bytes(c.cap, c.lscores != nullptr) +
+    if (s->ex.engine != GRAND_ENGINE_PLAIN) {
+        const size_t base = grand_fused_lds_bytes(c.cap, s->search == kML || c.lscores != nullptr) +
                             4 * ((size_t)((s->code->search_lds_words + 3) & ~3) + (size_t)c.claim_batch * c.stage_words) +
                             4 * ((size_t)kPairBitmapWords + kPairHeadWords);
         const size_t budget = std::min<size_t>(65536, kLdsPerCu / (size_t)std::max(wpc, 1));

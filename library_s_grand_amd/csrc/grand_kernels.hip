@@ -560,6 +560,7 @@ __device__ __forceinline__ PrefHit scan_prefix(const DevCode &code, const uint32
     for (int j = 0; j < len; ++j) {
         const uint32_t t = acc[WP - 1];
         const int last = c[WP - 1];
+        const uint32_t nxt = sg[last + 1 < M ? last + 1 : last];   // the successor's signature, read ahead
         const unsigned bi = t & bmsk;
         unsigned pos = ((bm[bi >> 5] >> (bi & 31u)) & 1u) ? (t & hm) : 0xFFFFFFFFu;
         for (; pos != 0xFFFFFFFFu;) {
@@ -591,7 +592,7 @@ __device__ __forceinline__ PrefHit scan_prefix(const DevCode &code, const uint32
         // lexicographic successor among WP-subsets of [0, M)
         c[WP - 1] += 1;
         if (c[WP - 1] < M) {
-            acc[WP - 1] = ((WP >= 2) ? acc[(WP >= 2) ? WP - 2 : 0] : t0) ^ sg[c[WP - 1]];
+            acc[WP - 1] = ((WP >= 2) ? acc[(WP >= 2) ? WP - 2 : 0] : t0) ^ nxt;
         } else {
             int i = -1;
 #pragma unroll
@@ -825,6 +826,131 @@ __device__ __forceinline__ PrefHit scan_stems_first(const DevCode &code, const u
     return h;
 }
 
+// ML probe of one (w-2)-stem of cost scost (its LLRs summed in order): among the
+// pairs (y, x), last < y < x, that complete the syndrome exactly, the cheapest
+// (scost + llr[y] + llr[x] in that order, fp32 round-to-nearest; equal costs:
+// the lower (y, x), i.e. the lower rank within the stem).  Returns (cost bits <<
+// 32) | y << 16 | x, or ~0 for none.
+template <int WS>
+__device__ __attribute__((noinline)) uint64_t probe_pairs_ml(const DevCode &code, const uint32_t LDS *s, uint32_t T,
+                                                             int last, float scost, const int LDS *act,
+                                                             const float LDS *llr, const uint32_t LDS *pkey,
+                                                             const uint32_t LDS *pval, const uint32_t LDS *phead,
+                                                             int p0, int p1, int p2, int p3) {
+    const int pc[4] = {p0, p1, p2, p3};
+    unsigned e = phead[(T >> 16) & (unsigned)(kPairBuckets - 1)];
+    int by = -1, bx = -1;
+    float bc = 0.0f;
+    while (e != kPairNone) {
+        const uint32_t v = pval[e];
+        const uint32_t key = pkey[e];
+        e = v >> 20;
+        if (key != T) continue;
+        const int y = (int)(v & 0x3FFu), x = (int)((v >> 10) & 0x3FFu);
+        if (y <= last) continue;
+        int cols[WS + 2];
+#pragma unroll
+        for (int l = 0; l < WS; ++l) cols[l] = act[pc[l]];
+        cols[WS] = act[y];
+        cols[WS + 1] = act[x];
+        if (!exact_match<WS + 2>(code, s, cols, WS + 2)) continue;
+        const float cost = __fadd_rn(__fadd_rn(scost, llr[y]), llr[x]);
+        if (by < 0 || cost < bc || (cost == bc && (y < by || (y == by && x < bx)))) { by = y; bx = x; bc = cost; }
+    }
+    return by < 0 ? ~0ull : (((uint64_t)__float_as_uint(bc) << 32) | ((uint64_t)(uint32_t)by << 16) | (uint32_t)bx);
+}
+
+// ML walk of a weight-(WS + 2) class by its (WS)-stems over the pair table:
+// every stem of the lane's range (positions q0 .. q0 + len - 1 of the
+// lexicographic stems of [0, tam - 2)) is probed; the cheapest pattern wins,
+// equal costs the lowest rank.  The stem's signature and cost are kept per
+// element, the next element's signature and LLR read ahead.
+template <int WS>
+__device__ __forceinline__ PrefHit scan_stems_ml(const DevCode &code, const uint32_t LDS *s, uint32_t t0, uint64_t q0,
+                                                 int len, int tam, const int LDS *act, const uint32_t LDS *sg,
+                                                 const float LDS *llr, const uint32_t LDS *pkey,
+                                                 const uint32_t LDS *pval, const uint32_t LDS *phead,
+                                                 const uint32_t LDS *pbm) {
+    PrefHit h{~0ull, 0.0f, -1};
+    if (len <= 0) return h;
+    const int M = tam - 2;
+    int c[WS];
+    const bool small = tam < kNcrLdsRows;
+    if (small) unrank_t<WS>(q0, M, ncr_lds(), c); else unrank_t<WS>(q0, M, code.nCr, c);
+    uint32_t acc[WS];
+    float cacc[WS];
+    uint32_t a = t0;
+    float ca = 0.0f;
+#pragma unroll
+    for (int l = 0; l < WS; ++l) { a ^= sg[c[l]]; acc[l] = a; ca = __fadd_rn(ca, llr[c[l]]); cacc[l] = ca; }
+    uint32_t hi = (WS >= 2) ? acc[(WS >= 2) ? WS - 2 : 0] : t0;
+    float hic = (WS >= 2) ? cacc[(WS >= 2) ? WS - 2 : 0] : 0.0f;
+    uint32_t t = acc[WS - 1];
+    float sc = cacc[WS - 1];
+    for (int j = 0;;) {
+        const int last = c[WS - 1];
+        const uint32_t nxt = sg[last + 1];            // last + 1 <= M = tam - 2
+        const float nxl = llr[last + 1];
+        if ((pbm[(t & 0x7FFFu) >> 5] >> (t & 31u)) & 1u) {
+            const uint64_t v = probe_pairs_ml<WS>(code, s, t, last, sc, act, llr, pkey, pval, phead, c[0],
+                                                  WS > 1 ? c[WS > 1 ? 1 : 0] : 0, WS > 2 ? c[WS > 2 ? 2 : 0] : 0,
+                                                  WS > 3 ? c[WS > 3 ? 3 : 0] : 0);
+            if (v != ~0ull) {
+                int cc[WS + 1];
+#pragma unroll
+                for (int l = 0; l < WS; ++l) cc[l] = c[l];
+                cc[WS] = (int)((v >> 16) & 0xFFFFu);
+                const int x = (int)(v & 0xFFFFu);
+                const float cost = __uint_as_float((uint32_t)(v >> 32));
+                const uint64_t r = small ? rank_with_last<WS + 1>(cc, x, tam, ncr_lds())
+                                         : rank_with_last<WS + 1>(cc, x, tam, code.nCr);
+                if (h.j < 0 || cost < h.cost || (cost == h.cost && r < h.rank)) { h.rank = r; h.cost = cost; h.j = j; }
+            }
+        }
+        if (++j == len) break;
+        if (last + 1 < M) {
+            c[WS - 1] = last + 1;
+            t = hi ^ nxt;
+            sc = __fadd_rn(hic, nxl);
+        } else {
+            int i = -1;
+#pragma unroll
+            for (int l = 0; l < WS - 1; ++l) i = (c[l] < M - WS + l) ? l : i;
+            int nc[WS];
+#pragma unroll
+            for (int l = 0; l < WS; ++l)
+                nc[l] = (l < i) ? c[l] : (l == i) ? c[l] + 1 : (l == 0) ? c[0] : nc[(l == 0) ? 0 : l - 1] + 1;
+            uint32_t v[WS];
+            float lv[WS];
+#pragma unroll
+            for (int l = 0; l < WS; ++l) { v[l] = sg[nc[l]]; lv[l] = llr[nc[l]]; }
+            uint32_t a2 = t0;
+            float c2 = 0.0f;
+#pragma unroll
+            for (int l = 0; l < WS; ++l) {
+                a2 ^= v[l]; acc[l] = a2; c2 = __fadd_rn(c2, lv[l]); cacc[l] = c2; c[l] = nc[l];
+            }
+            hi = (WS >= 2) ? acc[(WS >= 2) ? WS - 2 : 0] : t0;
+            hic = (WS >= 2) ? cacc[(WS >= 2) ? WS - 2 : 0] : 0.0f;
+            t = acc[WS - 1];
+            sc = cacc[WS - 1];
+        }
+    }
+    return h;
+}
+__device__ __attribute__((noinline)) PrefHit scan_stems_ml_w(int w, const DevCode &code, const uint32_t LDS *s,
+                                                            uint32_t t0, uint64_t q0, int len, int tam,
+                                                            const int LDS *act, const uint32_t LDS *sg,
+                                                            const float LDS *llr, const uint32_t LDS *pkey,
+                                                            const uint32_t LDS *pval, const uint32_t LDS *phead,
+                                                            const uint32_t LDS *pbm) {
+    switch (w) {
+        case 3: return scan_stems_ml<1>(code, s, t0, q0, len, tam, act, sg, llr, pkey, pval, phead, pbm);
+        case 4: return scan_stems_ml<2>(code, s, t0, q0, len, tam, act, sg, llr, pkey, pval, phead, pbm);
+        case 5: return scan_stems_ml<3>(code, s, t0, q0, len, tam, act, sg, llr, pkey, pval, phead, pbm);
+        default: return scan_stems_ml<4>(code, s, t0, q0, len, tam, act, sg, llr, pkey, pval, phead, pbm);
+    }
+}
 __device__ __attribute__((noinline)) PrefHit scan_stems_w(int w, const DevCode &code, const uint32_t LDS *s,
                                                          uint32_t t0, uint64_t q0, int len, int tam,
                                                          const int LDS *act, const uint32_t LDS *sg,
@@ -997,14 +1123,16 @@ __device__ __forceinline__ int class_chunk_r(const DevCall &call, uint64_t units
 // cch (lane i: first chunk of class i, lane nclass: the chunk count) and cR
 // (lane i: units per lane per chunk of class i); bq is a recent best_q unit
 // index of generation gen (~0: none known).
+template <bool ML>
 __device__ unsigned long long task_chunk(const DevCode &code, TaskSlot *T, const uint32_t LDS *s, uint32_t t0,
-                                        int tam, bool ml, unsigned ch, unsigned gen, unsigned long long bq, int nclass,
+                                        int tam, unsigned ch, unsigned gen, unsigned long long bq, int nclass,
                                         unsigned wpack, unsigned ppack, unsigned cch, int cR, const int LDS *act,
                                         const uint32_t LDS *sg, const float LDS *llr,
                                         const unsigned long long LDS *ht, unsigned hmask,
                                         const uint32_t LDS *pkey, const uint32_t LDS *pval,
                                         const uint32_t LDS *phead, const uint32_t LDS *pbm,
                                         unsigned long long *work) {
+    constexpr bool ml = ML;
     const int lane = lane_id();
     int w = 0, R = 1;
     bool pairs = false;
@@ -1028,7 +1156,8 @@ __device__ unsigned long long task_chunk(const DevCode &code, TaskSlot *T, const
     if (!ml && ubase + base > bq) return ~0ull - 1;
     const uint64_t q0 = base + (uint64_t)lane * (uint64_t)R;
     const int len = q0 >= NP ? 0 : (int)((NP - q0) < (uint64_t)R ? (NP - q0) : (uint64_t)R);
-    const PrefHit h = ml ? scan_prefix_w<1>(w, code, s, t0, q0, len, tam, act, sg, llr, ht, hmask)
+    const PrefHit h = ml ? (pairs ? scan_stems_ml_w(w, code, s, t0, q0, len, tam, act, sg, llr, pkey, pval, phead, pbm)
+                                  : scan_prefix_w<1>(w, code, s, t0, q0, len, tam, act, sg, llr, ht, hmask))
                     : pairs ? scan_stems_w(w, code, s, t0, q0, len, tam, act, sg, pkey, pval, phead, pbm)
                             : scan_prefix_w<0>(w, code, s, t0, q0, len, tam, act, sg, llr, ht, hmask);
     count_work(work, pairs ? 1 : 0, walked(h, len, ml));
@@ -1176,8 +1305,9 @@ __device__ int publish_shared(const DevCode &code, const DevCall &call, int shot
 // result in best (~0 = no match): first-hit, the lowest matching pattern index
 // over the classes concatenated in order; ML (one class), the lowest (cost
 // key, rank).
+template <bool ML>
 __device__ void work_shared(const DevCode &code, const DevCall &call, int slot, const uint32_t LDS *s, uint32_t t0,
-                            int tam, bool ml, int nclass, unsigned wpack, unsigned ppack, const int LDS *act,
+                            int tam, int nclass, unsigned wpack, unsigned ppack, const int LDS *act,
                             const uint32_t LDS *sg, const float LDS *llr, const unsigned long long LDS *ht,
                             unsigned hmask, const uint32_t LDS *pkey, const uint32_t LDS *pval,
                             const uint32_t LDS *phead, const uint32_t LDS *pbm, unsigned long long &best) {
@@ -1192,7 +1322,7 @@ __device__ void work_shared(const DevCode &code, const DevCall &call, int slot, 
         unsigned long long bq;
         const unsigned ch = claim_chunk(T, &call.task_counters[0], g, bq);
         if (ch == ~0u) break;
-        (void)task_chunk(code, T, s, t0, tam, ml, ch, g, bq, nclass, wpack, ppack, cchv, myR, act, sg, llr, ht, hmask,
+        (void)task_chunk<ML>(code, T, s, t0, tam, ch, g, bq, nclass, wpack, ppack, cchv, myR, act, sg, llr, ht, hmask,
                          pkey, pval, phead, pbm, call.work);
         wait_mem();                               // the chunk's result atomics complete before it counts as done
         if (lane == 0) atomicAdd(&T->done, 1u);
@@ -1230,14 +1360,15 @@ __device__ void cancel_shared(const DevCall &call, int slot) {
 
 // Owner side: publish a task and work on it; false when no slot is free (the
 // caller searches alone).
+template <bool ML>
 __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, const uint32_t LDS *s, uint32_t t0,
-                           int tam, bool ml, int nclass, unsigned wpack, unsigned ppack, const int LDS *act,
+                           int tam, int nclass, unsigned wpack, unsigned ppack, const int LDS *act,
                            const uint32_t LDS *sg, const float LDS *llr, const unsigned long long LDS *ht, unsigned hmask,
                            const uint32_t LDS *pkey, const uint32_t LDS *pval,
                            const uint32_t LDS *phead, const uint32_t LDS *pbm, unsigned long long &best) {
-    const int slot = publish_shared(code, call, shot, s, t0, tam, ml, nclass, wpack, ppack, act, sg, llr);
+    const int slot = publish_shared(code, call, shot, s, t0, tam, ML, nclass, wpack, ppack, act, sg, llr);
     if (slot < 0) return false;
-    work_shared(code, call, slot, s, t0, tam, ml, nclass, wpack, ppack, act, sg, llr, ht, hmask, pkey, pval, phead,
+    work_shared<ML>(code, call, slot, s, t0, tam, nclass, wpack, ppack, act, sg, llr, ht, hmask, pkey, pval, phead,
                 pbm, best);
     return true;
 }
@@ -1251,13 +1382,14 @@ __device__ bool run_shared(const DevCode &code, const DevCall &call, int shot, c
 struct HelpCache {
     int key;
     unsigned hmask;
-    int tam, ml, nclass;
+    int tam, nclass;
     unsigned wpack, ppack;
     unsigned cch;                     // lane i: first chunk of class i; lane nclass: chunk count
     int cR;                           // lane i: units per lane per chunk of class i
     uint32_t t0;
 };
 
+template <bool ML>
 __device__ __forceinline__ bool help_task(const DevCode &code, const DevCall &call, const WaveLds &L, HelpCache &hc) {
     const int lane = lane_id();
     if (__builtin_amdgcn_readfirstlane(ld_relaxed(&call.task_counters[0])) == 0u) return false;
@@ -1291,7 +1423,6 @@ __device__ __forceinline__ bool help_task(const DevCode &code, const DevCall &ca
         // keep the compiler from moving them above that branch.
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         hc.tam = ld_relaxed(&T->tam);
-        hc.ml = ld_relaxed(&T->ml);
         hc.nclass = ld_relaxed(&T->nclass);
         hc.wpack = (unsigned)ld_relaxed(&T->wpack);
         hc.ppack = (unsigned)ld_relaxed(&T->ppack);
@@ -1300,13 +1431,12 @@ __device__ __forceinline__ bool help_task(const DevCode &code, const DevCall &ca
         hc.cR = (lane < kTaskClasses) ? ld_relaxed(&T->cR[lane]) : 1;
         const uint32_t sword = (lane < kMaxWords) ? ld_relaxed(&T->s[lane]) : 0u;
         hc.tam = uni((unsigned)hc.tam);
-        hc.ml = uni((unsigned)hc.ml);
         hc.nclass = uni((unsigned)hc.nclass);
         hc.wpack = uni(hc.wpack);
         hc.ppack = uni(hc.ppack);
         hc.t0 = uni(hc.t0);
         if (lane < kMaxWords) L.sw[lane] = sword;
-        const bool ml = hc.ml != 0;
+        constexpr bool ml = ML;
         int LDS *act = ml ? L.P : L.A;
         uint32_t LDS *sg = ml ? L.sigP : L.sigA;
         const uint32_t *arr = call.task_arrays + (size_t)slot * 3 * call.cap;
@@ -1321,11 +1451,11 @@ __device__ __forceinline__ bool help_task(const DevCode &code, const DevCall &ca
         if ((hc.ppack & all) != all) hc.hmask = build_ht(L.ht, sg, hc.tam);
         hc.key = key;
     }
-    const bool ml = hc.ml != 0;
+    constexpr bool ml = ML;
     const uint32_t LDS *s = L.sw;
     const int LDS *act = ml ? L.P : L.A;
     const uint32_t LDS *sg = ml ? L.sigP : L.sigA;
-    (void)task_chunk(code, T, s, hc.t0, hc.tam, ml, ch, g, bq, hc.nclass, hc.wpack, hc.ppack, hc.cch, hc.cR, act, sg,
+    (void)task_chunk<ML>(code, T, s, hc.t0, hc.tam, ch, g, bq, hc.nclass, hc.wpack, hc.ppack, hc.cch, hc.cR, act, sg,
                      ml ? L.llrP : nullptr, L.ht, hc.hmask, L.pkey, L.pval, L.phead, L.pbm, call.work);
     wait_mem();                                   // the chunk's result atomics complete before it counts as done
     if (lane == 0) atomicAdd(&T->done, 1u);
@@ -1534,7 +1664,7 @@ __device__ int first_hit_wave(const DevCode &code, const DevCall &call, int shot
                         const bool left = ctl_next(snap) < ctl_nch(snap);
                         if (left && need_pairs && !have_pairs) { build_pairs(L, L.sigA, tam); have_pairs = true; }
                         if (left && need_ht && !have_table) { hmask = build_ht(L.ht, L.sigA, tam); have_table = true; }
-                        work_shared(code, call, spec, s, t0, tam, false, nc, wpack, ppack, L.A, L.sigA, nullptr, L.ht,
+                        work_shared<false>(code, call, spec, s, t0, tam, nc, wpack, ppack, L.A, L.sigA, nullptr, L.ht,
                                     hmask, L.pkey, L.pval, L.phead, L.pbm, best);
                         shared = true;
                     } else {
@@ -1547,7 +1677,7 @@ __device__ int first_hit_wave(const DevCode &code, const DevCall &call, int shot
                     // the list's tables, for this wave's own chunks of the run
                     if (need_pairs && !have_pairs) { build_pairs(L, L.sigA, tam); have_pairs = true; }
                     if (need_ht && !have_table) { hmask = build_ht(L.ht, L.sigA, tam); have_table = true; }
-                    shared = run_shared(code, call, shot, s, t0, tam, false, nc, wpack, ppack, L.A, L.sigA, nullptr, L.ht,
+                    shared = run_shared<false>(code, call, shot, s, t0, tam, nc, wpack, ppack, L.A, L.sigA, nullptr, L.ht,
                                         hmask, L.pkey, L.pval, L.phead, L.pbm, best);
                 }
                 if (shared) {
@@ -1618,6 +1748,7 @@ __device__ bool ml_wave(const DevCode &code, const DevCall &call, int shot, cons
     const int lane = lane_id();
     sort_by_llr(code, L, tam);
     const unsigned hmask = build_ht(L.ht, L.sigP, tam);
+    bool have_pairs = false;
     const int max_w = call.W < tam ? call.W : tam;
     float prefix[kMaxW + 1];
     prefix[0] = 0.0f;
@@ -1639,11 +1770,16 @@ __device__ bool ml_wave(const DevCode &code, const DevCall &call, int shot, cons
             count_work(call.work, 2, lane == 0 ? (uint32_t)tam : 0u);
             if (h.j >= 0) { chit = true; cbest = h.cost; crank = h.rank; }
         } else {
-            const uint64_t NP = ncr_w(code, tam - 1, w - 1);
+            // deep classes by (w-2)-stems over the pair table of the sorted list
+            // (built once per list), the others by (w-1)-prefixes
+            const bool pairs = use_pairs(code, call, w, tam);
+            if (pairs && !have_pairs) { build_pairs(L, L.sigP, tam); have_pairs = true; }
+            const uint64_t NP = class_units(code, tam, w, pairs);
             unsigned long long shared_best;
-            if (call.split_threshold && NP > call.split_threshold && Nw < (1ull << 32) &&
-                run_shared(code, call, shot, s, t0, tam, true, 1, (unsigned)w, 0u, L.P, L.sigP, L.llrP, L.ht, hmask,
-                           nullptr, nullptr, nullptr, nullptr, shared_best)) {
+            if (call.split_threshold && NP * (pairs ? (uint64_t)kStemCost : 1u) > call.split_threshold &&
+                Nw < (1ull << 32) &&
+                run_shared<true>(code, call, shot, s, t0, tam, 1, (unsigned)w, pairs ? 1u : 0u, L.P, L.sigP, L.llrP,
+                           L.ht, hmask, L.pkey, L.pval, L.phead, L.pbm, shared_best)) {
                 if (shared_best != ~0ull) {
                     chit = true;
                     cbest = key_to_float((uint32_t)(shared_best >> 32));
@@ -1657,8 +1793,11 @@ __device__ bool ml_wave(const DevCode &code, const DevCall &call, int shot, cons
                     const int R = (int)(r64 < (uint64_t)call.rmax ? r64 : (uint64_t)call.rmax);
                     const uint64_t q0 = pos + (uint64_t)lane * (uint64_t)R;
                     const int len = q0 >= NP ? 0 : (int)((NP - q0) < (uint64_t)R ? (NP - q0) : (uint64_t)R);
-                    const PrefHit h = scan_prefix_w<1>(w, code, s, t0, q0, len, tam, L.P, L.sigP, L.llrP, L.ht, hmask);
-                    count_work(call.work, 0, walked(h, len, true));
+                    const PrefHit h = pairs ? scan_stems_ml_w(w, code, s, t0, q0, len, tam, L.P, L.sigP, L.llrP, L.pkey,
+                                                              L.pval, L.phead, L.pbm)
+                                            : scan_prefix_w<1>(w, code, s, t0, q0, len, tam, L.P, L.sigP, L.llrP, L.ht,
+                                                               hmask);
+                    count_work(call.work, pairs ? 1 : 0, walked(h, len, true));
                     const uint32_t key = (h.j >= 0) ? order_key(h.cost) : 0xFFFFFFFFu;
                     const uint32_t kmin = wave_min(key);
                     const uint64_t b = ballot(h.j >= 0 && key == kmin);
@@ -2040,6 +2179,7 @@ __device__ __attribute__((noinline)) int publish_iteration3(const DevCode &code,
     return publish_shared(code, call, shot, s, t0, tam3, false, nc, wpack, ppack, L.A, L.sigA, nullptr);
 }
 
+template <bool ML>
 __device__ __forceinline__ void decode_shot_wave(const DevCode &code, const DevCall &call, const uint32_t LDS *slot,
                                                  const WaveLds &L) {
     ShotOutcome o;
@@ -2055,9 +2195,9 @@ __device__ __forceinline__ void decode_shot_wave(const DevCode &code, const DevC
     for (int l = 0; l < kMaxWin; ++l) o.win[l] = -1;
     int it = 0, tam = 0, tam_old = 0;
     int spec = -1;                                 // the third iteration's run, published early (slot)
-    const bool spec_ok = call.kind != kML && call.family == kCC && call.lscores == nullptr && call.split_threshold &&
+    const bool spec_ok = !ML && call.family == kCC && call.lscores == nullptr && call.split_threshold &&
                          code.max_soma >= 3;
-    const bool learned_hit = call.lscores && call.kind != kML && learned_pass(code, call, shot, s, t0, L, o);
+    const bool learned_hit = !ML && call.lscores && learned_pass(code, call, shot, s, t0, L, o);
     while (!learned_hit && it < code.max_soma && it < 3) {
         const int thr = code.max_soma - it;
         ++it;
@@ -2080,7 +2220,7 @@ __device__ __forceinline__ void decode_shot_wave(const DevCode &code, const DevC
         }
         o.hibrido = (br > 0) ? (o.hibrido == -1 ? 1 : 0) : -1;
         bool hit;
-        if (call.kind == kML) {
+        if constexpr (ML) {
             hit = ml_wave(code, call, shot, s, t0, tam, L, o.win, o.fw, o.count, o.wops);
         } else {
             int wl[2 * kMaxW];
@@ -2896,10 +3036,11 @@ __global__ __launch_bounds__(64) void triage_kernel(const DevCode *__restrict__ 
 }
 
 // Persistent search kernel: waves claim worklist entries in batches until none remain.
+template <bool ML>
 __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ codep, const DevCall *__restrict__ callp) {
     const DevCode &code = *codep;
     const DevCall &call = *callp;
-    const WaveLds L = carve_lds(call.cap, call.kind == kML || call.lscores != nullptr, code.lds_words,
+    const WaveLds L = carve_lds(call.cap, ML || call.lscores != nullptr, code.lds_words,
                                 call.claim_batch * call.stage_words, call.pair_cap);
     if (code.lds_words > 0) {
         uint32_t LDS *dst = lds_base() + kLdsCodeOffset;
@@ -2995,7 +3136,7 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
         bool helped = false;
         const bool have_claim = !drained && got < total && (unsigned)cur + kClaimWords * got < total;
         if (open != 0u && !have_claim)
-            while (help_task(code, call, L, hc)) helped = true;
+            while (help_task<ML>(code, call, L, hc)) helped = true;
         if (!drained) {
             ++claims;
             const unsigned p0 = (unsigned)cur + kClaimWords * got;
@@ -3003,7 +3144,7 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
                 const unsigned left = (total - 1u - p0) / kClaimWords + 1u;   // positions p0, p0 + 8, ... below total
                 const int nb = (int)(left < B ? left : B);
                 stage_claim(code, call, p0, nb, heavy, wfx, wbx, L);
-                for (int j = 0; j < nb; ++j) decode_shot_wave(code, call, L.stage + j * call.stage_words, L);
+                for (int j = 0; j < nb; ++j) decode_shot_wave<ML>(code, call, L.stage + j * call.stage_words, L);
                 hc.key = -1;
                 continue;
             }
@@ -3322,7 +3463,10 @@ GRAND_INTERNAL hipError_t grand_launch_fused(const DevCode *code, const DevCall 
                                            (hcall->pair_cap > 0 ? (size_t)grand::kPairBitmapWords + grand::kPairHeadWords +
                                                                       2 * (size_t)hcall->pair_cap
                                                                 : 0));
-    hipLaunchKernelGGL(grand::fused_kernel, dim3(grid), dim3(64), lds, st, code, call);
+    if (hcall->kind == grand::kML)
+        hipLaunchKernelGGL(grand::fused_kernel<true>, dim3(grid), dim3(64), lds, st, code, call);
+    else
+        hipLaunchKernelGGL(grand::fused_kernel<false>, dim3(grid), dim3(64), lds, st, code, call);
     return hipGetLastError();
 }
 

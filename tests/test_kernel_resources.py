@@ -43,10 +43,11 @@ def test_search_kernel_does_not_spill(kernel_asm):
         priv = int(re.search(r"\.amdhsa_private_segment_fixed_size (\d+)", m.group(2)).group(1))
         vgpr = int(re.search(r"\.amdhsa_next_free_vgpr (\d+)", m.group(2)).group(1))
         kernels[m.group(1)] = (priv, vgpr)
+    # the search kernel is compiled twice: first-hit and ML
     fused = [v for k, v in kernels.items() if "fused_kernel" in k]
     triage = [v for k, v in kernels.items() if "triage_kernel" in k]
-    assert fused and triage
-    assert fused[0][0] <= 16 and fused[0][1] <= 256, fused
+    assert len(fused) == 2 and triage
+    assert all(priv <= 16 and vgpr <= 256 for priv, vgpr in fused), fused
     assert triage[0][0] == 0, triage
 
 
