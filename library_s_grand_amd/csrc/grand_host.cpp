@@ -592,13 +592,14 @@ int kind_search(int kind) {
 // shared above 65536 prefix steps of remaining work (144.4 and 82.3 us; 32768:
 // 146.6 and 85.5 us; 131072: 171.7 and 92.5 us).  ML: 4 waves
 // per CU, whose LDS share holds the pair table the deep classes are walked
-// by (the DEM ML workload, 1M shots: 14.4 M shots/s, against 13.2 M at 8
-// waves per CU, where the table does not fit).
+// by, one shot per claim, runs shared above 65536 prefix steps (the DEM ML
+// workload, 1M shots: 15.5 M shots/s; claim batch 2 and split 32768: 14.5 M;
+// 8 waves per CU, where the table does not fit: 13.2 M).
 // The pair-table LDS (pair_table_kb: 0 = the wave's whole remaining share,
 // -1 = none, k = at most k KiB) applies to both.
 struct Tuning { int waves_per_cu; int claim_batch; int rmax; int split_threshold; int pair_table_kb; };
 constexpr Tuning kTuningFirstHit{1, 2, 64, 65536, 0};
-constexpr Tuning kTuningMl{4, 2, 64, 32768, 0};
+constexpr Tuning kTuningMl{4, 1, 64, 65536, 0};
 Tuning g_tuning = kTuningFirstHit;
 Tuning g_tuning_ml = kTuningMl;
 const Tuning &tuning_for(int search) { return search == kML ? g_tuning_ml : g_tuning; }
