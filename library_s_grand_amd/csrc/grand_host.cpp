@@ -593,7 +593,8 @@ int kind_search(int kind) {
 // 146.6 and 85.5 us; 131072: 171.7 and 92.5 us).  ML: 4 waves
 // per CU, whose LDS share holds the pair table the deep classes are walked
 // by, one shot per claim, runs shared above 65536 prefix steps (the DEM ML
-// workload, 1M shots: 15.5 M shots/s; claim batch 2 and split 32768: 14.5 M;
+// workload, 1M shots, with 2048 pair bucket heads: 15.5 M shots/s; claim batch
+// 2 and split 32768: 14.5 M;
 // 8 waves per CU, where the table does not fit: 13.2 M).
 // The pair-table LDS (pair_table_kb: 0 = the wave's whole remaining share,
 // -1 = none, k = at most k KiB) applies to both.
@@ -901,7 +902,7 @@ int decode_view(GrandSession *s, const WsView &w, const int *d_input, int input_
     if (s->ex.engine != GRAND_ENGINE_PLAIN) {
         const size_t base = grand_fused_lds_bytes(c.cap, s->search == kML || c.lscores != nullptr) +
                             4 * ((size_t)((s->code->search_lds_words + 3) & ~3) + (size_t)c.claim_batch * c.stage_words) +
-                            4 * ((size_t)kPairBitmapWords + kPairHeadWords);
+                            4 * ((size_t)kPairBitmapWords + pair_head_words(s->search == kML));
         const size_t budget = std::min<size_t>(65536, kLdsPerCu / (size_t)std::max(wpc, 1));
         const long long all = (long long)c.cap * (c.cap - 1) / 2;
         const int kb = (s->ex.pair_table_kb != 0) ? s->ex.pair_table_kb : tuning_for(s->search).pair_table_kb;

@@ -170,7 +170,7 @@ struct WaveLds {
     unsigned long long LDS *ht;   // (signature << 32) | position, empty = kHtEmpty
     uint32_t LDS *stage;       // search kernel: the claimed shots, DevCall::stage_words each
     uint32_t LDS *pbm;         // pair table (search kernel, pair_cap > 0): key filter [kPairBitmapWords],
-    uint32_t LDS *phead;       //   bucket chain heads [kPairBuckets], keys and (y | x << 10 | next << 20)
+    uint32_t LDS *phead;       //   bucket chain heads [pair_buckets(kind)], keys and (y | x << 10 | next << 20)
     uint32_t LDS *pkey;
     uint32_t LDS *pval;
     uint32_t LDS *sw;  // syndrome words of the shot being searched, kMaxWords
@@ -208,7 +208,7 @@ constexpr int kLdsNcrOffset = 2 * kMaxWords;
 constexpr int kLdsCodeOffset = kLdsNcrOffset + 2 * kNcrLdsRows * kNcrStride;
 
 __device__ __forceinline__ WaveLds carve_lds(int cap, bool ml, int code_words = 0, int batch_words = 0,
-                                             int pair_cap = 0) {
+                                             int pair_cap = 0, int head_words = pair_head_words(false)) {
     uint32_t LDS *base = lds_base();
     WaveLds L;
     L.sw = base;
@@ -230,7 +230,7 @@ __device__ __forceinline__ WaveLds carve_lds(int cap, bool ml, int code_words = 
     }
     if (pair_cap > 0) {
         L.pbm = p; p += kPairBitmapWords;
-        L.phead = p; p += kPairHeadWords;
+        L.phead = p; p += head_words;
         L.pkey = p; p += pair_cap;
         L.pval = p; p += pair_cap;
     } else {
@@ -659,13 +659,14 @@ __device__ __attribute__((noinline)) PrefHit scan_prefix_learned(int w, const De
 // kPairNone entries); pbm has the bit of every key's low 15 bits.  Each lane
 // takes a contiguous block of the C(tam, 2) pairs in lexicographic order and
 // pushes each on its bucket's chain with one atomic exchange.
+template <int NB>
 __device__ __attribute__((noinline)) void build_pairs_lds(uint32_t LDS *phead, uint32_t LDS *pbm, uint32_t LDS *pkey,
                                                         uint32_t LDS *pval, const uint32_t LDS *sg, int tam) {
     const int lane = lane_id();
-    static_assert(kPairBuckets % 256 == 0 && kPairBitmapWords % 256 == 0, "clearing in 4-word stores");
+    static_assert(NB % 256 == 0 && kPairBitmapWords % 256 == 0, "clearing in 4-word stores");
     const uint64_t none2 = ((uint64_t)kPairNone << 32) | kPairNone;
 #pragma unroll
-    for (int i = 0; i < kPairBuckets / 256; ++i)
+    for (int i = 0; i < NB / 256; ++i)
         *reinterpret_cast<uint64_t LDS *>(phead + 4 * (lane + 64 * i)) = none2,
         *reinterpret_cast<uint64_t LDS *>(phead + 4 * (lane + 64 * i) + 2) = none2;
 #pragma unroll
@@ -706,14 +707,14 @@ __device__ __attribute__((noinline)) void build_pairs_lds(uint32_t LDS *phead, u
         }
         // Unconditional atomics (a conditional one becomes a branch that waits for
         // its result): past the block's end a lane ORs 0 into the filter and
-        // exchanges on the spare head kPairBuckets, which no lookup reads.
+        // exchanges on the spare head NB, which no lookup reads.
         uint32_t nxt[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const bool ok = k + u < k1;
             __hip_atomic_fetch_or(&pbm[(key[u] & 0x7FFFu) >> 5], ok ? 1u << (key[u] & 31u) : 0u, __ATOMIC_RELAXED,
                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-            uint32_t LDS *hp = &phead[ok ? (key[u] >> 16) & (unsigned)(kPairBuckets - 1) : (unsigned)kPairBuckets];
+            uint32_t LDS *hp = &phead[ok ? (key[u] >> 16) & (unsigned)(NB - 1) : (unsigned)NB];
             nxt[u] = __hip_atomic_exchange(hp, (uint32_t)(k + u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
 #pragma unroll
@@ -726,8 +727,9 @@ __device__ __attribute__((noinline)) void build_pairs_lds(uint32_t LDS *phead, u
     __syncthreads();
 }
 
+template <bool ML>
 __device__ __forceinline__ void build_pairs(const WaveLds &L, const uint32_t LDS *sg, int tam) {
-    build_pairs_lds(L.phead, L.pbm, L.pkey, L.pval, sg, tam);
+    build_pairs_lds<pair_buckets(ML)>(L.phead, L.pbm, L.pkey, L.pval, sg, tam);
 }
 
 // Lexicographically smallest pair (y, x), y > last, with key T whose pattern
@@ -738,7 +740,7 @@ __device__ __attribute__((noinline)) int probe_pairs(const DevCode &code, const 
                                                     const uint32_t LDS *pval, const uint32_t LDS *phead, int p0,
                                                     int p1, int p2, int p3) {
     const int pc[4] = {p0, p1, p2, p3};
-    unsigned e = phead[(T >> 16) & (unsigned)(kPairBuckets - 1)];
+    unsigned e = phead[(T >> 16) & (unsigned)(kPairBucketsFirstHit - 1)];
     int best = -1;
     while (e != kPairNone) {
         const uint32_t v = pval[e];
@@ -838,7 +840,7 @@ __device__ __attribute__((noinline)) uint64_t probe_pairs_ml(const DevCode &code
                                                              const uint32_t LDS *pval, const uint32_t LDS *phead,
                                                              int p0, int p1, int p2, int p3) {
     const int pc[4] = {p0, p1, p2, p3};
-    unsigned e = phead[(T >> 16) & (unsigned)(kPairBuckets - 1)];
+    unsigned e = phead[(T >> 16) & (unsigned)(kPairBucketsMl - 1)];
     int by = -1, bx = -1;
     float bc = 0.0f;
     while (e != kPairNone) {
@@ -1447,7 +1449,7 @@ __device__ __forceinline__ bool help_task(const DevCode &code, const DevCall &ca
         }
         __syncthreads();
         const unsigned all = (hc.nclass >= 32) ? ~0u : ((1u << hc.nclass) - 1u);
-        if ((hc.ppack & all) != 0u) build_pairs(L, sg, hc.tam);
+        if ((hc.ppack & all) != 0u) build_pairs<ML>(L, sg, hc.tam);
         if ((hc.ppack & all) != all) hc.hmask = build_ht(L.ht, sg, hc.tam);
         hc.key = key;
     }
@@ -1662,7 +1664,7 @@ __device__ int first_hit_wave(const DevCode &code, const DevCall &call, int shot
                         // claimed them all; the claims never come back)
                         const unsigned long long snap = uni64(ld_relaxed(&T->ctl));
                         const bool left = ctl_next(snap) < ctl_nch(snap);
-                        if (left && need_pairs && !have_pairs) { build_pairs(L, L.sigA, tam); have_pairs = true; }
+                        if (left && need_pairs && !have_pairs) { build_pairs<false>(L, L.sigA, tam); have_pairs = true; }
                         if (left && need_ht && !have_table) { hmask = build_ht(L.ht, L.sigA, tam); have_table = true; }
                         work_shared<false>(code, call, spec, s, t0, tam, nc, wpack, ppack, L.A, L.sigA, nullptr, L.ht,
                                     hmask, L.pkey, L.pval, L.phead, L.pbm, best);
@@ -1675,7 +1677,7 @@ __device__ int first_hit_wave(const DevCode &code, const DevCall &call, int shot
                 }
                 if (!shared) {
                     // the list's tables, for this wave's own chunks of the run
-                    if (need_pairs && !have_pairs) { build_pairs(L, L.sigA, tam); have_pairs = true; }
+                    if (need_pairs && !have_pairs) { build_pairs<false>(L, L.sigA, tam); have_pairs = true; }
                     if (need_ht && !have_table) { hmask = build_ht(L.ht, L.sigA, tam); have_table = true; }
                     shared = run_shared<false>(code, call, shot, s, t0, tam, nc, wpack, ppack, L.A, L.sigA, nullptr, L.ht,
                                         hmask, L.pkey, L.pval, L.phead, L.pbm, best);
@@ -1702,7 +1704,7 @@ __device__ int first_hit_wave(const DevCode &code, const DevCall &call, int shot
                 }
             }
             if (pairs) {
-                if (!have_pairs) { build_pairs(L, L.sigA, tam); have_pairs = true; }
+                if (!have_pairs) { build_pairs<false>(L, L.sigA, tam); have_pairs = true; }
             } else if (!have_table && w > 1) {
                 hmask = build_ht(L.ht, L.sigA, tam);
                 have_table = true;
@@ -1773,7 +1775,7 @@ __device__ bool ml_wave(const DevCode &code, const DevCall &call, int shot, cons
             // deep classes by (w-2)-stems over the pair table of the sorted list
             // (built once per list), the others by (w-1)-prefixes
             const bool pairs = use_pairs(code, call, w, tam);
-            if (pairs && !have_pairs) { build_pairs(L, L.sigP, tam); have_pairs = true; }
+            if (pairs && !have_pairs) { build_pairs<true>(L, L.sigP, tam); have_pairs = true; }
             const uint64_t NP = class_units(code, tam, w, pairs);
             unsigned long long shared_best;
             if (call.split_threshold && NP * (pairs ? (uint64_t)kStemCost : 1u) > call.split_threshold &&
@@ -3041,7 +3043,7 @@ __global__ __launch_bounds__(64) void fused_kernel(const DevCode *__restrict__ c
     const DevCode &code = *codep;
     const DevCall &call = *callp;
     const WaveLds L = carve_lds(call.cap, ML || call.lscores != nullptr, code.lds_words,
-                                call.claim_batch * call.stage_words, call.pair_cap);
+                                call.claim_batch * call.stage_words, call.pair_cap, pair_head_words(ML));
     if (code.lds_words > 0) {
         uint32_t LDS *dst = lds_base() + kLdsCodeOffset;
         const int nw = code.D * code.m;
@@ -3460,7 +3462,7 @@ GRAND_INTERNAL hipError_t grand_launch_fused(const DevCode *code, const DevCall 
                               int grid, hipStream_t st) {
     const size_t lds = grand_fused_lds_bytes(hcall->cap, hcall->kind == grand::kML || hcall->lscores != nullptr) +
                        sizeof(uint32_t) * ((size_t)((code_lds_words + 3) & ~3) + (size_t)hcall->claim_batch * hcall->stage_words +
-                                           (hcall->pair_cap > 0 ? (size_t)grand::kPairBitmapWords + grand::kPairHeadWords +
+                                           (hcall->pair_cap > 0 ? (size_t)grand::kPairBitmapWords + grand::pair_head_words(hcall->kind == grand::kML) +
                                                                       2 * (size_t)hcall->pair_cap
                                                                 : 0));
     if (hcall->kind == grand::kML)

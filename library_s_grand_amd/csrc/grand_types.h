@@ -73,10 +73,16 @@ constexpr int kTaskSlots = 128;
 constexpr int kCodeLdsMaxWords = 4096;   // codes up to 16 KB of staged words (DevCode::lds_words) go to LDS
 constexpr int kMaxBatch = 8;             // most shots per worklist claim
 // Pair table of the search kernel (first-hit classes of weight >= 3)
-constexpr int kPairBuckets = 2048;       // buckets, by signature bits 16..26
+// Bucket heads by signature bits 16..: 2048 for the first-hit kinds (lists of up to 4095
+// pairs, short chains), 1024 for ML, whose waves hold a quarter of a CU's LDS each, so that a
+// list of 64 columns (2016 pairs) fits with its heads.
+constexpr int kPairBucketsFirstHit = 2048;
+constexpr int kPairBucketsMl = 1024;
+constexpr int pair_buckets(bool ml) { return ml ? kPairBucketsMl : kPairBucketsFirstHit; }
 constexpr int kPairBitmapWords = 1024;   // 2^15-bit filter, by signature bits 0..14
 constexpr unsigned kPairNone = 0xFFFu;   // end of a bucket chain; also the most pairs a table holds
-constexpr int kPairHeadWords = kPairBuckets + 4;   // bucket heads, a spare head (never looked up), padding
+// bucket heads, a spare head (never looked up), padding
+constexpr int pair_head_words(bool ml) { return pair_buckets(ml) + 4; }
 constexpr size_t kLdsPerCu = 160 * 1024; // gfx950
 // Search of a shot by its own lane in the triage kernel (first two iterations).
 // Active-list entries a lane keeps: a longer list makes every lane of its wave
